@@ -1,0 +1,840 @@
+// fsa_capi.hip -- the C ABI of fluidsim_amd.h: per-scene context, buffers,
+// uploads/downloads, and the step stages in the reference's order
+// (src/calculation.cu:23-33).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "fluidsim_amd.h"
+#include "fsa_internal.h"
+
+using namespace fsa;
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return set_err(e_ == hipErrorOutOfMemory ? FSA_ERR_NOMEM : FSA_ERR_HIP,               \
+                           std::string(#expr) + ": " + hipGetErrorString(e_));                    \
+    } while (0)
+
+}  // namespace
+
+struct fsa_sim {
+    fsa_params params{};
+    DevParams P{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+
+    KindBuffers fluid, solid, virt;
+    SolidExtra sx;
+    VirtExtra vx;
+    float4 *virt_x = nullptr;  // constant wall positions (x, y, z, 0)
+    float4 *rates = nullptr;   // fluid (ax, ay, az, drho) when debugging
+    ScanScratch scan;
+    unsigned long long *jumps = nullptr;
+
+    bool uploaded[3] = {false, false, false};
+    bool hashed = false;  // cell ids/ranks of the current order are fresh
+    unsigned debug = 0;
+
+    // stage timing: a ring of event pairs per stage, read back lazily so the
+    // host never waits on the GPU while it enqueues steps
+    static constexpr int kRing = 256;
+    bool timing = false;
+    hipEvent_t ev_begin[FSA_STAGE_COUNT][kRing] = {};
+    hipEvent_t ev_end[FSA_STAGE_COUNT][kRing] = {};
+    int ev_head[FSA_STAGE_COUNT] = {};     // next ring slot
+    int ev_pending[FSA_STAGE_COUNT] = {};  // recorded, not yet read back
+    bool ev_open[FSA_STAGE_COUNT] = {};
+    double stage_ms[FSA_STAGE_COUNT] = {};
+    int64_t stage_n[FSA_STAGE_COUNT] = {};
+
+    std::vector<void *> allocations;
+    int64_t bytes = 0;
+
+    template <class T> int alloc(T **p, size_t count) {
+        *p = nullptr;
+        if (count == 0) return FSA_OK;
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, count * sizeof(T));
+        if (e != hipSuccess)
+            return set_err(FSA_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                                              " bytes: " + hipGetErrorString(e));
+        allocations.push_back(q);
+        bytes += (int64_t)(count * sizeof(T));
+        *p = static_cast<T *>(q);
+        return FSA_OK;
+    }
+};
+
+namespace fsa {
+DevParams make_dev_params(const fsa_params &p) {
+    DevParams d{};
+    d.nf = p.n_fluid;
+    d.ns = p.n_solid;
+    d.nv = p.n_virt;
+    d.gx = p.gravity[0];
+    d.gy = p.gravity[1];
+    d.gz = p.gravity[2];
+    d.cs = p.fluid_sound_speed;
+    d.rho0 = p.fluid_reference_density;
+    d.rho0_inv = p.fluid_reference_density_inverse;
+    d.gamma = p.fluid_gamma;
+    d.visc = p.fluid_viscosity;
+    d.rho2p = p.fluid_coefficient_rho2p;
+    d.p_min = p.fluid_min_pressure;
+    d.rho_min = p.fluid_min_density;
+    d.rho_max = p.fluid_max_density;
+    d.viscid = p.fluid_is_viscid ? 1 : 0;
+    d.rho_s = p.solid_reference_density;
+    d.lambda = p.solid_lambda;
+    d.mu = p.solid_nu;
+    d.cs_s = p.solid_sound_speed;
+    d.alpha = p.solid_artificial_viscosity[0];
+    d.beta = p.solid_artificial_viscosity[1];
+    for (int a = 0; a < 3; ++a) {
+        d.dmin[a] = p.domain_min[a];
+        d.dmax[a] = p.domain_max[a];
+        d.cmin[a] = p.cell_min[a];
+        d.inv[a] = p.interval_inv[a];
+        d.cn[a] = p.cell_number[a];
+    }
+    d.ncell = p.cell_number_total;
+    d.dt = p.dt;
+    d.sub = p.solid_sub_step;
+    // solidUpdate, src/solid.cu:219
+    d.dt_s = p.solid_sub_step > 0 ? p.dt / (float)p.solid_sub_step : p.dt;
+    d.h = p.smoothing_length;
+    d.h2 = p.smoothing_length_square;
+    d.h_inv2 = p.smoothing_length_inverse_square;
+    d.R2 = p.impact_distance_square;
+    d.c1 = p.kernel_coefficient_1;
+    d.c2 = p.kernel_coefficient_2;
+    d.kd = p.kernel_differential;
+    return d;
+}
+}  // namespace fsa
+
+namespace {
+
+int check_params(const fsa_params &p) {
+    if (p.n_fluid < 0 || p.n_solid < 0 || p.n_virt < 0) return set_err(FSA_ERR_INVALID, "negative particle count");
+    for (int a = 0; a < 3; ++a)
+        if (p.cell_number[a] <= 0) return set_err(FSA_ERR_INVALID, "cell_number must be positive on every axis");
+    const int64_t total = (int64_t)p.cell_number[0] * p.cell_number[1] * p.cell_number[2];
+    if (total != p.cell_number_total) return set_err(FSA_ERR_INVALID, "cell_number_total != product of cell_number");
+    if (total >= (int64_t)INT32_MAX - 2 * kScanTile) return set_err(FSA_ERR_INVALID, "cell grid too large for int32 ids");
+    if (!(p.dt > 0.0f)) return set_err(FSA_ERR_INVALID, "dt must be positive");
+    if (p.solid_sub_step < 1) return set_err(FSA_ERR_INVALID, "solid_sub_step must be >= 1");
+    if (!(p.impact_distance_square > 0.0f)) return set_err(FSA_ERR_INVALID, "impact_distance_square must be positive");
+    // the 27-cell search needs cells at least one cut-off wide
+    for (int a = 0; a < 3; ++a) {
+        const float w = p.interval[a];
+        if (!(w * w * 1.0001f >= p.impact_distance_square))
+            return set_err(FSA_ERR_INVALID, "cell interval smaller than the kernel cut-off");
+    }
+    return FSA_OK;
+}
+
+int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_state) {
+    k.n = n;
+    const int ncell = s->P.ncell;
+    const size_t padded = (size_t)(ncell + kScanTile - 1) / kScanTile * kScanTile;
+    int rc;
+    if ((rc = s->alloc(&k.start, (size_t)ncell + 1))) return rc;
+    if ((rc = s->alloc(&k.count, padded))) return rc;
+    if (n == 0) return FSA_OK;
+    for (int b = 0; b < 2; ++b) {
+        if (b == 0 || double_state) {
+            if ((rc = s->alloc(&k.pos[b], n))) return rc;
+            if ((rc = s->alloc(&k.vel[b], n))) return rc;
+        }
+        if ((rc = s->alloc(&k.id[b], n))) return rc;
+        if ((rc = s->alloc(&k.cell[b], n))) return rc;
+        if (with_ref && (rc = s->alloc(&k.ref[b], n))) return rc;
+    }
+    if ((rc = s->alloc(&k.key, n))) return rc;
+    if ((rc = s->alloc(&k.rank, n))) return rc;
+    if ((rc = s->alloc(&k.tmp_slot, n))) return rc;
+    if ((rc = s->alloc(&k.tmp_id, n))) return rc;
+    return FSA_OK;
+}
+
+// start arrays of kinds without particles are all zero (every cell empty)
+int zero_starts(fsa_sim *s, KindBuffers &k) {
+    HIP_TRY(launch_fill_i32(k.start, 0, s->P.ncell + 1, s->stream));
+    return FSA_OK;
+}
+
+// host-side grid check of uploaded positions (findPositionCell must land
+// inside the grid; the reference indexes out of bounds otherwise)
+int check_in_grid(const fsa_sim *s, const float *x, int n, const char *what) {
+    const DevParams &P = s->P;
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            const float v = x[3 * i + a];
+            if (!(v >= P.cmin[a]))
+                return set_err(FSA_ERR_INVALID, std::string(what) + " particle " + std::to_string(i) +
+                                                    " lies outside the cell grid");
+            const int c = (int)((v - P.cmin[a]) * P.inv[a]);
+            if (c < 0 || c >= P.cn[a])
+                return set_err(FSA_ERR_INVALID, std::string(what) + " particle " + std::to_string(i) +
+                                                    " lies outside the cell grid");
+        }
+    return FSA_OK;
+}
+
+// full cell build of one kind: hash + count, scan, place + rank + gather
+int build_cells(fsa_sim *s, KindBuffers &k, bool has_ref, bool count_jumps) {
+    if (k.n == 0) return FSA_OK;
+    HIP_TRY(launch_hash_count(s->P, k, s->jumps, count_jumps, s->stream));
+    HIP_TRY(launch_scan(k.count, k.start, s->P.ncell, s->scan, s->stream));
+    HIP_TRY(launch_sort_gather(s->P, k, has_ref, s->stream));
+    return FSA_OK;
+}
+
+template <class T> int download_sorted(fsa_sim *s, const T *dev, int n, std::vector<T> &host) {
+    host.resize(n);
+    if (n == 0) return FSA_OK;
+    HIP_TRY(hipMemcpyAsync(host.data(), dev, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost, s->stream));
+    return FSA_OK;
+}
+
+int sorted_ids(fsa_sim *s, const KindBuffers &k, std::vector<int32_t> &ids) {
+    int rc = download_sorted(s, k.id[k.ord], k.n, ids);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return FSA_OK;
+}
+
+// fold every recorded sample of one stage into the totals (waits for them)
+int stage_collect(fsa_sim *s, int st) {
+    const int ring = fsa_sim::kRing;
+    while (s->ev_pending[st] > 0) {
+        const int slot = (s->ev_head[st] - s->ev_pending[st] + ring) % ring;
+        HIP_TRY(hipEventSynchronize(s->ev_end[st][slot]));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev_begin[st][slot], s->ev_end[st][slot]));
+        s->stage_ms[st] += ms;
+        s->stage_n[st] += 1;
+        s->ev_pending[st] -= 1;
+    }
+    return FSA_OK;
+}
+int stage_open(fsa_sim *s, int st) {
+    if (!s->timing) return FSA_OK;
+    if (s->ev_pending[st] >= fsa_sim::kRing) {
+        int rc = stage_collect(s, st);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(s->ev_begin[st][s->ev_head[st]], s->stream));
+    s->ev_open[st] = true;
+    return FSA_OK;
+}
+void stage_end(fsa_sim *s, int st) {
+    if (!s->timing || !s->ev_open[st]) return;
+    if (hipEventRecord(s->ev_end[st][s->ev_head[st]], s->stream) != hipSuccess) return;
+    s->ev_head[st] = (s->ev_head[st] + 1) % fsa_sim::kRing;
+    s->ev_pending[st] += 1;
+    s->ev_open[st] = false;
+}
+
+#define CHECK_SIM(s)                                                                   \
+    do {                                                                               \
+        if (!(s)) return set_err(FSA_ERR_INVALID, "null context");                     \
+    } while (0)
+
+int require_uploaded(fsa_sim *s) {
+    const int n[3] = {s->P.nf, s->P.ns, s->P.nv};
+    for (int k = 0; k < 3; ++k)
+        if (n[k] > 0 && !s->uploaded[k])
+            return set_err(FSA_ERR_STATE, "stage called before every particle kind was uploaded");
+    return FSA_OK;
+}
+
+SolidArgs solid_args(fsa_sim *s) {
+    SolidArgs a{};
+    a.spos = s->solid.pos[s->solid.st];
+    a.svel = s->solid.vel[s->solid.st];
+    a.sref = s->solid.ref[s->solid.ord];
+    a.scell = s->solid.cell[s->solid.ord];
+    a.fstart = s->fluid.start;
+    a.sstart = s->solid.start;
+    a.fpos = s->fluid.pos[s->fluid.st];
+    a.fvel = s->fluid.vel[s->fluid.st];
+    a.x = s->sx;
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsa_abi_version(void) { return FSA_ABI_VERSION; }
+
+const char *fsa_last_error(void) { return g_err.c_str(); }
+
+int fsa_device_count(int *count) {
+    if (!count) return set_err(FSA_ERR_INVALID, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    int good = 0;
+    for (int d = 0; d < n; ++d) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d) == hipSuccess && std::strncmp(prop.gcnArchName, "gfx950", 6) == 0)
+            ++good;
+    }
+    *count = good;
+    return FSA_OK;
+}
+
+int fsa_create(const fsa_params *params, void *stream, fsa_sim **out) {
+    if (!params || !out) return set_err(FSA_ERR_INVALID, "null argument");
+    *out = nullptr;
+    int rc = check_params(*params);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_err(FSA_ERR_NODEVICE, "no HIP device visible");
+    fsa_sim *s = new (std::nothrow) fsa_sim;
+    if (!s) return set_err(FSA_ERR_NOMEM, "host allocation failed");
+    s->params = *params;
+    s->P = make_dev_params(*params);
+    hipGetDevice(&s->device);
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, s->device) != hipSuccess ||
+            std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+            delete s;
+            return set_err(FSA_ERR_NODEVICE, "current device is not gfx950");
+        }
+    }
+    if (stream) {
+        s->stream = static_cast<hipStream_t>(stream);
+    } else {
+        if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete s;
+            return set_err(FSA_ERR_HIP, "hipStreamCreate failed");
+        }
+        s->own_stream = true;
+    }
+    auto fail = [&](int code) {
+        fsa_destroy(s);
+        return code;
+    };
+    if ((rc = alloc_kind(s, s->fluid, s->P.nf, false, true))) return fail(rc);
+    if ((rc = alloc_kind(s, s->solid, s->P.ns, true, true))) return fail(rc);
+    if ((rc = alloc_kind(s, s->virt, s->P.nv, false, false))) return fail(rc);
+    const int ns = s->P.ns;
+    if ((rc = s->alloc(&s->sx.np, ns)) || (rc = s->alloc(&s->sx.acc, ns)) || (rc = s->alloc(&s->sx.couple, ns)) ||
+        (rc = s->alloc(&s->sx.F, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.P, 9 * (size_t)ns)) ||
+        (rc = s->alloc(&s->sx.L, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.PL, 9 * (size_t)ns)) ||
+        (rc = s->alloc(&s->sx.A, 9 * (size_t)ns)))
+        return fail(rc);
+    if ((rc = s->alloc(&s->vx.nrm, s->P.nv)) || (rc = s->alloc(&s->virt_x, s->P.nv))) return fail(rc);
+    s->scan.ntiles_cap = (s->P.ncell + kScanTile - 1) / kScanTile;
+    if ((rc = s->alloc(&s->scan.tile_sums, (size_t)s->scan.ntiles_cap))) return fail(rc);
+    if ((rc = s->alloc(&s->jumps, 1))) return fail(rc);
+    if (hipMemsetAsync(s->jumps, 0, sizeof(unsigned long long), s->stream) != hipSuccess)
+        return fail(set_err(FSA_ERR_HIP, "hipMemsetAsync failed"));
+    if ((rc = zero_starts(s, s->fluid)) || (rc = zero_starts(s, s->solid)) || (rc = zero_starts(s, s->virt)))
+        return fail(rc);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(set_err(FSA_ERR_HIP, "stream sync failed"));
+    *out = s;
+    return FSA_OK;
+}
+
+int fsa_destroy(fsa_sim *s) {
+    if (!s) return FSA_OK;
+    if (s->stream) hipStreamSynchronize(s->stream);
+    for (void *p : s->allocations) hipFree(p);
+    for (int st = 0; st < FSA_STAGE_COUNT; ++st)
+        for (int r = 0; r < fsa_sim::kRing; ++r) {
+            if (s->ev_begin[st][r]) hipEventDestroy(s->ev_begin[st][r]);
+            if (s->ev_end[st][r]) hipEventDestroy(s->ev_end[st][r]);
+        }
+    if (s->own_stream && s->stream) hipStreamDestroy(s->stream);
+    delete s;
+    return FSA_OK;
+}
+
+int fsa_upload_fluid(fsa_sim *s, const float *x, const float *u, const float *rho, const float *p) {
+    CHECK_SIM(s);
+    const int n = s->P.nf;
+    if (n > 0 && (!x || !u || !rho || !p)) return set_err(FSA_ERR_INVALID, "null fluid array");
+    int rc = check_in_grid(s, x, n, "fluid");
+    if (rc) return rc;
+    std::vector<float4> pos(n), vel(n);
+    std::vector<int32_t> id(n);
+    for (int i = 0; i < n; ++i) {
+        pos[i] = make_float4(x[3 * i], x[3 * i + 1], x[3 * i + 2], rho[i]);
+        vel[i] = make_float4(u[3 * i], u[3 * i + 1], u[3 * i + 2], p[i]);
+        id[i] = i;
+    }
+    KindBuffers &k = s->fluid;
+    if (n > 0) {
+        k.st = 0;
+        k.ord = 0;
+        HIP_TRY(hipMemcpyAsync(k.pos[0], pos.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(k.vel[0], vel.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(k.id[0], id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s->stream));
+        if ((rc = build_cells(s, k, false, false))) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->uploaded[FSA_FLUID] = true;
+    return FSA_OK;
+}
+
+int fsa_upload_solid(fsa_sim *s, const float *x, const float *u, const float *X, const int32_t *type) {
+    CHECK_SIM(s);
+    const int n = s->P.ns;
+    if (n > 0 && (!x || !u || !X || !type)) return set_err(FSA_ERR_INVALID, "null solid array");
+    int rc = check_in_grid(s, x, n, "solid");
+    if (rc) return rc;
+    std::vector<float4> pos(n), vel(n), ref(n);
+    std::vector<int32_t> id(n);
+    for (int i = 0; i < n; ++i) {
+        if (type[i] != FSA_UNCONSTRAINED_SOLID && type[i] != FSA_FIXED_SOLID)
+            return set_err(FSA_ERR_INVALID, "solid type must be 0 (unconstrained) or 1 (fixed)");
+        pos[i] = make_float4(x[3 * i], x[3 * i + 1], x[3 * i + 2], 0.0f);
+        vel[i] = make_float4(u[3 * i], u[3 * i + 1], u[3 * i + 2], 0.0f);
+        float tbits;
+        std::memcpy(&tbits, &type[i], sizeof tbits);
+        ref[i] = make_float4(X[3 * i], X[3 * i + 1], X[3 * i + 2], tbits);
+        id[i] = i;
+    }
+    KindBuffers &k = s->solid;
+    if (n > 0) {
+        k.st = 0;
+        k.ord = 0;
+        HIP_TRY(hipMemcpyAsync(k.pos[0], pos.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(k.vel[0], vel.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(k.ref[0], ref.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(k.id[0], id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s->stream));
+        if ((rc = build_cells(s, k, true, false))) return rc;
+        // Solid_Host_List::addList defaults (include/host_data.cuh:92-99):
+        // F = I, P = 0, L = I, n = 0, p = 0, du_dt = 0
+        std::vector<float> eye(9 * (size_t)n, 0.0f), zero9(9 * (size_t)n, 0.0f);
+        for (int e : {0, 4, 8})
+            for (int i = 0; i < n; ++i) eye[(size_t)e * n + i] = 1.0f;
+        std::vector<float4> z4(n, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        HIP_TRY(hipMemcpyAsync(s->sx.F, eye.data(), 36 * (size_t)n, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->sx.L, eye.data(), 36 * (size_t)n, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->sx.P, zero9.data(), 36 * (size_t)n, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->sx.np, z4.data(), 16 * (size_t)n, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->sx.acc, z4.data(), 16 * (size_t)n, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->sx.couple, z4.data(), 16 * (size_t)n, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->uploaded[FSA_SOLID] = true;
+    return FSA_OK;
+}
+
+int fsa_upload_virt(fsa_sim *s, const float *x, const int32_t *type) {
+    CHECK_SIM(s);
+    const int n = s->P.nv;
+    if (n > 0 && (!x || !type)) return set_err(FSA_ERR_INVALID, "null virt array");
+    int rc = check_in_grid(s, x, n, "virtual");
+    if (rc) return rc;
+    KindBuffers &k = s->virt;
+    if (n > 0) {
+        // Virt_Host_List::addList defaults: u = 0, p = 0, rho = rho0
+        std::vector<float4> pos(n), vel(n, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        std::vector<int32_t> id(n);
+        for (int i = 0; i < n; ++i) {
+            pos[i] = make_float4(x[3 * i], x[3 * i + 1], x[3 * i + 2], s->params.fluid_reference_density);
+            id[i] = i;
+        }
+        // walls are static: sort once, with pos/vel scratch in the second
+        // buffers of the fluid-sized temporaries
+        float4 *pos_tmp = nullptr, *vel_tmp = nullptr;
+        HIP_TRY(hipMalloc(&pos_tmp, sizeof(float4) * n));
+        if (hipMalloc(&vel_tmp, sizeof(float4) * n) != hipSuccess) {
+            hipFree(pos_tmp);
+            return set_err(FSA_ERR_NOMEM, "hipMalloc failed for wall scratch");
+        }
+        k.pos[1] = pos_tmp;
+        k.vel[1] = vel_tmp;
+        k.st = 1;
+        k.ord = 0;
+        int err = FSA_OK;
+        if (hipMemcpyAsync(pos_tmp, pos.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+            hipMemcpyAsync(vel_tmp, vel.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+            hipMemcpyAsync(k.id[0], id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+            err = set_err(FSA_ERR_HIP, "wall upload copy failed");
+        if (!err) err = build_cells(s, k, false, false);  // gathers into pos[0]/vel[0]
+        if (!err && hipMemcpyAsync(s->virt_x, k.pos[0], sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream) !=
+                        hipSuccess)
+            err = set_err(FSA_ERR_HIP, "wall position copy failed");
+        hipStreamSynchronize(s->stream);
+        (void)hipFree(pos_tmp);
+        (void)hipFree(vel_tmp);
+        k.pos[1] = nullptr;
+        k.vel[1] = nullptr;
+        if (err) return err;
+        (void)type;  // FIXED_VIRT only; MOVING_VIRT is declared but unused by the reference
+        s->vx.normals_ready = false;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->uploaded[FSA_VIRT] = true;
+    return FSA_OK;
+}
+
+int fsa_download_fluid(fsa_sim *s, float *x, float *u, float *rho, float *p) {
+    CHECK_SIM(s);
+    const KindBuffers &k = s->fluid;
+    std::vector<int32_t> ids;
+    std::vector<float4> pos, vel;
+    int rc;
+    if ((rc = download_sorted(s, k.pos[k.st], k.n, pos)) || (rc = download_sorted(s, k.vel[k.st], k.n, vel)) ||
+        (rc = sorted_ids(s, k, ids)))
+        return rc;
+    for (int t = 0; t < k.n; ++t) {
+        const int i = ids[t];
+        if (x) { x[3 * i] = pos[t].x; x[3 * i + 1] = pos[t].y; x[3 * i + 2] = pos[t].z; }
+        if (u) { u[3 * i] = vel[t].x; u[3 * i + 1] = vel[t].y; u[3 * i + 2] = vel[t].z; }
+        if (rho) rho[i] = pos[t].w;
+        if (p) p[i] = vel[t].w;
+    }
+    return FSA_OK;
+}
+
+int fsa_download_fluid_rates(fsa_sim *s, float *drho, float *a) {
+    CHECK_SIM(s);
+    if (!s->rates) return set_err(FSA_ERR_STATE, "pair sums are recorded only with FSA_DEBUG_PAIR_SUMS");
+    const KindBuffers &k = s->fluid;
+    std::vector<int32_t> ids;
+    std::vector<float4> r;
+    int rc;
+    if ((rc = download_sorted(s, (const float4 *)s->rates, k.n, r)) || (rc = sorted_ids(s, k, ids))) return rc;
+    for (int t = 0; t < k.n; ++t) {
+        const int i = ids[t];
+        if (drho) drho[i] = r[t].w;
+        if (a) { a[3 * i] = r[t].x; a[3 * i + 1] = r[t].y; a[3 * i + 2] = r[t].z; }
+    }
+    return FSA_OK;
+}
+
+int fsa_download_solid(fsa_sim *s, float *x, float *u, float *p, float *nrm, float *du_dt, float *couple, float *F,
+                       float *Pk, float *L) {
+    CHECK_SIM(s);
+    const KindBuffers &k = s->solid;
+    const int n = k.n;
+    std::vector<int32_t> ids;
+    std::vector<float4> pos, vel, np, acc, cpl;
+    std::vector<float> mF, mP, mL;
+    int rc;
+    if ((rc = download_sorted(s, k.pos[k.st], n, pos)) || (rc = download_sorted(s, k.vel[k.st], n, vel)) ||
+        (rc = download_sorted(s, (const float4 *)s->sx.np, n, np)) ||
+        (rc = download_sorted(s, (const float4 *)s->sx.acc, n, acc)) ||
+        (rc = download_sorted(s, (const float4 *)s->sx.couple, n, cpl)) ||
+        (rc = download_sorted(s, (const float *)s->sx.F, 9 * n, mF)) ||
+        (rc = download_sorted(s, (const float *)s->sx.P, 9 * n, mP)) ||
+        (rc = download_sorted(s, (const float *)s->sx.L, 9 * n, mL)) || (rc = sorted_ids(s, k, ids)))
+        return rc;
+    auto put3 = [](float *o, int i, const float4 &v) {
+        if (o) { o[3 * i] = v.x; o[3 * i + 1] = v.y; o[3 * i + 2] = v.z; }
+    };
+    // SoA row-major planes -> glm column-major 9 floats per particle
+    auto put9 = [n](float *o, int i, int t, const std::vector<float> &m) {
+        if (!o) return;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) o[9 * i + 3 * c + r] = m[(size_t)(3 * r + c) * n + t];
+    };
+    for (int t = 0; t < n; ++t) {
+        const int i = ids[t];
+        put3(x, i, pos[t]);
+        put3(u, i, vel[t]);
+        if (p) p[i] = np[t].w;
+        put3(nrm, i, np[t]);
+        put3(du_dt, i, acc[t]);
+        put3(couple, i, cpl[t]);
+        put9(F, i, t, mF);
+        put9(Pk, i, t, mP);
+        put9(L, i, t, mL);
+    }
+    return FSA_OK;
+}
+
+int fsa_download_virt(fsa_sim *s, float *x, float *nrm, float *u, float *rho, float *p) {
+    CHECK_SIM(s);
+    const KindBuffers &k = s->virt;
+    const int n = k.n;
+    std::vector<int32_t> ids;
+    std::vector<float4> pos, vel, nn;
+    int rc;
+    if ((rc = download_sorted(s, k.pos[0], n, pos)) || (rc = download_sorted(s, k.vel[0], n, vel)) ||
+        (rc = download_sorted(s, (const float4 *)s->vx.nrm, n, nn)) || (rc = sorted_ids(s, k, ids)))
+        return rc;
+    for (int t = 0; t < n; ++t) {
+        const int i = ids[t];
+        if (x) { x[3 * i] = pos[t].x; x[3 * i + 1] = pos[t].y; x[3 * i + 2] = pos[t].z; }
+        if (nrm) {
+            const bool ready = s->vx.normals_ready;
+            nrm[3 * i] = ready ? nn[t].x : 0.0f;
+            nrm[3 * i + 1] = ready ? nn[t].y : 0.0f;
+            nrm[3 * i + 2] = ready ? nn[t].z : 0.0f;
+        }
+        if (u) { u[3 * i] = vel[t].x; u[3 * i + 1] = vel[t].y; u[3 * i + 2] = vel[t].z; }
+        if (rho) rho[i] = pos[t].w;
+        if (p) p[i] = vel[t].w;
+    }
+    return FSA_OK;
+}
+
+// ---- stages ----------------------------------------------------------------
+
+int fsa_particle_cell_update(fsa_sim *s) {
+    CHECK_SIM(s);
+    int rc = require_uploaded(s);
+    if (rc) return rc;
+    if ((rc = stage_open(s, FSA_STAGE_CELL))) return rc;
+    // wall cell ids never change (the walls do not move): only fluid and solid
+    if (s->fluid.n) HIP_TRY(launch_hash_count(s->P, s->fluid, s->jumps, true, s->stream));
+    if (s->solid.n) HIP_TRY(launch_hash_count(s->P, s->solid, s->jumps, true, s->stream));
+    s->hashed = true;
+    return FSA_OK;
+}
+
+int fsa_fill_cell(fsa_sim *s) {
+    CHECK_SIM(s);
+    int rc = require_uploaded(s);
+    if (rc) return rc;
+    // the sort consumes the ranks of the last hash, which describe the
+    // current order only until the next sort
+    if (!s->hashed) return set_err(FSA_ERR_STATE, "fill_cell needs particle_cell_update first");
+    s->hashed = false;
+    if (s->fluid.n) {
+        HIP_TRY(launch_scan(s->fluid.count, s->fluid.start, s->P.ncell, s->scan, s->stream));
+        HIP_TRY(launch_sort_gather(s->P, s->fluid, false, s->stream));
+    }
+    if (s->solid.n) {
+        HIP_TRY(launch_scan(s->solid.count, s->solid.start, s->P.ncell, s->scan, s->stream));
+        HIP_TRY(launch_sort_gather(s->P, s->solid, true, s->stream));
+    }
+    stage_end(s, FSA_STAGE_CELL);
+    return FSA_OK;
+}
+
+int fsa_couple_calculation(fsa_sim *s) {
+    CHECK_SIM(s);
+    int rc = require_uploaded(s);
+    if (rc) return rc;
+    // DeviceCalculation::coupleCalculation, src/solid.cu:346-359
+    if (s->P.nf == 0 || s->P.ns == 0) return FSA_OK;
+    if ((rc = stage_open(s, FSA_STAGE_COUPLE))) return rc;
+    HIP_TRY(launch_couple(s->P, solid_args(s), s->stream));
+    stage_end(s, FSA_STAGE_COUPLE);
+    return FSA_OK;
+}
+
+int fsa_virt_update(fsa_sim *s) {
+    CHECK_SIM(s);
+    int rc = require_uploaded(s);
+    if (rc) return rc;
+    if (s->P.nv == 0) return FSA_OK;
+    if ((rc = stage_open(s, FSA_STAGE_VIRT))) return rc;
+    VirtArgs a{};
+    a.vpos_in = s->virt_x;
+    a.vpos = s->virt.pos[0];
+    a.vvel = s->virt.vel[0];
+    a.vnrm = s->vx.nrm;
+    a.vcell = s->virt.cell[s->virt.ord];
+    a.fstart = s->fluid.start;
+    a.vstart = s->virt.start;
+    a.fpos = s->fluid.pos[s->fluid.st];
+    a.fvel = s->fluid.vel[s->fluid.st];
+    a.compute_normals = !s->vx.normals_ready;
+    HIP_TRY(launch_virt(s->P, a, s->stream));
+    s->vx.normals_ready = true;
+    stage_end(s, FSA_STAGE_VIRT);
+    return FSA_OK;
+}
+
+int fsa_fluid_calculation(fsa_sim *s) {
+    CHECK_SIM(s);
+    int rc = require_uploaded(s);
+    if (rc) return rc;
+    if (s->P.nf == 0) return FSA_OK;
+    if ((rc = stage_open(s, FSA_STAGE_FLUID))) return rc;
+    KindBuffers &f = s->fluid;
+    FluidStepArgs a{};
+    a.fpos = f.pos[f.st];
+    a.fvel = f.vel[f.st];
+    a.fcell = f.cell[f.ord];
+    a.fstart = f.start;
+    a.sstart = s->solid.start;
+    a.vstart = s->virt.start;
+    a.spos = s->solid.pos[s->solid.st];
+    a.svel = s->solid.vel[s->solid.st];
+    a.snp = s->sx.np;
+    a.vpos = s->virt.pos[0];
+    a.vvel = s->virt.vel[0];
+    a.fpos_out = f.pos[1 - f.st];
+    a.fvel_out = f.vel[1 - f.st];
+    a.rates = (s->debug & FSA_DEBUG_PAIR_SUMS) ? s->rates : nullptr;
+    HIP_TRY(launch_fluid(s->P, a, s->stream));
+    f.st = 1 - f.st;
+    stage_end(s, FSA_STAGE_FLUID);
+    return FSA_OK;
+}
+
+int fsa_solid_calculation(fsa_sim *s) {
+    CHECK_SIM(s);
+    int rc = require_uploaded(s);
+    if (rc) return rc;
+    if (s->P.ns == 0) return FSA_OK;
+    if ((rc = stage_open(s, FSA_STAGE_SOLID))) return rc;
+    const SolidArgs a = solid_args(s);
+    for (int sub = 0; sub < s->P.sub; ++sub) {
+        HIP_TRY(launch_solid_deformation(s->P, a, s->stream));
+        HIP_TRY(launch_solid_force(s->P, a, s->stream));
+        HIP_TRY(launch_solid_update(s->P, a, s->stream));
+    }
+    stage_end(s, FSA_STAGE_SOLID);
+    return FSA_OK;
+}
+
+int fsa_step(fsa_sim *s, int n_steps) {
+    CHECK_SIM(s);
+    if (n_steps < 0) return set_err(FSA_ERR_INVALID, "negative step count");
+    for (int i = 0; i < n_steps; ++i) {
+        int rc;
+        if ((rc = fsa_particle_cell_update(s)) || (rc = fsa_fill_cell(s)) || (rc = fsa_couple_calculation(s)) ||
+            (rc = fsa_virt_update(s)) || (rc = fsa_fluid_calculation(s)) || (rc = fsa_solid_calculation(s)))
+            return rc;
+    }
+    return FSA_OK;
+}
+
+int fsa_synchronize(fsa_sim *s) {
+    CHECK_SIM(s);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return FSA_OK;
+}
+
+// ---- introspection ---------------------------------------------------------
+
+static KindBuffers *kind_of(fsa_sim *s, int kind) {
+    switch (kind) {
+    case FSA_FLUID: return &s->fluid;
+    case FSA_SOLID: return &s->solid;
+    case FSA_VIRT: return &s->virt;
+    default: return nullptr;
+    }
+}
+
+int fsa_get_cell_ids(fsa_sim *s, int kind, int32_t *out) {
+    CHECK_SIM(s);
+    KindBuffers *k = kind_of(s, kind);
+    if (!k || !out) return set_err(FSA_ERR_INVALID, "bad kind or null output");
+    std::vector<int32_t> ids, cells;
+    int rc;
+    if ((rc = download_sorted(s, (const int32_t *)k->cell[k->ord], k->n, cells)) || (rc = sorted_ids(s, *k, ids)))
+        return rc;
+    for (int t = 0; t < k->n; ++t) out[ids[t]] = cells[t];
+    return FSA_OK;
+}
+
+int fsa_get_cell_start(fsa_sim *s, int kind, int32_t *out) {
+    CHECK_SIM(s);
+    KindBuffers *k = kind_of(s, kind);
+    if (!k || !out) return set_err(FSA_ERR_INVALID, "bad kind or null output");
+    HIP_TRY(hipMemcpyAsync(out, k->start, sizeof(int32_t) * ((size_t)s->P.ncell + 1), hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return FSA_OK;
+}
+
+int fsa_get_sorted_ids(fsa_sim *s, int kind, int32_t *out) {
+    CHECK_SIM(s);
+    KindBuffers *k = kind_of(s, kind);
+    if (!k || !out) return set_err(FSA_ERR_INVALID, "bad kind or null output");
+    std::vector<int32_t> ids;
+    int rc = sorted_ids(s, *k, ids);
+    if (rc) return rc;
+    std::copy(ids.begin(), ids.end(), out);
+    return FSA_OK;
+}
+
+int fsa_get_jump_count(fsa_sim *s, int64_t *count) {
+    CHECK_SIM(s);
+    if (!count) return set_err(FSA_ERR_INVALID, "null output");
+    unsigned long long v = 0;
+    HIP_TRY(hipMemcpyAsync(&v, s->jumps, sizeof v, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    *count = (int64_t)v;
+    return FSA_OK;
+}
+
+int fsa_set_debug(fsa_sim *s, unsigned flags) {
+    CHECK_SIM(s);
+    if ((flags & FSA_DEBUG_PAIR_SUMS) && !s->rates && s->P.nf > 0) {
+        int rc = s->alloc(&s->rates, s->P.nf);
+        if (rc) return rc;
+    }
+    s->debug = flags;
+    return FSA_OK;
+}
+
+int fsa_set_timing(fsa_sim *s, int enabled) {
+    CHECK_SIM(s);
+    for (int st = 0; st < FSA_STAGE_COUNT; ++st) {
+        int rc = stage_collect(s, st);
+        if (rc) return rc;
+        if (!enabled) continue;
+        for (int r = 0; r < fsa_sim::kRing; ++r) {
+            if (!s->ev_begin[st][r]) HIP_TRY(hipEventCreate(&s->ev_begin[st][r]));
+            if (!s->ev_end[st][r]) HIP_TRY(hipEventCreate(&s->ev_end[st][r]));
+        }
+    }
+    s->timing = enabled != 0;
+    return FSA_OK;
+}
+
+int fsa_get_stage_times(fsa_sim *s, double *ms, int64_t *samples) {
+    CHECK_SIM(s);
+    for (int st = 0; st < FSA_STAGE_COUNT; ++st) {
+        int rc = stage_collect(s, st);
+        if (rc) return rc;
+        if (ms) ms[st] = s->stage_ms[st];
+        if (samples) samples[st] = s->stage_n[st];
+    }
+    return FSA_OK;
+}
+
+int fsa_reset_stage_times(fsa_sim *s) {
+    CHECK_SIM(s);
+    for (int st = 0; st < FSA_STAGE_COUNT; ++st) {
+        int rc = stage_collect(s, st);
+        if (rc) return rc;
+        s->stage_ms[st] = 0.0;
+        s->stage_n[st] = 0;
+    }
+    return FSA_OK;
+}
+
+int fsa_memory_used(fsa_sim *s, int64_t *bytes) {
+    CHECK_SIM(s);
+    if (!bytes) return set_err(FSA_ERR_INVALID, "null output");
+    *bytes = s->bytes;
+    return FSA_OK;
+}
+
+}  // extern "C"

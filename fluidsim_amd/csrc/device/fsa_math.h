@@ -1,0 +1,59 @@
+// fsa_math.h -- device helpers shared by the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fsa_internal.h"
+
+namespace fsa {
+
+// include/global.h:6-8
+constexpr float kEpsForSum = 0.1f;
+constexpr float kEpsFloat = 1.0e-6f;
+constexpr float kLargeFloat = 1.0e6f;
+
+// squared length fma(z,z,fma(y,y,x*x)): the one form used for every
+// glm::length2 of the reference, so candidate tests are bit-reproducible
+__device__ __forceinline__ float len2(float x, float y, float z) {
+    return __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x));
+}
+
+// findPositionCell, src/pair.cu:8-24; indices are clamped into the grid
+// (in-grid positions are unaffected)
+__device__ __forceinline__ int find_cell(const DevParams &P, float x, float y, float z) {
+    int ix = (int)((x - P.cmin[0]) * P.inv[0]);
+    int iy = (int)((y - P.cmin[1]) * P.inv[1]);
+    int iz = (int)((z - P.cmin[2]) * P.inv[2]);
+    ix = min(max(ix, 0), P.cn[0] - 1);
+    iy = min(max(iy, 0), P.cn[1] - 1);
+    iz = min(max(iz, 0), P.cn[2] - 1);
+    return (ix * P.cn[1] + iy) * P.cn[2] + iz;
+}
+
+// The 27 neighbour cells of cell c as 9 columns (ox, oy ascending), each
+// the contiguous z-run cz-1..cz+1 clipped to the grid: `body(c0, c1)` gets
+// the first cell and one past the last cell of each column.
+template <class F>
+__device__ __forceinline__ void for_each_column(const DevParams &P, int c, F &&body) {
+    const int ny = P.cn[1], nz = P.cn[2];
+    const int nyz = ny * nz;
+    const int cx = c / nyz;
+    const int rem = c - cx * nyz;
+    const int cy = rem / nz;
+    const int cz = rem - cy * nz;
+    const int zlo = max(cz - 1, 0), zhi = min(cz + 1, nz - 1);
+#pragma unroll 1
+    for (int ox = -1; ox <= 1; ++ox) {
+        const int gx = cx + ox;
+        if ((unsigned)gx >= (unsigned)P.cn[0]) continue;
+#pragma unroll 1
+        for (int oy = -1; oy <= 1; ++oy) {
+            const int gy = cy + oy;
+            if ((unsigned)gy >= (unsigned)ny) continue;
+            const int base = (gx * ny + gy) * nz;
+            body(base + zlo, base + zhi + 1);
+        }
+    }
+}
+
+}  // namespace fsa

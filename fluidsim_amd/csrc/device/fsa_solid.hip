@@ -1,0 +1,318 @@
+// fsa_solid.hip -- the solid side: coupling fields (K6), total-Lagrangian
+// deformation and stress (K10), forces (K11) and the sub-step integrator
+// (K12); src/solid.cu:10-359.
+//
+// One thread per solid particle in sorted order.  Neighbours are found in
+// the cell lists of the step (built from start-of-step positions) and, for
+// solid-solid terms, filtered by reference-configuration distance, exactly
+// as the reference walks its lists (src/solid.cu:53-69, 195-203).  Matrices
+// live in 9 SoA planes; the per-particle factors of the pair force that do
+// not depend on the partner (P_i L_i and det(F_i) F_i^-T) are computed once
+// in the deformation kernel instead of once per pair.
+#include "fsa_internal.h"
+#include "fsa_math.h"
+
+namespace fsa {
+
+namespace {
+
+struct M3 {
+    float m[3][3];  // row-major: m[r][c]
+};
+
+__device__ __forceinline__ M3 mzero() {
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r.m[i][j] = 0.0f;
+    return r;
+}
+__device__ __forceinline__ M3 mident() {
+    M3 r = mzero();
+    r.m[0][0] = r.m[1][1] = r.m[2][2] = 1.0f;
+    return r;
+}
+__device__ __forceinline__ M3 mmul(const M3 &a, const M3 &b) {
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r.m[i][j] = a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j] + a.m[i][2] * b.m[2][j];
+    return r;
+}
+__device__ __forceinline__ float mdet(const M3 &a) {
+    const auto &m = a.m;
+    return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+           m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+}
+// adjugate / determinant
+__device__ __forceinline__ M3 minverse(const M3 &a) {
+    const auto &m = a.m;
+    const float c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+    const float c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+    const float c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+    const float inv_det = 1.0f / (m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02);
+    M3 r;
+    r.m[0][0] = c00 * inv_det;
+    r.m[1][0] = c01 * inv_det;
+    r.m[2][0] = c02 * inv_det;
+    r.m[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * inv_det;
+    r.m[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * inv_det;
+    r.m[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * inv_det;
+    r.m[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * inv_det;
+    r.m[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * inv_det;
+    r.m[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * inv_det;
+    return r;
+}
+__device__ __forceinline__ M3 load9(const float *__restrict__ base, int n, int i) {
+    M3 r;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) r.m[e / 3][e % 3] = base[(size_t)e * n + i];
+    return r;
+}
+__device__ __forceinline__ void store9(float *__restrict__ base, int n, int i, const M3 &a) {
+#pragma unroll
+    for (int e = 0; e < 9; ++e) base[(size_t)e * n + i] = a.m[e / 3][e % 3];
+}
+
+// ---- K6: solidCoupleUpdate (src/solid.cu:262-344) ------------------------
+__global__ __launch_bounds__(kBlock) void k_couple(DevParams P, SolidArgs a) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= P.ns) return;
+    const float4 xi4 = a.spos[i];
+    const float xi = xi4.x, yi = xi4.y, zi = xi4.z;
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f, pressure = 0.0f, wsum = 0.0f;
+    for_each_column(P, a.scell[i], [&](int c0, int c1) {
+        // Shepard pressure from fluid (solidPressureCalculate, solid.cu:275-291)
+        for (int j = a.fstart[c0], e = a.fstart[c1]; j < e; ++j) {
+            const float4 pj = a.fpos[j];
+            const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+            const float d2 = len2(dx, dy, dz);
+            if (d2 < P.R2) {
+                const float w = (expf(-(d2 * P.h_inv2)) - P.c1) * P.c2;
+                wsum += w;
+                pressure += a.fvel[j].w * w;
+            }
+        }
+        // normal from solid neighbours (solidNormalCalculate, solid.cu:262-273)
+        for (int j = a.sstart[c0], e = a.sstart[c1]; j < e; ++j) {
+            const float4 pj = a.spos[j];
+            const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+            const float d2 = len2(dx, dy, dz);
+            if (d2 < P.R2) {
+                const float kc = expf(-(d2 * P.h_inv2)) * P.kd;
+                nx -= dx * kc;
+                ny -= dy * kc;
+                nz -= dz * kc;
+            }
+        }
+    });
+    const float n2 = len2(nx, ny, nz);
+    if (n2 < kEpsForSum) {
+        nx = ny = nz = 0.0f;
+    } else {
+        const float r = 1.0f / sqrtf(n2);
+        nx *= r;
+        ny *= r;
+        nz *= r;
+    }
+    const float p = wsum > kEpsFloat ? fmaxf(pressure / wsum, P.p_min) : 0.0f;
+    a.x.np[i] = make_float4(nx, ny, nz, p);
+}
+
+// ---- K10: solidDeformationCalculate (src/solid.cu:10-79) -----------------
+__global__ __launch_bounds__(kBlock) void k_deformation(DevParams P, SolidArgs a) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= P.ns) return;
+    const float4 Xi = a.sref[i];
+    const float4 xi = a.spos[i];
+    M3 F = mzero(), B = mzero();
+    for_each_column(P, a.scell[i], [&](int c0, int c1) {
+        for (int j = a.sstart[c0], e = a.sstart[c1]; j < e; ++j) {
+            const float4 Xj = a.sref[j];
+            const float dX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
+            const float d2 = len2(dX[0], dX[1], dX[2]);
+            if (d2 < P.R2) {
+                const float4 xj = a.spos[j];
+                const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+                const float kc = expf(-(d2 * P.h_inv2)) * P.kd;
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        F.m[r][c] += (dx[r] * dX[c]) * kc;
+                        B.m[r][c] += (dX[r] * dX[c]) * kc;
+                    }
+            }
+        }
+    });
+    const M3 L = mdet(B) != 0.0f ? minverse(B) : mident();
+    F = mmul(F, L);
+    // St Venant-Kirchhoff (solidStressCalculation, solid.cu:27-36)
+    M3 E;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float ftf = F.m[0][r] * F.m[0][c] + F.m[1][r] * F.m[1][c] + F.m[2][r] * F.m[2][c];
+            E.m[r][c] = 0.5f * (ftf - (r == c ? 1.0f : 0.0f));
+        }
+    const float tr = E.m[0][0] + E.m[1][1] + E.m[2][2];
+    M3 S;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) S.m[r][c] = 2.0f * P.mu * E.m[r][c] + (r == c ? P.lambda * tr : 0.0f);
+    const M3 Pk = mmul(F, S);
+    const int n = P.ns;
+    store9(a.x.F, n, i, F);
+    store9(a.x.P, n, i, Pk);
+    store9(a.x.L, n, i, L);
+    store9(a.x.PL, n, i, mmul(Pk, L));
+    // det(F) F^-T, the factor of the artificial-viscosity force
+    const M3 Fi = minverse(F);
+    const float dF = mdet(F);
+    M3 A;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) A.m[r][c] = dF * Fi.m[c][r];
+    store9(a.x.A, n, i, A);
+}
+
+// ---- K11: solidForceCalculate (src/solid.cu:81-207) ----------------------
+__global__ __launch_bounds__(kBlock) void k_force(DevParams P, SolidArgs a) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= P.ns) return;
+    const int n = P.ns;
+    const float4 xi = a.spos[i], ui = a.svel[i], Xi = a.sref[i];
+    const float4 np = a.x.np[i];
+    const M3 PLi = load9(a.x.PL, n, i);
+    const M3 Li = load9(a.x.L, n, i);
+    const M3 Ai = load9(a.x.A, n, i);
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;  // couple_dudt
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    const float eps = 0.01f * P.h2;
+    for_each_column(P, a.scell[i], [&](int c0, int c1) {
+        // fluid pressure on the solid (solidFluidCouple, solid.cu:81-117):
+        // the solid side takes the fluid velocity mirrored about its normal
+        for (int j = a.fstart[c0], e = a.fstart[c1]; j < e; ++j) {
+            const float4 pj = a.fpos[j];
+            const float dx = pj.x - xi.x, dy = pj.y - xi.y, dz = pj.z - xi.z;
+            const float d2 = len2(dx, dy, dz);
+            if (d2 < P.R2) {
+                const float4 uj = a.fvel[j];
+                const float m = 2.0f * ((uj.x - ui.x) * np.x + (uj.y - ui.y) * np.y + (uj.z - ui.z) * np.z);
+                const float uix = uj.x - m * np.x, uiy = uj.y - m * np.y, uiz = uj.z - m * np.z;
+                const float kc = expf(-(d2 * P.h_inv2)) * P.kd;
+                const float rhoj = pj.w, rhoi = rhoj;
+                float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+                if (sqrtf(d2) != 0.0f) {
+                    const float r = 1.0f / sqrtf(d2);
+                    ex = dx * r;
+                    ey = dy * r;
+                    ez = dz * r;
+                }
+                float ps = (rhoi * uj.w + rhoj * np.w) / (rhoi + rhoj);
+                const float du = (uix - uj.x) * ex + (uiy - uj.y) * ey + (uiz - uj.z) * ez;
+                if (du >= 0.0f) ps += rhoi * rhoj * du / (rhoi + rhoj) * P.cs;
+                const float s = -2.0f * ps / P.rho_s;
+                cx += (s * dx) * kc;
+                cy += (s * dy) * kc;
+                cz += (s * dz) * kc;
+            }
+        }
+        // stress divergence + artificial viscosity (solidSolidForce, solid.cu:119-157)
+        for (int j = a.sstart[c0], e = a.sstart[c1]; j < e; ++j) {
+            const float4 Xj = a.sref[j];
+            const float dX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
+            const float d2 = len2(dX[0], dX[1], dX[2]);
+            if (d2 < P.R2) {
+                const float kc = expf(-(d2 * P.h_inv2)) * P.kd;
+                float v[3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    float t = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) t += (a.x.PL[(size_t)(3 * r + c) * n + j] + PLi.m[r][c]) * dX[c];
+                    v[r] = t * kc / P.rho_s;
+                }
+                ax += v[0];
+                ay += v[1];
+                az += v[2];
+                const float4 xj = a.spos[j], uj = a.svel[j];
+                const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                const float dudx = (uj.x - ui.x) * dx + (uj.y - ui.y) * dy + (uj.z - ui.z) * dz;
+                const float nu = P.h * dudx / (len2(dx, dy, dz) + eps);
+                float visc = P.alpha * nu * P.cs_s;
+                visc -= P.beta * nu * nu;
+                // ((((det F F^-T) visc) L_i) dX) k
+                M3 AV;
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) AV.m[r][c] = Ai.m[r][c] * visc;
+                const M3 AL = mmul(AV, Li);
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const float t = AL.m[r][0] * dX[0] + AL.m[r][1] * dX[1] + AL.m[r][2] * dX[2];
+                    v[r] = t * kc;
+                }
+                ax += v[0];
+                ay += v[1];
+                az += v[2];
+            }
+        }
+    });
+    a.x.couple[i] = make_float4(cx, cy, cz, 0.0f);
+    a.x.acc[i] = make_float4(ax + cx, ay + cy, az + cz, 0.0f);
+}
+
+// ---- K12: solidUpdate (src/solid.cu:209-239) -----------------------------
+__global__ __launch_bounds__(kBlock) void k_solid_update(DevParams P, SolidArgs a) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= P.ns) return;
+    const int type = __float_as_int(a.sref[i].w);
+    const float dt = P.dt_s;
+    if (type == FSA_FIXED_SOLID) {
+        a.svel[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        a.x.acc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else if (type == FSA_UNCONSTRAINED_SOLID) {
+        const float4 x = a.spos[i], u = a.svel[i], ac = a.x.acc[i];
+        float px = x.x + u.x * dt, py = x.y + u.y * dt, pz = x.z + u.z * dt;
+        const float4 un = make_float4(u.x + (ac.x + P.gx) * dt, u.y + (ac.y + P.gy) * dt, u.z + (ac.z + P.gz) * dt, 0.0f);
+        px = fminf(fmaxf(px, P.dmin[0]), P.dmax[0]);
+        py = fminf(fmaxf(py, P.dmin[1]), P.dmax[1]);
+        pz = fminf(fmaxf(pz, P.dmin[2]), P.dmax[2]);
+        a.spos[i] = make_float4(px, py, pz, 0.0f);
+        a.svel[i] = un;
+    }
+}
+
+inline dim3 grid_for(int n) { return dim3((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+hipError_t launch_couple(const DevParams &P, const SolidArgs &a, hipStream_t s) {
+    if (P.ns == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_couple, grid_for(P.ns), dim3(kBlock), 0, s, P, a);
+    return hipGetLastError();
+}
+hipError_t launch_solid_deformation(const DevParams &P, const SolidArgs &a, hipStream_t s) {
+    if (P.ns == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_deformation, grid_for(P.ns), dim3(kBlock), 0, s, P, a);
+    return hipGetLastError();
+}
+hipError_t launch_solid_force(const DevParams &P, const SolidArgs &a, hipStream_t s) {
+    if (P.ns == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_force, grid_for(P.ns), dim3(kBlock), 0, s, P, a);
+    return hipGetLastError();
+}
+hipError_t launch_solid_update(const DevParams &P, const SolidArgs &a, hipStream_t s) {
+    if (P.ns == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_solid_update, grid_for(P.ns), dim3(kBlock), 0, s, P, a);
+    return hipGetLastError();
+}
+
+}  // namespace fsa

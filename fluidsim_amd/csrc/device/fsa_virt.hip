@@ -1,0 +1,101 @@
+// fsa_virt.hip -- virtual wall particles (getVirtData, src/virt.cu:9-75).
+//
+// Per wall particle: the nearest fluid particle over the 27 neighbour cells
+// (strict <, first in traversal order wins, no cut-off) gives density,
+// hydrostatically corrected pressure and the velocity that is mirrored about
+// the wall normal; with no fluid in reach the wall keeps rho0, p = 0, u = 0.
+// The normal is the normalised negative kernel-gradient sum over the other
+// wall particles in reach.  Wall particles never move, so the normal is the
+// same every step: it is computed on the first update and read back after,
+// which gives bit-identical results to recomputing it.
+#include "fsa_internal.h"
+#include "fsa_math.h"
+
+namespace fsa {
+
+namespace {
+
+template <bool NORMALS>
+__global__ __launch_bounds__(kBlock) void k_virt(DevParams P, VirtArgs a) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= P.nv) return;
+    const float4 xi4 = a.vpos_in[i];
+    const float xi = xi4.x, yi = xi4.y, zi = xi4.z;
+    float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    float pressure = 0.0f, density = P.rho0, nearest = kLargeFloat;
+    int best = -1;
+    float bdx = 0.0f, bdy = 0.0f, bdz = 0.0f;
+
+    for_each_column(P, a.vcell[i], [&](int c0, int c1) {
+        for (int j = a.fstart[c0], e = a.fstart[c1]; j < e; ++j) {
+            const float4 pj = a.fpos[j];
+            const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+            const float d2 = len2(dx, dy, dz);
+            if (d2 < nearest) {
+                nearest = d2;
+                best = j;
+                bdx = dx;
+                bdy = dy;
+                bdz = dz;
+            }
+        }
+        if (NORMALS) {
+            for (int j = a.vstart[c0], e = a.vstart[c1]; j < e; ++j) {
+                const float4 pj = a.vpos_in[j];
+                const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+                const float d2 = len2(dx, dy, dz);
+                if (d2 < P.R2 && j != i) {
+                    const float kc = expf(-(d2 * P.h_inv2)) * P.kd;
+                    nx += (-dx) * kc;
+                    ny += (-dy) * kc;
+                    nz += (-dz) * kc;
+                }
+            }
+        }
+    });
+
+    if (best >= 0) {
+        const float4 pj = a.fpos[best];
+        const float4 vj = a.fvel[best];
+        density = pj.w;
+        pressure = vj.w - density * (P.gx * bdx + P.gy * bdy + P.gz * bdz);
+        ux = vj.x;
+        uy = vj.y;
+        uz = vj.z;
+    }
+    if (NORMALS) {
+        const float n2 = len2(nx, ny, nz);
+        if (n2 < kEpsForSum) {
+            nx = ny = nz = 0.0f;
+        } else {
+            const float r = 1.0f / sqrtf(n2);
+            nx *= r;
+            ny *= r;
+            nz *= r;
+        }
+        a.vnrm[i] = make_float4(nx, ny, nz, 0.0f);
+    } else {
+        const float4 n4 = a.vnrm[i];
+        nx = n4.x;
+        ny = n4.y;
+        nz = n4.z;
+    }
+    const float m = 2.0f * (ux * nx + uy * ny + uz * nz);
+    a.vpos[i] = make_float4(xi, yi, zi, density);
+    a.vvel[i] = make_float4(ux - m * nx, uy - m * ny, uz - m * nz, pressure);
+}
+
+}  // namespace
+
+hipError_t launch_virt(const DevParams &P, const VirtArgs &a, hipStream_t s) {
+    if (P.nv == 0) return hipSuccess;
+    const dim3 grid((P.nv + kBlock - 1) / kBlock);
+    if (a.compute_normals)
+        hipLaunchKernelGGL(k_virt<true>, grid, dim3(kBlock), 0, s, P, a);
+    else
+        hipLaunchKernelGGL(k_virt<false>, grid, dim3(kBlock), 0, s, P, a);
+    return hipGetLastError();
+}
+
+}  // namespace fsa

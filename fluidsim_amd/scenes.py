@@ -1,0 +1,278 @@
+"""Seeded dam-break-onto-an-elastic-gate scenes in the reference's formats.
+
+The reference ships one case generator (cases/dam_break/particle_generate.cpp)
+that fills a lattice at spacing dp, marks sites outside the tank as virtual
+wall particles, sites in the gate box as solid (FIXED below the floor), and
+sites in the water column as fluid, then writes the three .dat files
+(particle_generate.cpp:93-137).  This module builds the same kind of scene for
+any size, with gravity along -y as the solver hard-wires it
+(src/physics.cpp:12-15): x is the flow direction, y is up, z is the width.
+
+A scene gives
+  * `ini_text()`            -- config/input.ini with the reference's keys
+  * `arrays()`              -- float32/int32 particle arrays in .dat order
+  * `params()`              -- the derived fsa_params, from the C++ host layer
+  * `write_case(root)`      -- <root>/config/input.ini + <root>/data/*.dat
+"""
+from __future__ import annotations
+
+import math
+import os
+import tempfile
+from dataclasses import dataclass, field, replace
+from pathlib import Path
+
+import numpy as np
+
+from . import _native
+
+
+@dataclass(frozen=True)
+class DamBreak:
+    """Water column in a closed tank, flowing onto a gate clamped in the floor.
+
+    Sizes are lattice counts (sites of spacing dp) as (x, y, z) = (flow, up, width).
+    """
+
+    dp: float = 0.004
+    fluid: tuple = (50, 35, 25)       # water column, starting at the tank corner
+    tank: tuple = (100, 70, 25)       # inner tank size
+    gate: tuple | None = (4, 20, 25)  # gate size including its clamped rows
+    gate_x0: int = 75                 # first gate column
+    gate_z0: int | None = None        # first gate layer in z (None: centred)
+    gate_clamp_rows: int = 3          # bottom rows below the floor, type FIXED
+    wall_layers: int = 3
+    jitter: float = 0.01              # fluid position jitter, fraction of dp (uniform +-)
+    seed: int = 0
+    # physics (config/input.ini keys)
+    h_ratio: float = 1.3              # smoothing_length / particle_diameter
+    impact_ratio: float = 2.0         # impact_length_hsml_ratio
+    fluid_density: float = 1000.0
+    sound_speed: float = 30.0
+    gamma: float = 7.0
+    viscosity: float = 1.0e-3         # dynamic viscosity (the key's meaning, include/properties.h:13)
+    is_viscid: bool = True
+    gravity: float = 9.81
+    solid_density: float = 1100.0
+    youngs_modulus: float = 12.0e6
+    poisson_ratio: float = 0.4
+    arti_vis_alpha: float = 2.5
+    arti_vis_beta: float = 2.5
+    dt: float | None = None           # default 0.25 h / c0
+    solid_sub_step: int = 4
+    iend: int = 1
+    result_interval: int = 1000000
+
+    # ---- derived --------------------------------------------------------
+    @property
+    def h(self) -> float:
+        return self.h_ratio * self.dp
+
+    @property
+    def time_step(self) -> float:
+        return self.dt if self.dt is not None else 0.25 * self.h / self.sound_speed
+
+    def domain(self):
+        lo = (0.0, 0.0, 0.0)
+        hi = tuple(n * self.dp for n in self.tank)
+        return lo, hi
+
+    def ini_text(self, extra: dict | None = None) -> str:
+        lo, hi = self.domain()
+        keys = {
+            "gravity": self.gravity,
+            "is_viscid": "true" if self.is_viscid else "false",
+            "fluid_density": self.fluid_density,
+            "fluid_sound_speed": self.sound_speed,
+            "gamma": self.gamma,
+            "viscosity": self.viscosity,
+            "solid_density": self.solid_density,
+            "youngs_modulus": self.youngs_modulus,
+            "poisson_ratio": self.poisson_ratio,
+            "arti_vis_alpha": self.arti_vis_alpha,
+            "arti_vis_beta": self.arti_vis_beta,
+            "xmin": lo[0], "xmax": hi[0], "ymin": lo[1], "ymax": hi[1], "zmin": lo[2], "zmax": hi[2],
+            "dt": self.time_step,
+            "istart": 0,
+            "iend": self.iend,
+            "result_interval": self.result_interval,
+            "sort_every_step": 1,
+            "solid_sub_step": self.solid_sub_step,
+            "smoothing_length": self.h,
+            "particle_diameter": self.dp,
+            "impact_length_hsml_ratio": self.impact_ratio,
+            "thread_num": 256,
+        }
+        if extra:
+            keys.update(extra)
+        sections = {
+            "physics": ["gravity"],
+            "fluid": ["is_viscid", "fluid_density", "fluid_sound_speed", "gamma", "viscosity"],
+            "solid": ["solid_density", "youngs_modulus", "poisson_ratio", "arti_vis_alpha", "arti_vis_beta"],
+            "domain": ["xmin", "xmax", "ymin", "ymax", "zmin", "zmax"],
+            "time": ["dt", "istart", "iend", "result_interval", "sort_every_step", "solid_sub_step"],
+            "kernel": ["smoothing_length", "particle_diameter", "impact_length_hsml_ratio", "thread_num"],
+        }
+        lines = []
+        used = set()
+        for sec, names in sections.items():
+            lines.append(f"[{sec}]")
+            for n in names:
+                lines.append(f"{n} = {_fmt(keys[n])}")
+                used.add(n)
+            lines.append("")
+        rest = [k for k in keys if k not in used]
+        if rest:
+            lines.append("[extra]")
+            lines += [f"{k} = {_fmt(keys[k])}" for k in rest]
+            lines.append("")
+        return "\n".join(lines)
+
+    # ---- particles --------------------------------------------------------
+    def _sites(self, lo, hi):
+        ax = [np.arange(lo[a], hi[a], dtype=np.int64) for a in range(3)]
+        g = np.meshgrid(*ax, indexing="ij")
+        return np.stack([x.ravel() for x in g], axis=1)
+
+    def _gate_box(self):
+        if self.gate is None:
+            return None
+        gx, gy, gz = self.gate
+        z0 = self.gate_z0 if self.gate_z0 is not None else (self.tank[2] - gz) // 2
+        lo = (self.gate_x0, -self.gate_clamp_rows, z0)
+        hi = (self.gate_x0 + gx, -self.gate_clamp_rows + gy, z0 + gz)
+        return lo, hi
+
+    def arrays(self) -> dict:
+        dp = np.float64(self.dp)
+        W = self.wall_layers
+        L = self.tank
+        gate = self._gate_box()
+
+        def in_box(s, box):
+            if box is None:
+                return np.zeros(len(s), dtype=bool)
+            lo, hi = box
+            return np.all((s >= np.array(lo)) & (s < np.array(hi)), axis=1)
+
+        def pos(s):
+            return ((s.astype(np.float64) + 0.5) * dp).astype(np.float32)
+
+        # solid gate
+        if gate is not None:
+            ss = self._sites(*gate)
+            s_pos = pos(ss)
+            s_type = np.where(ss[:, 1] < 0, _native.FIXED_SOLID, _native.UNCONSTRAINED_SOLID).astype(np.int32)
+        else:
+            ss = np.zeros((0, 3), dtype=np.int64)
+            s_pos = np.zeros((0, 3), dtype=np.float32)
+            s_type = np.zeros(0, dtype=np.int32)
+
+        # walls: every lattice site of the padded box outside the tank, minus gate sites
+        vs = self._sites((-W, -W, -W), (L[0] + W, L[1] + W, L[2] + W))
+        inside = np.all((vs >= 0) & (vs < np.array(L)), axis=1)
+        vs = vs[~inside & ~in_box(vs, gate)]
+        v_pos = pos(vs)
+        v_type = np.zeros(len(vs), dtype=np.int32)  # FIXED_VIRT
+
+        # fluid column with seeded jitter
+        fs = self._sites((0, 0, 0), tuple(min(self.fluid[a], L[a]) for a in range(3)))
+        fs = fs[~in_box(fs, gate)]
+        rng = np.random.default_rng(self.seed)
+        jit = rng.uniform(-self.jitter, self.jitter, size=fs.shape) * dp
+        f_pos = ((fs.astype(np.float64) + 0.5) * dp + jit).astype(np.float32)
+        nf = len(f_pos)
+        return {
+            "fluid_position": f_pos,
+            "fluid_velocity": np.zeros((nf, 3), dtype=np.float32),
+            "fluid_density": np.full(nf, self.fluid_density, dtype=np.float32),
+            "fluid_pressure": np.zeros(nf, dtype=np.float32),
+            "solid_position": s_pos,
+            "solid_velocity": np.zeros_like(s_pos),
+            "solid_reference_position": s_pos.copy(),
+            "solid_type": s_type,
+            "virt_position": v_pos,
+            "virt_type": v_type,
+        }
+
+    def counts(self) -> tuple:
+        a = self.arrays()
+        return len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])
+
+    def params(self, counts: tuple | None = None, extra: dict | None = None) -> _native.FsaParams:
+        """Derived parameters from the host layer's Parameter::initiate."""
+        nf, ns, nv = counts if counts is not None else self.counts()
+        return params_from_ini_text(self.ini_text(extra), nf, ns, nv)
+
+    def write_case(self, root, arrays: dict | None = None, extra: dict | None = None) -> Path:
+        root = Path(root)
+        a = arrays if arrays is not None else self.arrays()
+        (root / "config").mkdir(parents=True, exist_ok=True)
+        (root / "data").mkdir(parents=True, exist_ok=True)
+        (root / "config" / "input.ini").write_text(self.ini_text(extra))
+        write_dat(root / "data", a)
+        return root
+
+    def with_(self, **kw) -> "DamBreak":
+        return replace(self, **kw)
+
+
+def _fmt(v) -> str:
+    if isinstance(v, float):
+        return repr(v)
+    return str(v)
+
+
+def params_from_ini_text(text: str, nf: int, ns: int, nv: int) -> _native.FsaParams:
+    lib = _native.host_lib()
+    fd, path = tempfile.mkstemp(suffix=".ini")
+    try:
+        with os.fdopen(fd, "w") as f:
+            f.write(text)
+        p = _native.FsaParams()
+        _native.check_host(lib.fsh_parameter_from_ini(path.encode(), nf, ns, nv, p))
+        return p
+    finally:
+        os.unlink(path)
+
+
+def write_dat(data_dir, a: dict) -> None:
+    """The three .dat files in the reference's text layout (include/host_data.cuh:25, 84, 152)."""
+    data_dir = Path(data_dir)
+    fp, fu, fr, fpr = a["fluid_position"], a["fluid_velocity"], a["fluid_density"], a["fluid_pressure"]
+    rows = np.concatenate([fp, fu, fr[:, None], fpr[:, None]], axis=1).astype(np.float32)
+    _write_rows(data_dir / "fluid_particle.dat", rows, None)
+    sp, su, sX, st = (a["solid_position"], a["solid_velocity"], a["solid_reference_position"], a["solid_type"])
+    rows = np.concatenate([sp, su, sX], axis=1).astype(np.float32)
+    _write_rows(data_dir / "solid_particle.dat", rows, st)
+    _write_rows(data_dir / "virt_particle.dat", a["virt_position"].astype(np.float32), a["virt_type"])
+
+
+def _write_rows(path, rows: np.ndarray, ints) -> None:
+    # %.9g round-trips every float32 exactly through the reader's istream >> float
+    with open(path, "w") as f:
+        f.write(f"{len(rows)}\n")
+        for r in range(len(rows)):
+            vals = " ".join(f"{float(v):.9g}" for v in rows[r])
+            f.write(vals + (f" {int(ints[r])}" if ints is not None else "") + "\n")
+
+
+# BASELINE scenes (y-up mapping of the z-up specs)
+def c1_cpu_scene(seed: int = 0) -> DamBreak:
+    """43,750 fluid (50x35x25, dp 4 mm) + 2,000 solid (4x20x25) + 3 wall layers, S = 4."""
+    return DamBreak(dp=0.004, fluid=(50, 35, 25), tank=(100, 70, 25), gate=(4, 20, 25), gate_x0=75,
+                    gate_clamp_rows=3, wall_layers=3, seed=seed, iend=500)
+
+
+def c2_showcase_scene(seed: int = 0) -> DamBreak:
+    """4,000,000 fluid (200x200x100, dp 1 mm) + 27,000 solid (6x60x75) + 3 wall layers, S = 4."""
+    return DamBreak(dp=0.001, fluid=(200, 200, 100), tank=(400, 400, 100), gate=(6, 60, 75), gate_x0=300,
+                    gate_clamp_rows=3, wall_layers=3, seed=seed, iend=1000)
+
+
+def tiny_scene(seed: int = 0, **kw) -> DamBreak:
+    """A few thousand particles: every stage active, fits an oracle run in well under a second."""
+    base = dict(dp=0.01, fluid=(8, 10, 6), tank=(16, 14, 6), gate=(2, 8, 6), gate_x0=11, gate_clamp_rows=2,
+                wall_layers=2, seed=seed, sound_speed=20.0, youngs_modulus=2.0e6, solid_density=1100.0)
+    base.update(kw)
+    return DamBreak(**base)

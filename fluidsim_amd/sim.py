@@ -1,0 +1,175 @@
+"""Python view of one device context (the C ABI of include/fluidsim_amd.h).
+
+`DeviceSim` mirrors the reference's DeviceCalculation stage functions
+(include/calculation.cuh:12-20) by name; every method is a thin call into
+libfluidsim_amd.so and raises FsaError on a non-zero status.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native
+from ._native import FLUID, SOLID, VIRT, FsaParams, check
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+
+
+def _f32(a, n_cols=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a
+
+
+def _ptr(a, t=_fp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class DeviceSim:
+    def __init__(self, params: FsaParams, stream: int | None = None):
+        self.lib = _native.device_lib()
+        self.params = params
+        self.nf, self.ns, self.nv = params.n_fluid, params.n_solid, params.n_virt
+        self.ncell = params.cell_number_total
+        h = C.c_void_p()
+        check(self.lib.fsa_create(C.byref(params), C.c_void_p(stream) if stream else None, C.byref(h)))
+        self.handle = h
+
+    # ---- lifetime ----------------------------------------------------------
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.fsa_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- uploads -------------------------------------------------------------
+    def upload(self, a: dict):
+        L = self.lib
+        if self.nf:
+            x, u = _f32(a["fluid_position"]), _f32(a["fluid_velocity"])
+            r, p = _f32(a["fluid_density"]), _f32(a["fluid_pressure"])
+            check(L.fsa_upload_fluid(self.handle, _ptr(x), _ptr(u), _ptr(r), _ptr(p)))
+        if self.ns:
+            x, u, X = _f32(a["solid_position"]), _f32(a["solid_velocity"]), _f32(a["solid_reference_position"])
+            t = np.ascontiguousarray(a["solid_type"], dtype=np.int32)
+            check(L.fsa_upload_solid(self.handle, _ptr(x), _ptr(u), _ptr(X), _ptr(t, _ip)))
+        if self.nv:
+            x = _f32(a["virt_position"])
+            t = np.ascontiguousarray(a["virt_type"], dtype=np.int32)
+            check(L.fsa_upload_virt(self.handle, _ptr(x), _ptr(t, _ip)))
+
+    # ---- stages (DeviceCalculation) -------------------------------------------
+    def particle_cell_update(self):
+        check(self.lib.fsa_particle_cell_update(self.handle))
+
+    def fill_cell(self):
+        check(self.lib.fsa_fill_cell(self.handle))
+
+    def couple_calculation(self):
+        check(self.lib.fsa_couple_calculation(self.handle))
+
+    def virt_update(self):
+        check(self.lib.fsa_virt_update(self.handle))
+
+    def fluid_calculation(self):
+        check(self.lib.fsa_fluid_calculation(self.handle))
+
+    def solid_calculation(self):
+        check(self.lib.fsa_solid_calculation(self.handle))
+
+    def step(self, n: int = 1):
+        check(self.lib.fsa_step(self.handle, int(n)))
+
+    def synchronize(self):
+        check(self.lib.fsa_synchronize(self.handle))
+
+    # ---- downloads (.dat order) ------------------------------------------------
+    def fluid(self) -> dict:
+        n = self.nf
+        x, u = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+        r, p = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        check(self.lib.fsa_download_fluid(self.handle, _ptr(x), _ptr(u), _ptr(r), _ptr(p)))
+        return {"position": x, "velocity": u, "density": r, "pressure": p}
+
+    def fluid_rates(self) -> dict:
+        n = self.nf
+        drho, a = np.zeros(n, np.float32), np.zeros((n, 3), np.float32)
+        check(self.lib.fsa_download_fluid_rates(self.handle, _ptr(drho), _ptr(a)))
+        return {"drho_dt": drho, "du_dt": a}
+
+    def solid(self) -> dict:
+        n = self.ns
+        out = {k: np.zeros((n, 3), np.float32) for k in ("position", "velocity", "normal", "du_dt", "couple_dudt")}
+        out["pressure"] = np.zeros(n, np.float32)
+        for k in ("deformation", "stress_1st_piola", "correction"):
+            out[k] = np.zeros((n, 9), np.float32)
+        check(self.lib.fsa_download_solid(self.handle, _ptr(out["position"]), _ptr(out["velocity"]),
+                                          _ptr(out["pressure"]), _ptr(out["normal"]), _ptr(out["du_dt"]),
+                                          _ptr(out["couple_dudt"]), _ptr(out["deformation"]),
+                                          _ptr(out["stress_1st_piola"]), _ptr(out["correction"])))
+        return out
+
+    def virt(self) -> dict:
+        n = self.nv
+        out = {k: np.zeros((n, 3), np.float32) for k in ("position", "normal", "velocity")}
+        out["density"] = np.zeros(n, np.float32)
+        out["pressure"] = np.zeros(n, np.float32)
+        check(self.lib.fsa_download_virt(self.handle, _ptr(out["position"]), _ptr(out["normal"]),
+                                         _ptr(out["velocity"]), _ptr(out["density"]), _ptr(out["pressure"])))
+        return out
+
+    # ---- introspection ---------------------------------------------------------
+    def _count(self, kind):
+        return {FLUID: self.nf, SOLID: self.ns, VIRT: self.nv}[kind]
+
+    def cell_ids(self, kind: int) -> np.ndarray:
+        out = np.zeros(self._count(kind), np.int32)
+        check(self.lib.fsa_get_cell_ids(self.handle, kind, _ptr(out, _ip)))
+        return out
+
+    def cell_start(self, kind: int) -> np.ndarray:
+        out = np.zeros(self.ncell + 1, np.int32)
+        check(self.lib.fsa_get_cell_start(self.handle, kind, _ptr(out, _ip)))
+        return out
+
+    def sorted_ids(self, kind: int) -> np.ndarray:
+        out = np.zeros(self._count(kind), np.int32)
+        check(self.lib.fsa_get_sorted_ids(self.handle, kind, _ptr(out, _ip)))
+        return out
+
+    def jump_count(self) -> int:
+        v = C.c_int64()
+        check(self.lib.fsa_get_jump_count(self.handle, C.byref(v)))
+        return int(v.value)
+
+    def set_debug(self, flags: int):
+        check(self.lib.fsa_set_debug(self.handle, flags))
+
+    def set_timing(self, on: bool):
+        check(self.lib.fsa_set_timing(self.handle, 1 if on else 0))
+
+    def stage_times(self) -> dict:
+        ms = (C.c_double * 5)()
+        n = (C.c_int64 * 5)()
+        check(self.lib.fsa_get_stage_times(self.handle, ms, n))
+        return {name: (ms[i], n[i]) for i, name in enumerate(_native.STAGES)}
+
+    def reset_stage_times(self):
+        check(self.lib.fsa_reset_stage_times(self.handle))
+
+    def memory_used(self) -> int:
+        v = C.c_int64()
+        check(self.lib.fsa_memory_used(self.handle, C.byref(v)))
+        return int(v.value)

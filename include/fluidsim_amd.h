@@ -1,0 +1,228 @@
+/*
+ * fluidsim_amd.h -- C ABI of the MI355X-native FluidSim step pipeline.
+ *
+ * The reference has no plugin API: its step pipeline is a set of static C++
+ * entry points over process-global state (include/calculation.cuh:9-21,
+ * src/calculation.cu:14-35) fed from `__constant__` copies of `Parameter`
+ * and `Data_Pointer` (src/memory_copy.cu:5-12).  This header replaces that
+ * interface with an opaque per-scene context, the same stage granularity,
+ * plain pointers and sizes, and integer status codes.
+ *
+ * Conventions
+ *  - Every function returns FSA_OK (0) or a non-zero FSA_ERR_* code; the
+ *    message of the last failure on the calling thread is fsa_last_error().
+ *    Nothing in the library calls exit() (the reference's CHECK macro did,
+ *    include/cuda_macro.cuh:19-33).
+ *  - Host arrays are caller-owned and in *reference order* (the order of the
+ *    .dat files); vectors are packed xyz triples (3*N floats), 3x3 matrices
+ *    are 9 floats per particle in glm column-major order (m[col][row]).
+ *  - Device arrays are owned by the context.  Particles are kept sorted by
+ *    cell inside the context; downloads undo the permutation.
+ *  - One context is not thread-safe; distinct contexts on distinct streams
+ *    are independent.
+ */
+#ifndef FLUIDSIM_AMD_H
+#define FLUIDSIM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSA_ABI_VERSION 1
+
+enum fsa_status {
+    FSA_OK = 0,
+    FSA_ERR_INVALID = 1, /* bad argument, size or particle outside the cell grid */
+    FSA_ERR_HIP = 2,     /* HIP runtime failure (message holds hipGetErrorString) */
+    FSA_ERR_NOMEM = 3,   /* device allocation failed */
+    FSA_ERR_STATE = 4,   /* call out of order (e.g. step before upload) */
+    FSA_ERR_NODEVICE = 5 /* no usable gfx950 device */
+};
+
+/* particle kinds, same numbering as the reference ParticleType (include/types.h:4-9) */
+enum fsa_kind { FSA_FLUID = 0, FSA_SOLID = 1, FSA_VIRT = 2 };
+
+/* solid types, same numbering as SolidType (include/types.h:11-15) */
+enum fsa_solid_type { FSA_UNCONSTRAINED_SOLID = 0, FSA_FIXED_SOLID = 1 };
+
+/*
+ * Derived parameter block.  Field for field this is the part of the
+ * reference `Parameter` (include/parameter.h:17-49) that the device code
+ * reads through `dev_par`; the derivations are those of
+ * FluidProperties/SolidProperties::initiate (src/properties.cpp:47-66),
+ * Physics::readParameter (src/physics.cpp:9-19), Kernel::initiate
+ * (src/kernel.cpp:25-60), Domain::initiate (src/domain.cpp:28-47) and
+ * SimTime::readParameter (src/sim_time.cpp:8-24).  The host layer
+ * (Parameter::initiate) fills it from config/input.ini.
+ */
+typedef struct fsa_params {
+    int32_t n_fluid, n_solid, n_virt;
+
+    /* Physics */
+    float gravity[3];
+
+    /* FluidProperties */
+    float fluid_sound_speed;
+    float fluid_reference_density;
+    float fluid_reference_density_inverse;
+    float fluid_gamma;
+    float fluid_gamma_inv;
+    float fluid_viscosity; /* dynamic viscosity */
+    float fluid_coefficient_p2rho;
+    float fluid_coefficient_rho2p;
+    float fluid_min_pressure;
+    float fluid_min_density;
+    float fluid_max_density;
+    int32_t fluid_is_viscid;
+
+    /* SolidProperties */
+    float solid_reference_density;
+    float solid_youngs_modulus;
+    float solid_poisson_ratio;
+    float solid_lambda;
+    float solid_nu; /* shear modulus (the reference calls it nu) */
+    float solid_bulk_modulus;
+    float solid_sound_speed;
+    float solid_artificial_viscosity[2]; /* alpha, beta */
+
+    /* Domain */
+    float domain_min[3], domain_max[3];
+    float cell_min[3], cell_max[3];
+    float interval[3], interval_inv[3];
+    int32_t cell_number[3];
+    int32_t cell_number_total;
+
+    /* SimTime */
+    float dt;
+    int32_t istart, iend;
+    int32_t solid_sub_step;
+    int32_t result_interval;
+    int32_t sort_every_step;
+
+    /* Kernel */
+    float smoothing_length;
+    float particle_diameter;
+    float impact_length_hsml_ratio;
+    float smoothing_length_square;
+    float smoothing_length_inverse;
+    float smoothing_length_inverse_square;
+    float impact_distance_square;
+    float kernel_coefficient_1;
+    float kernel_coefficient_2;
+    float kernel_differential;
+    int32_t thread_num; /* read for key compatibility; block size is fixed at 256 on gfx950 */
+} fsa_params;
+
+typedef struct fsa_sim fsa_sim;
+
+/* ---- library ---------------------------------------------------------- */
+
+/* ABI version of the loaded library (FSA_ABI_VERSION at build time). */
+int fsa_abi_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *fsa_last_error(void);
+
+/* Number of visible gfx950 devices (0 when none). */
+int fsa_device_count(int *count);
+
+/* ---- context lifetime (replaces Sim::initiate + MemoryCopy::copyConstant,
+ *      src/sim.cu:304-321, src/memory_copy.cu:8-12) --------------------- */
+
+/* Allocate a context and its device buffers on the current device.
+ * `stream` is a hipStream_t or NULL for a stream owned by the context. */
+int fsa_create(const fsa_params *params, void *stream, fsa_sim **out);
+int fsa_destroy(fsa_sim *sim);
+
+/* Upload particles from host arrays in .dat order (Sim::readParticleData +
+ * Sim::copyData, src/sim.cu:40-151, 304-314).  Initial per-particle state is
+ * that of the *_Host_List::addList defaults (include/host_data.cuh:20-163).
+ * Every particle must lie inside the cell grid, else FSA_ERR_INVALID. */
+int fsa_upload_fluid(fsa_sim *sim, const float *position, const float *velocity,
+                     const float *density, const float *pressure);
+int fsa_upload_solid(fsa_sim *sim, const float *position, const float *velocity,
+                     const float *reference_position, const int32_t *solid_type);
+int fsa_upload_virt(fsa_sim *sim, const float *position, const int32_t *virt_type);
+
+/* Download state in .dat order; any pointer may be NULL to skip a field.
+ * (Replaces the thrust host_vector copies of src/output.cu.) */
+int fsa_download_fluid(fsa_sim *sim, float *position, float *velocity,
+                       float *density, float *pressure);
+int fsa_download_solid(fsa_sim *sim, float *position, float *velocity,
+                       float *pressure, float *normal, float *du_dt, float *couple_dudt,
+                       float *deformation, float *stress_1st_piola, float *correction);
+int fsa_download_virt(fsa_sim *sim, float *position, float *normal, float *velocity,
+                      float *density, float *pressure);
+
+/* ---- step stages (DeviceCalculation, include/calculation.cuh:12-20) ----
+ * Stages are enqueued on the context stream and return without waiting. */
+
+/* K1-K3: cell id of every fluid/solid particle (src/pair.cu:8-70). */
+int fsa_particle_cell_update(fsa_sim *sim);
+/* Cell lists: counting sort of particles by (cell id, particle id) and
+ * per-cell start offsets; replaces the linked-list rebuild of
+ * src/pair.cu:72-207.  Every cell holds the same particle set as the
+ * reference lists; the order inside a cell is ascending .dat index, and a
+ * particle that moved more than one cell is kept instead of dropped. */
+int fsa_fill_cell(fsa_sim *sim);
+/* K6: solid normals and Shepard pressure (src/solid.cu:262-359). */
+int fsa_couple_calculation(fsa_sim *sim);
+/* K7: virtual wall particle state (src/virt.cu:9-87). */
+int fsa_virt_update(fsa_sim *sim);
+/* K8+K9 fused: Riemann-SPH pair sums and explicit update (src/fluid.cu:26-284). */
+int fsa_fluid_calculation(fsa_sim *sim);
+/* K10-K12 x solid_sub_step (src/solid.cu:10-260). */
+int fsa_solid_calculation(fsa_sim *sim);
+/* The whole step loop body of src/calculation.cu:23-33, n_steps times. */
+int fsa_step(fsa_sim *sim, int n_steps);
+/* Wait for all work enqueued on the context stream. */
+int fsa_synchronize(fsa_sim *sim);
+
+/* ---- introspection for parity checks and diagnostics ------------------ */
+
+/* Cell id of every particle of `kind`, in .dat order. */
+int fsa_get_cell_ids(fsa_sim *sim, int kind, int32_t *cell_id);
+/* Per-cell start offsets into the sorted order (cell_number_total+1 ints). */
+int fsa_get_cell_start(fsa_sim *sim, int kind, int32_t *cell_start);
+/* .dat index of the particle at each sorted slot (N ints). */
+int fsa_get_sorted_ids(fsa_sim *sim, int kind, int32_t *sorted_id);
+/* Fluid pair sums of the last fluid stage (drho_dt N floats, du_dt 3N),
+ * in .dat order.  Only recorded while FSA_DEBUG_PAIR_SUMS is set. */
+int fsa_download_fluid_rates(fsa_sim *sim, float *drho_dt, float *du_dt);
+/* Number of particles that moved more than one cell in one step since the
+ * context was created (the reference silently dropped such particles from
+ * its lists, src/pair.cu:87-150; this build keeps them). */
+int fsa_get_jump_count(fsa_sim *sim, int64_t *count);
+
+#define FSA_DEBUG_PAIR_SUMS 1u
+int fsa_set_debug(fsa_sim *sim, unsigned flags);
+
+/* ---- timing ------------------------------------------------------------ */
+
+enum fsa_stage {
+    FSA_STAGE_CELL = 0,   /* particleCellUpdate + fillCell */
+    FSA_STAGE_COUPLE = 1, /* coupleCalculation */
+    FSA_STAGE_VIRT = 2,   /* virtUpdate */
+    FSA_STAGE_FLUID = 3,  /* fluidCalculation (the fused pair+update kernel) */
+    FSA_STAGE_SOLID = 4,  /* solidCalculation (all sub-steps) */
+    FSA_STAGE_COUNT = 5
+};
+
+/* When enabled, HIP events bracket every stage on the context stream. */
+int fsa_set_timing(fsa_sim *sim, int enabled);
+/* Sum of event-measured stage times (ms) and number of samples per stage
+ * since the last reset; synchronizes the stream. */
+int fsa_get_stage_times(fsa_sim *sim, double *ms, int64_t *samples);
+int fsa_reset_stage_times(fsa_sim *sim);
+
+/* Device bytes held by the context. */
+int fsa_memory_used(fsa_sim *sim, int64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUIDSIM_AMD_H */

@@ -1,0 +1,198 @@
+"""ctypes wrapper of the CPU oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "build" / "liboracle.so"
+
+CANONICAL, EXACT = 0, 1
+FLUID, SOLID, VIRT = 0, 1, 2
+
+_lib = None
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile the oracle with its Makefile (g++ -fopenmp)."""
+    proc = subprocess.run(["make", "-B", "-C", str(HERE)], capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{proc.stdout}\n{proc.stderr}")
+    if verbose:
+        print(proc.stdout)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError("oracle is not built; run `make -C oracle`")
+        L = C.CDLL(str(LIB_PATH))
+        from fluidsim_amd._native import FsaParams
+        vp, ip, dp, fp = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_float)
+        sig = {
+            "orc_last_error": (C.c_char_p, []),
+            "orc_set_threads": (C.c_int, [C.c_int]),
+            "orc_create": (C.c_int, [C.POINTER(FsaParams), C.c_int, C.c_int, C.POINTER(vp)]),
+            "orc_destroy": (C.c_int, [vp]),
+            "orc_upload_fluid": (C.c_int, [vp, fp, fp, fp, fp]),
+            "orc_upload_solid": (C.c_int, [vp, fp, fp, fp, ip]),
+            "orc_upload_virt": (C.c_int, [vp, fp, ip]),
+            "orc_download_fluid": (C.c_int, [vp, dp, dp, dp, dp, dp, dp]),
+            "orc_download_solid": (C.c_int, [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp]),
+            "orc_download_virt": (C.c_int, [vp, dp, dp, dp, dp]),
+            "orc_particle_cell_update": (C.c_int, [vp]),
+            "orc_fill_cell": (C.c_int, [vp]),
+            "orc_couple_calculation": (C.c_int, [vp]),
+            "orc_virt_update": (C.c_int, [vp]),
+            "orc_fluid_calculation": (C.c_int, [vp]),
+            "orc_solid_calculation": (C.c_int, [vp]),
+            "orc_solid_deformation": (C.c_int, [vp]),
+            "orc_solid_force": (C.c_int, [vp]),
+            "orc_solid_update": (C.c_int, [vp]),
+            "orc_step": (C.c_int, [vp, C.c_int]),
+            "orc_get_cell_ids": (C.c_int, [vp, C.c_int, ip]),
+            "orc_get_cell_start": (C.c_int, [vp, C.c_int, ip]),
+            "orc_get_sorted_ids": (C.c_int, [vp, C.c_int, ip]),
+            "orc_neighbors": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint, C.c_int, ip, ip, C.c_int]),
+            "orc_brute_neighbors": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint, C.c_int, ip, ip, C.c_int]),
+            "orc_lost_count": (C.c_int64, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype, fn.argtypes = res, args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(f"oracle error {rc}: {lib().orc_last_error().decode()}")
+
+
+def set_threads(n: int) -> int:
+    return lib().orc_set_threads(int(n))
+
+
+def _p(a, t=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+class Oracle:
+    """CPU restatement of the step pipeline; precision 32 or 64, mode CANONICAL or EXACT."""
+
+    def __init__(self, params, precision: int = 32, mode: int = CANONICAL):
+        self.L = lib()
+        self.params = params
+        self.nf, self.ns, self.nv = params.n_fluid, params.n_solid, params.n_virt
+        self.ncell = params.cell_number_total
+        h = C.c_void_p()
+        _check(self.L.orc_create(C.byref(params), precision, mode, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, a: dict):
+        f32 = lambda k: np.ascontiguousarray(a[k], dtype=np.float32)
+        i32 = lambda k: np.ascontiguousarray(a[k], dtype=np.int32)
+        if self.nf:
+            x, u, r, p = f32("fluid_position"), f32("fluid_velocity"), f32("fluid_density"), f32("fluid_pressure")
+            _check(self.L.orc_upload_fluid(self.h, _p(x, C.c_float), _p(u, C.c_float), _p(r, C.c_float),
+                                           _p(p, C.c_float)))
+        if self.ns:
+            x, u, X, t = (f32("solid_position"), f32("solid_velocity"), f32("solid_reference_position"),
+                          i32("solid_type"))
+            _check(self.L.orc_upload_solid(self.h, _p(x, C.c_float), _p(u, C.c_float), _p(X, C.c_float),
+                                           _p(t, C.c_int32)))
+        if self.nv:
+            x, t = f32("virt_position"), i32("virt_type")
+            _check(self.L.orc_upload_virt(self.h, _p(x, C.c_float), _p(t, C.c_int32)))
+
+    # stages
+    def particle_cell_update(self): _check(self.L.orc_particle_cell_update(self.h))
+    def fill_cell(self): _check(self.L.orc_fill_cell(self.h))
+    def couple_calculation(self): _check(self.L.orc_couple_calculation(self.h))
+    def virt_update(self): _check(self.L.orc_virt_update(self.h))
+    def fluid_calculation(self): _check(self.L.orc_fluid_calculation(self.h))
+    def solid_calculation(self): _check(self.L.orc_solid_calculation(self.h))
+    def solid_deformation(self): _check(self.L.orc_solid_deformation(self.h))
+    def solid_force(self): _check(self.L.orc_solid_force(self.h))
+    def solid_update(self): _check(self.L.orc_solid_update(self.h))
+    def step(self, n: int = 1): _check(self.L.orc_step(self.h, int(n)))
+
+    def fluid(self) -> dict:
+        n = self.nf
+        out = {"position": np.zeros((n, 3)), "velocity": np.zeros((n, 3)), "density": np.zeros(n),
+               "pressure": np.zeros(n), "drho_dt": np.zeros(n), "du_dt": np.zeros((n, 3))}
+        _check(self.L.orc_download_fluid(self.h, *(_p(out[k]) for k in
+                                                   ("position", "velocity", "density", "pressure", "drho_dt",
+                                                    "du_dt"))))
+        return out
+
+    def solid(self) -> dict:
+        n = self.ns
+        out = {k: np.zeros((n, 3)) for k in ("position", "velocity", "normal", "du_dt", "couple_dudt")}
+        out["pressure"] = np.zeros(n)
+        for k in ("deformation", "stress_1st_piola", "correction"):
+            out[k] = np.zeros((n, 9))
+        _check(self.L.orc_download_solid(self.h, *(_p(out[k]) for k in
+                                                   ("position", "velocity", "pressure", "normal", "du_dt",
+                                                    "couple_dudt", "deformation", "stress_1st_piola",
+                                                    "correction"))))
+        return out
+
+    def virt(self) -> dict:
+        n = self.nv
+        out = {"normal": np.zeros((n, 3)), "velocity": np.zeros((n, 3)), "density": np.zeros(n),
+               "pressure": np.zeros(n)}
+        _check(self.L.orc_download_virt(self.h, *(_p(out[k]) for k in ("normal", "velocity", "density",
+                                                                         "pressure"))))
+        return out
+
+    def _count(self, kind):
+        return {FLUID: self.nf, SOLID: self.ns, VIRT: self.nv}[kind]
+
+    def cell_ids(self, kind) -> np.ndarray:
+        out = np.zeros(self._count(kind), np.int32)
+        _check(self.L.orc_get_cell_ids(self.h, kind, _p(out, C.c_int32)))
+        return out
+
+    def cell_start(self, kind) -> np.ndarray:
+        out = np.zeros(self.ncell + 1, np.int32)
+        _check(self.L.orc_get_cell_start(self.h, kind, _p(out, C.c_int32)))
+        return out
+
+    def sorted_ids(self, kind) -> np.ndarray:
+        out = np.zeros(self._count(kind), np.int32)
+        _check(self.L.orc_get_sorted_ids(self.h, kind, _p(out, C.c_int32)))
+        return out
+
+    def neighbors(self, kind_i, i, mask=7, reference_metric=False, brute=False):
+        """(kind, j) pairs within the cut-off in visiting order (brute: O(N) scan, kind-major)."""
+        fn = self.L.orc_brute_neighbors if brute else self.L.orc_neighbors
+        cap = 4096
+        ok, oj = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        n = fn(self.h, kind_i, int(i), mask, 1 if reference_metric else 0, _p(ok, C.c_int32), _p(oj, C.c_int32),
+               cap)
+        assert n <= cap
+        return list(zip(ok[:n].tolist(), oj[:n].tolist()))
+
+    def lost_count(self) -> int:
+        return int(self.L.orc_lost_count(self.h))

@@ -1,0 +1,64 @@
+"""Shared helpers of the test suite: scenes, run drivers and tolerance checks."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from fluidsim_amd import scenes
+
+# ---- tolerances (relative, stated once for every parity test) --------------
+# Per-particle fluid density, velocity and position: 1e-5 relative.
+#   position  |dx|   <= 1e-5 * L          (L = largest domain extent)
+#   velocity  |du|   <= 1e-5 * U          (U = max(max|u|, sqrt(2 g H)), H = water depth)
+#   density   |drho| <= 1e-5 * rho
+#   pressure  |dp|   <= 1e-5 * rho0 * c0^2 * 1.1^(gamma-1)   (density tolerance through the EOS slope)
+REL = 1.0e-5
+
+
+def velocity_scale(sc: scenes.DamBreak, *us) -> float:
+    H = sc.fluid[1] * sc.dp
+    umax = max([float(np.abs(u).max()) if len(u) else 0.0 for u in us] + [0.0])
+    return max(umax, math.sqrt(2.0 * sc.gravity * H))
+
+
+def length_scale(sc: scenes.DamBreak) -> float:
+    return max(sc.tank) * sc.dp
+
+
+def pressure_tolerance(sc: scenes.DamBreak, rel: float = REL) -> float:
+    return rel * sc.fluid_density * sc.sound_speed ** 2 * 1.1 ** (sc.gamma - 1.0)
+
+
+def max_abs(a, b) -> float:
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.abs(a - b).max()) if a.size else 0.0
+
+
+def compare_fluid(sc, got: dict, ref: dict, rel: float = REL) -> dict:
+    """Errors of a fluid state against a reference state, and whether each is within tolerance."""
+    L = length_scale(sc)
+    U = velocity_scale(sc, ref["velocity"])
+    dx = max_abs(got["position"], ref["position"])
+    du = max_abs(got["velocity"], ref["velocity"])
+    drho = float(np.max(np.abs(np.asarray(got["density"], np.float64) - ref["density"]) / np.abs(ref["density"])))
+    dp = max_abs(got["pressure"], ref["pressure"])
+    ptol = pressure_tolerance(sc, rel)
+    return {
+        "position": (dx / L, dx <= rel * L),
+        "velocity": (du / U, du <= rel * U),
+        "density": (drho, drho <= rel),
+        "pressure": (dp / ptol * rel, dp <= ptol),
+    }
+
+
+def assert_within(report: dict, what: str = ""):
+    bad = {k: v[0] for k, v in report.items() if not v[1]}
+    assert not bad, f"{what}: relative errors over tolerance {bad}; all: { {k: v[0] for k, v in report.items()} }"
+
+
+def same_class(got, o32, o64, scale: float, factor: float = 4.0, rel: float = REL) -> tuple:
+    """FP32 result `got` is as accurate as the FP32 oracle: |got-o64| <= factor*|o32-o64| + rel*scale."""
+    e_got = max_abs(got, o64)
+    e_32 = max_abs(o32, o64)
+    return e_got, e_32, e_got <= factor * e_32 + rel * scale
