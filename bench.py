@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Throughput of the FluidSim step pipeline on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--scene c2|c1|tiny]
+
+One step is one pass of the reference's step loop body (src/calculation.cu:23-33:
+cell hash, cell lists, solid coupling fields, walls, fluid pair+update,
+solid_sub_step solid sub-steps) over the resident scene.  The default scene
+is the 4,000,000 fluid + 27,000 solid dam break (dp = 1 mm, 3 wall layers,
+h = 1.3 dp, dt = 0.25 h / c0, 4 solid sub-steps).  Inputs are uploaded once
+before the timed region.  Prints one JSON line on stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from fluidsim_amd import scenes  # noqa: E402
+from fluidsim_amd.sim import DeviceSim  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def scene_for(name: str) -> scenes.DamBreak:
+    if name == "c2":
+        return scenes.c2_showcase_scene()
+    if name == "c1":
+        return scenes.c1_cpu_scene()
+    if name == "tiny":
+        return scenes.tiny_scene()
+    raise SystemExit(f"unknown scene {name}")
+
+
+def active_walls(params, fluid_pos: np.ndarray, virt_pos: np.ndarray) -> int:
+    """Wall particles inside the 27-cell neighbourhood of some fluid cell (the walls the fluid pass reads)."""
+    cmin = np.array(params.cell_min, np.float32)
+    inv = np.array(params.interval_inv, np.float32)
+    n = np.array(params.cell_number)
+    fc = np.clip(((fluid_pos - cmin) * inv).astype(np.int64), 0, n - 1)
+    occ = np.zeros(tuple(n), dtype=bool)
+    occ[fc[:, 0], fc[:, 1], fc[:, 2]] = True
+    dil = occ.copy()
+    for ax in range(3):
+        d = dil.copy()
+        sl_lo = [slice(None)] * 3
+        sl_hi = [slice(None)] * 3
+        sl_lo[ax], sl_hi[ax] = slice(0, -1), slice(1, None)
+        d[tuple(sl_lo)] |= dil[tuple(sl_hi)]
+        d[tuple(sl_hi)] |= dil[tuple(sl_lo)]
+        dil = d
+    vc = np.clip(((virt_pos - cmin) * inv).astype(np.int64), 0, n - 1)
+    return int(dil[vc[:, 0], vc[:, 1], vc[:, 2]].sum())
+
+
+def sync_all(dev):
+    dev.synchronize()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def cpu_baseline(params, arrays, budget_s: float, threads: int) -> dict:
+    """The FP32 oracle (OpenMP) on the same scene: a bounded sample of whole steps."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+
+    used = oracle.set_threads(threads)
+    o = oracle.Oracle(params, 32, oracle.CANONICAL)
+    o.upload(arrays)
+    t0 = time.perf_counter()
+    o.step(1)
+    first = time.perf_counter() - t0
+    steps = 1
+    more = max(0, min(20, int(budget_s / max(first, 1e-3)) - 1))
+    if more:
+        t1 = time.perf_counter()
+        o.step(more)
+        total = time.perf_counter() - t1
+        steps_timed, secs = more, total
+    else:
+        steps_timed, secs = 1, first
+    steps += more
+    o.close()
+    n = params.n_fluid + params.n_solid
+    return {
+        "value": n * steps_timed / secs,
+        "unit": "particle-steps/s",
+        "cores": used,
+        "kind": "port",
+        "sample": f"{steps_timed} step(s) of the same scene after 1 untimed step "
+                  f"({n} particles, FP32 canonical traversal, OpenMP over particles)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--scene", default="c2")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1 "
+                                   "(a scene is one GPU's share; see DESIGN.md for the sharding plan)"}))
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+
+    sc = scene_for(args.scene)
+    t = time.perf_counter()
+    arrays = sc.arrays()
+    nf, ns, nv = (len(arrays["fluid_position"]), len(arrays["solid_position"]), len(arrays["virt_position"]))
+    params = sc.params((nf, ns, nv))
+    log(f"scene {args.scene}: {nf} fluid + {ns} solid + {nv} wall particles, grid "
+        f"{list(params.cell_number)} = {params.cell_number_total} cells (built in {time.perf_counter() - t:.1f}s)")
+
+    dev = DeviceSim(params)
+    dev.upload(arrays)
+    sync_all(dev)
+    log(f"uploaded; device memory {dev.memory_used() / 2**20:.0f} MiB")
+
+    dev.step(args.warmup)
+    sync_all(dev)
+    log(f"warmup {args.warmup} steps done")
+
+    dev.set_timing(True)
+    dev.reset_stage_times()
+    sync_all(dev)
+    t0 = time.perf_counter()
+    dev.step(args.steps)
+    sync_all(dev)
+    t1 = time.perf_counter()
+    stage = dev.stage_times()
+    dev.set_timing(False)
+    secs = t1 - t0
+    ms_per_step = 1e3 * secs / args.steps
+    value = (nf + ns) * args.steps / secs
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.4g} particle-steps/s")
+    for k, (ms, n) in stage.items():
+        if n:
+            log(f"  stage {k:6s}: {ms / n:8.4f} ms avg over {n}")
+
+    # roofline of the dominant kernel: the fused fluid pair+update pass
+    fl = dev.fluid()
+    n_active = active_walls(params, fl["position"], arrays["virt_position"])
+    fluid_ms, fluid_n = stage["fluid"]
+    fluid_avg_s = fluid_ms / max(fluid_n, 1) / 1e3
+    bytes_per_launch = 68 * nf + 32 * n_active + 40 * ns + 8 * params.cell_number_total
+    achieved = bytes_per_launch / fluid_avg_s / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "traffic": None,
+        "kernel": "k_fluid (fused pair sums + update)",
+        "kernel_avg_ms": round(fluid_avg_s * 1e3, 5),
+        "algorithmic_bytes_per_launch": int(bytes_per_launch),
+        "active_walls": n_active,
+    }
+
+    result = {
+        "metric": "particle-steps/s @4M+27k FSI",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded dam-break lattice, uploaded once)",
+        "config": {
+            "workload": f"dam break onto elastic gate, scene {args.scene}",
+            "fluid": nf, "solid": ns, "walls": nv,
+            "dp": sc.dp, "h_over_dp": sc.h_ratio, "solid_sub_step": sc.solid_sub_step,
+            "cells": params.cell_number_total,
+        },
+        "stage_ms": {k: (ms / n if n else None) for k, (ms, n) in stage.items()},
+        "roofline": roofline,
+    }
+    if rank == 0 and not args.no_cpu_baseline:
+        log(f"cpu baseline: oracle FP32 on {args.cpu_threads} threads, ~{args.cpu_budget:.0f}s budget")
+        result["cpu_baseline"] = cpu_baseline(params, arrays, args.cpu_budget, args.cpu_threads)
+        log(f"cpu baseline: {result['cpu_baseline']['value']:.4g} particle-steps/s")
+    dev.close()
+    if rank == 0:
+        print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

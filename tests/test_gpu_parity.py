@@ -59,14 +59,15 @@ def test_one_step_stage_by_stage(gpu_available, tiny):
         s.fill_cell()
         s.couple_calculation()
         s.virt_update()
-    # wall state
+    # wall state; normals are unit vectors normalised from sums whose terms
+    # cancel, so they are held to 1e-5 on each component
     gv, ov = dev.virt(), o.virt()
-    np.testing.assert_allclose(gv["normal"], ov["normal"], atol=1e-6)
+    np.testing.assert_allclose(gv["normal"], ov["normal"], atol=REL)
     np.testing.assert_array_equal(gv["density"], ov["density"].astype(np.float32))
     assert max_abs(gv["velocity"], ov["velocity"]) <= REL * velocity_scale(tiny, ov["velocity"])
     # solid coupling fields
     gs, os_ = dev.solid(), o.solid()
-    np.testing.assert_allclose(gs["normal"], os_["normal"], atol=1e-6)
+    np.testing.assert_allclose(gs["normal"], os_["normal"], atol=REL)
     np.testing.assert_allclose(gs["pressure"], os_["pressure"], atol=1e-3, rtol=REL)
     # fluid pass
     for s in (dev, o, o64):
