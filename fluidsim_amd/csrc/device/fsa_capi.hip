@@ -125,6 +125,11 @@ DevParams make_dev_params(const fsa_params &p) {
     d.c1 = p.kernel_coefficient_1;
     d.c2 = p.kernel_coefficient_2;
     d.kd = p.kernel_differential;
+    d.kexp2 = (float)((double)p.smoothing_length_inverse_square * 1.4426950408889634);
+    d.cs_inv = 1.0f / p.fluid_sound_speed;
+    d.eps_vis = 0.01f * p.smoothing_length_square;
+    d.visc10 = 10.0f * p.fluid_viscosity;
+    d.max_speed = 0.3f * p.fluid_sound_speed;
     return d;
 }
 }  // namespace fsa

@@ -37,7 +37,24 @@ __global__ __launch_bounds__(kBlock) void k_hash_count(DevParams P, int n, const
     const float4 x = pos[s];
     const int c = find_cell(P, x.x, x.y, x.z);
     key[s] = c;
-    rank[s] = atomicAdd(&count[c], 1);
+    // Particles arrive sorted by their previous cell, so the lanes of a wave
+    // share few cells: one atomic per distinct cell and wave, the lanes of a
+    // cell take consecutive ranks (the order is fixed later by the rank pass).
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long todo = __ballot(1);
+    int r = 0;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int lc = __shfl(c, leader, 64);
+        const unsigned long long same = __ballot(c == lc);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&count[lc], __popcll(same));
+        base = __shfl(base, leader, 64);
+        if (c == lc) r = base + __popcll(same & below);
+        todo &= ~same;
+    }
+    rank[s] = r;
     if (count_jumps) {
         // more than one cell in any direction since the last build: the
         // reference's list rebuild would have lost this particle

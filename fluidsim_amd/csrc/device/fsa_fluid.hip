@@ -6,8 +6,20 @@
 // One thread per fluid particle in sorted order, so the 64 lanes of a
 // wavefront are ~64 spatially adjacent particles whose neighbour columns
 // overlap and stay in L1/L2.  Each column of the 27-cell neighbourhood is a
-// contiguous slot range per kind, read as float4 (x,y,z,rho) for the
-// candidate test and float4 (u,p) only for partners inside the cut-off.
+// contiguous slot range per kind.
+//
+// Only ~1 candidate in 6 lies inside the cut-off, and in SIMT a wave pays
+// for the pair math whenever any lane takes that branch.  So every run of
+// candidates is done in two phases: the candidate test appends the in-range
+// slots to a lane-private list in LDS (entry k of lane t at [k][t], so lanes
+// hit distinct banks), then a dense loop evaluates the pair terms of the
+// list.  Pairs are summed in exactly the candidate order.
+//
+// Pair arithmetic is FP32 with the hardware reciprocal, reciprocal square
+// root and exp2, and the algebraically reduced forms noted inline; the
+// results agree with the IEEE-division form of the reference to a few ulp per
+// term (see the parity tests for the bound held).
+//
 // The kernel reads the state buffer and writes the updated state into the
 // other buffer (Jacobi, one writer per particle), which removes the
 // round-trip of du_dt and drho_dt through HBM.
@@ -20,126 +32,169 @@ namespace {
 
 enum PairKind { kFluidFluid = 0, kFluidSolid = 1, kFluidVirt = 2 };
 
-// One Riemann pair term.  Partner state (uj, rhoj, pj) is already mirrored
-// for solids.  The solid partner dissipates with c0 instead of
-// min(3 du, c0) (fluid.cu:105-106); the wall partner clamps p* at 0
-// (fluid.cu:154).
+constexpr int kListCap = 32;  // list entries per lane (32 KB of LDS per 256-thread block)
+
+struct Self {
+    float x, y, z, rho, ux, uy, uz, p;
+    float inv_rho, two_rho;
+};
+
+struct Acc {
+    float drho, ax, ay, az;
+};
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float exp2_hw(float x) { return __builtin_amdgcn_exp2f(x); }
+
+// One Riemann pair term (fluid.cu:26-167) for a partner inside the cut-off.
+//   p*  = (rho_i p_j + rho_j p_i)/(rho_i+rho_j) + [du>=0] rho_i rho_j du/(rho_i+rho_j) * D
+//         D = min(3 du, c0) (fluid, wall) or c0 (solid); walls clamp p* >= 0
+//   v*  = (rho_i u_i + rho_j u_j)/(rho_i+rho_j) + (p_i-p_j)/(rho_i+rho_j)/c0 e
+//   a  += -2 p*/rho_i dx k + [viscid] 10 mu ((u_j-u_i).dx)/(|dx|^2+0.01h^2)/rho_i dx k
+//   rho' += 2 rho_i (u_i - v*).dx k
+// with (u_i - v*).dx reduced to rho_j/(rho_i+rho_j) (u_i-u_j).dx - (p_i-p_j)/(rho_i+rho_j)/c0 |dx|.
 template <int KIND>
-__device__ __forceinline__ void riemann_pair(const DevParams &P, float dx, float dy, float dz, float d2, float uix,
-                                             float uiy, float uiz, float rhoi, float pi, float ujx, float ujy,
-                                             float ujz, float rhoj, float pj, float &drho, float &ax, float &ay,
-                                             float &az) {
-    const float q2 = d2 * P.h_inv2;
-    const float kc = expf(-q2) * P.kd;
-    float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-    if (q2 != 0.0f) {
-        const float r = 1.0f / sqrtf(d2);
-        ex = dx * r;
-        ey = dy * r;
-        ez = dz * r;
-    }
-    const float rsum = rhoi + rhoj;
-    float ps = (rhoi * pj + rhoj * pi) / rsum;
-    const float du = (uix - ujx) * ex + (uiy - ujy) * ey + (uiz - ujz) * ez;
-    if (du >= 0.0f) {
-        const float diss = KIND == kFluidSolid ? P.cs : fminf(3.0f * du, P.cs);
-        ps += rhoi * rhoj * du / rsum * diss;
-    }
+__device__ __forceinline__ void pair_term(const DevParams &P, const Self &me, float dx, float dy, float dz, float d2,
+                                          float ujx, float ujy, float ujz, float rhoj, float pj, Acc &acc) {
+    const float kc = exp2_hw(-d2 * P.kexp2) * P.kd;
+    const float r = d2 != 0.0f ? rsq(d2) : 0.0f;  // 1/|dx|, e = dx r
+    const float inv_rsum = KIND == kFluidSolid ? 0.5f * me.inv_rho : rcp(me.rho + rhoj);
+    const float udx = (me.ux - ujx) * dx + (me.uy - ujy) * dy + (me.uz - ujz) * dz;  // (u_i-u_j).dx
+    const float du = udx * r;
+    float ps = (me.rho * pj + rhoj * me.p) * inv_rsum;
+    const float diss = KIND == kFluidSolid ? P.cs : fminf(3.0f * du, P.cs);
+    const float raise = me.rho * rhoj * inv_rsum * du * diss;
+    ps += du >= 0.0f ? raise : 0.0f;
     if (KIND == kFluidVirt) ps = fmaxf(ps, 0.0f);
-    const float dp = (pi - pj) / rsum / P.cs;
-    const float vsx = (rhoi * uix + rhoj * ujx) / rsum + dp * ex;
-    const float vsy = (rhoi * uiy + rhoj * ujy) / rsum + dp * ey;
-    const float vsz = (rhoi * uiz + rhoj * ujz) / rsum + dp * ez;
-    const float s = -2.0f * ps / rhoi;
-    ax += (s * dx) * kc;
-    ay += (s * dy) * kc;
-    az += (s * dz) * kc;
-    drho += 2.0f * rhoi * ((uix - vsx) * dx + (uiy - vsy) * dy + (uiz - vsz) * dz) * kc;
-    if (P.viscid) {
-        float vis = (ujx - uix) * dx + (ujy - uiy) * dy + (ujz - uiz) * dz;
-        vis /= d2 + 0.01f * P.h2;
-        const float t = 10.0f * P.visc * vis / rhoi;
-        ax += (t * dx) * kc;
-        ay += (t * dy) * kc;
-        az += (t * dz) * kc;
+    const float dp = (me.p - pj) * inv_rsum * P.cs_inv;
+    float coef = -2.0f * ps * me.inv_rho;
+    if (P.viscid) coef -= P.visc10 * udx * rcp(d2 + P.eps_vis) * me.inv_rho;
+    coef *= kc;
+    acc.ax += dx * coef;
+    acc.ay += dy * coef;
+    acc.az += dz * coef;
+    acc.drho += me.two_rho * kc * (rhoj * inv_rsum * udx - dp * (d2 * r));
+}
+
+template <int KIND>
+__device__ __forceinline__ void one_pair(const DevParams &P, const Self &me, const float4 &pj, const float4 &vj,
+                                         const float4 &n, Acc &acc) {
+    const float dx = pj.x - me.x, dy = pj.y - me.y, dz = pj.z - me.z;
+    const float d2 = len2(dx, dy, dz);
+    if (KIND == kFluidSolid) {
+        // mirrored solid velocity (fluid.cu:79-83); rho_j = rho_i, p_j = p_s
+        const float m = 2.0f * ((me.ux - vj.x) * n.x + (me.uy - vj.y) * n.y + (me.uz - vj.z) * n.z);
+        pair_term<KIND>(P, me, dx, dy, dz, d2, me.ux - m * n.x, me.uy - m * n.y, me.uz - m * n.z, me.rho, n.w, acc);
+    } else {
+        pair_term<KIND>(P, me, dx, dy, dz, d2, vj.x, vj.y, vj.z, pj.w, vj.w, acc);
     }
 }
 
-// Tait equation of state, densityToPressure (src/fluid.cu:16-24)
-__device__ __forceinline__ float density_to_pressure(const DevParams &P, float rho) {
-    const float p = (powf(rho * P.rho0_inv, P.gamma) - 1.0f) * P.rho2p;
-    return fmaxf(p, P.p_min);
+constexpr int kPairBatch = 4;  // partner gathers in flight in the pair loop
+constexpr int kScanBatch = 8;  // candidate loads in flight in the test loop
+
+// pair terms of list entries [0, cnt), in list order; the partner state of
+// kPairBatch entries is gathered before any of them is used
+template <int KIND>
+__device__ __forceinline__ void flush(const DevParams &P, const Self &me, const float4 *__restrict__ pos,
+                                      const float4 *__restrict__ vel, const float4 *__restrict__ np,
+                                      const int *list, int cnt, Acc &acc) {
+    for (int k = 0; k < cnt; k += kPairBatch) {
+        float4 pj[kPairBatch], vj[kPairBatch], nj[kPairBatch];
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) {
+            if (k + q < cnt) {
+                const int j = list[(k + q) * kBlock];
+                pj[q] = pos[j];
+                vj[q] = vel[j];
+                if (KIND == kFluidSolid) nj[q] = np[j];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q)
+            if (k + q < cnt) one_pair<KIND>(P, me, pj[q], vj[q], nj[q], acc);
+    }
+}
+
+// candidate test over slots [b, e) of one kind, then the pair terms of the
+// hits; kScanBatch candidate loads are issued before the first test
+template <int KIND>
+__device__ __forceinline__ void run(const DevParams &P, const Self &me, int b, int e, const float4 *__restrict__ pos,
+                                    const float4 *__restrict__ vel, const float4 *__restrict__ np, int *list,
+                                    Acc &acc) {
+    int cnt = 0;
+    for (int j = b; j < e; j += kScanBatch) {
+        float4 q[kScanBatch];
+#pragma unroll
+        for (int k = 0; k < kScanBatch; ++k)
+            if (j + k < e) q[k] = pos[j + k];
+#pragma unroll
+        for (int k = 0; k < kScanBatch; ++k) {
+            if (j + k < e) {
+                const float d2 = len2(q[k].x - me.x, q[k].y - me.y, q[k].z - me.z);
+                if (d2 < P.R2) {
+                    list[cnt * kBlock] = j + k;
+                    ++cnt;
+                }
+            }
+        }
+        // at most kScanBatch entries are added per round
+        if (cnt > kListCap - kScanBatch) {
+            flush<KIND>(P, me, pos, vel, np, list, cnt, acc);
+            cnt = 0;
+        }
+    }
+    flush<KIND>(P, me, pos, vel, np, list, cnt, acc);
 }
 
 template <bool RATES>
 __global__ __launch_bounds__(kBlock) void k_fluid(DevParams P, FluidStepArgs a) {
+    __shared__ int s_list[kListCap * kBlock];
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= P.nf) return;
+    if (i >= P.nf) return;  // lists are lane-private: no barrier follows
+    int *list = s_list + threadIdx.x;
+
     const float4 pi4 = a.fpos[i];
     const float4 vi4 = a.fvel[i];
-    const float xi = pi4.x, yi = pi4.y, zi = pi4.z, rhoi = pi4.w;
-    const float uix = vi4.x, uiy = vi4.y, uiz = vi4.z, pi = vi4.w;
-    const float R2 = P.R2;
-    float drho = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
+    Self me;
+    me.x = pi4.x;
+    me.y = pi4.y;
+    me.z = pi4.z;
+    me.rho = pi4.w;
+    me.ux = vi4.x;
+    me.uy = vi4.y;
+    me.uz = vi4.z;
+    me.p = vi4.w;
+    me.inv_rho = 1.0f / me.rho;
+    me.two_rho = 2.0f * me.rho;
+    Acc acc{0.0f, 0.0f, 0.0f, 0.0f};
 
     for_each_column(P, a.fcell[i], [&](int c0, int c1) {
-        // fluid partners (fluidFluidCalculate, fluid.cu:26-69)
-        for (int j = a.fstart[c0], e = a.fstart[c1]; j < e; ++j) {
-            const float4 pj = a.fpos[j];
-            const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-            const float d2 = len2(dx, dy, dz);
-            if (d2 < R2) {
-                const float4 vj = a.fvel[j];
-                riemann_pair<kFluidFluid>(P, dx, dy, dz, d2, uix, uiy, uiz, rhoi, pi, vj.x, vj.y, vj.z, pj.w, vj.w,
-                                          drho, ax, ay, az);
-            }
-        }
-        // solid partners (fluidSolidCalculate, fluid.cu:71-119): the partner
-        // velocity is u_i mirrored about the solid normal, rho_j = rho_i,
-        // p_j = the solid's Shepard pressure
-        for (int j = a.sstart[c0], e = a.sstart[c1]; j < e; ++j) {
-            const float4 pj = a.spos[j];
-            const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-            const float d2 = len2(dx, dy, dz);
-            if (d2 < R2) {
-                const float4 us = a.svel[j];
-                const float4 np = a.snp[j];
-                const float m = 2.0f * ((uix - us.x) * np.x + (uiy - us.y) * np.y + (uiz - us.z) * np.z);
-                riemann_pair<kFluidSolid>(P, dx, dy, dz, d2, uix, uiy, uiz, rhoi, pi, uix - m * np.x,
-                                          uiy - m * np.y, uiz - m * np.z, rhoi, np.w, drho, ax, ay, az);
-            }
-        }
-        // wall partners (fluidVirtCalculate, fluid.cu:121-167)
-        for (int j = a.vstart[c0], e = a.vstart[c1]; j < e; ++j) {
-            const float4 pj = a.vpos[j];
-            const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-            const float d2 = len2(dx, dy, dz);
-            if (d2 < R2) {
-                const float4 vj = a.vvel[j];
-                riemann_pair<kFluidVirt>(P, dx, dy, dz, d2, uix, uiy, uiz, rhoi, pi, vj.x, vj.y, vj.z, pj.w, vj.w,
-                                         drho, ax, ay, az);
-            }
-        }
+        run<kFluidFluid>(P, me, a.fstart[c0], a.fstart[c1], a.fpos, a.fvel, nullptr, list, acc);
+        run<kFluidSolid>(P, me, a.sstart[c0], a.sstart[c1], a.spos, a.svel, a.snp, list, acc);
+        run<kFluidVirt>(P, me, a.vstart[c0], a.vstart[c1], a.vpos, a.vvel, nullptr, list, acc);
     });
 
     // fluidUpdate (fluid.cu:234-269): NaN scrub, explicit step with the old
     // velocity for the position, density clamp, EOS, domain clamp, speed cap
+    float drho = acc.drho, ax = acc.ax, ay = acc.ay, az = acc.az;
     if (!(drho == drho)) drho = 0.0f;
     if (!(ax == ax && ay == ay && az == az)) ax = ay = az = 0.0f;
     if (RATES) a.rates[i] = make_float4(ax, ay, az, drho);
     const float dt = P.dt;
-    float x = xi + uix * dt, y = yi + uiy * dt, z = zi + uiz * dt;
-    float ux = uix + (ax + P.gx) * dt, uy = uiy + (ay + P.gy) * dt, uz = uiz + (az + P.gz) * dt;
-    float rho = rhoi + drho * dt;
+    float x = me.x + me.ux * dt, y = me.y + me.uy * dt, z = me.z + me.uz * dt;
+    float ux = me.ux + (ax + P.gx) * dt, uy = me.uy + (ay + P.gy) * dt, uz = me.uz + (az + P.gz) * dt;
+    float rho = me.rho + drho * dt;
     rho = fminf(fmaxf(rho, P.rho_min), P.rho_max);
-    const float p = density_to_pressure(P, rho);
+    // Tait equation of state (densityToPressure, fluid.cu:16-24)
+    const float p = fmaxf((powf(rho * P.rho0_inv, P.gamma) - 1.0f) * P.rho2p, P.p_min);
     x = fminf(fmaxf(x, P.dmin[0]), P.dmax[0]);
     y = fminf(fmaxf(y, P.dmin[1]), P.dmax[1]);
     z = fminf(fmaxf(z, P.dmin[2]), P.dmax[2]);
     const float speed = sqrtf(len2(ux, uy, uz));
-    const float max_speed = 0.3f * P.cs;
-    if (speed > max_speed) {
-        const float f = max_speed / speed;
+    if (speed > P.max_speed) {
+        const float f = P.max_speed / speed;
         ux *= f;
         uy *= f;
         uz *= f;

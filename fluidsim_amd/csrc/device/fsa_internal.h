@@ -42,6 +42,12 @@ struct DevParams {
     float dt, dt_s;
     int sub;
     float h, h2, h_inv2, R2, c1, c2, kd;
+    // host-folded constants of the hot loops
+    float kexp2;      // h^-2 * log2(e): exp(-q^2) = exp2(-d2 * kexp2)
+    float cs_inv;     // 1 / c0
+    float eps_vis;    // 0.01 h^2
+    float visc10;     // 10 * mu
+    float max_speed;  // 0.3 c0
 };
 
 DevParams make_dev_params(const fsa_params &p);

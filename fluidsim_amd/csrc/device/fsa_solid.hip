@@ -2,19 +2,27 @@
 // deformation and stress (K10), forces (K11) and the sub-step integrator
 // (K12); src/solid.cu:10-359.
 //
-// One thread per solid particle in sorted order.  Neighbours are found in
-// the cell lists of the step (built from start-of-step positions) and, for
-// solid-solid terms, filtered by reference-configuration distance, exactly
-// as the reference walks its lists (src/solid.cu:53-69, 195-203).  Matrices
-// live in 9 SoA planes; the per-particle factors of the pair force that do
-// not depend on the partner (P_i L_i and det(F_i) F_i^-T) are computed once
-// in the deformation kernel instead of once per pair.
+// Neighbours are found in the cell lists of the step (built from
+// start-of-step positions) and, for solid-solid terms, filtered by
+// reference-configuration distance, exactly as the reference walks its lists
+// (src/solid.cu:53-69, 195-203).
+//
+// A gate has only tens of thousands of particles, too few for one thread per
+// particle to fill 256 CUs, and each particle walks ~500 candidates.  So a
+// group of kSub = 16 lanes serves one particle: the lanes take the candidates
+// of every run round-robin, and the partial sums are combined by a fixed
+// butterfly, after which lane 0 of the group holds the sum and writes.
+// Matrices live in 9 SoA planes; the per-particle factors of the pair force
+// that do not depend on the partner (P_i L_i and det(F_i) F_i^-T) are
+// computed once in the deformation kernel instead of once per pair.
 #include "fsa_internal.h"
 #include "fsa_math.h"
 
 namespace fsa {
 
 namespace {
+
+constexpr int kSub = 16;
 
 struct M3 {
     float m[3][3];  // row-major: m[r][c]
@@ -76,16 +84,35 @@ __device__ __forceinline__ void store9(float *__restrict__ base, int n, int i, c
     for (int e = 0; e < 9; ++e) base[(size_t)e * n + i] = a.m[e / 3][e % 3];
 }
 
+// sum over the kSub lanes of a group; lane 0 of the group always combines
+// in the same order, so its result is deterministic
+__device__ __forceinline__ float group_sum(float v) {
+    v += __shfl_xor(v, 8, kSub);
+    v += __shfl_xor(v, 4, kSub);
+    v += __shfl_xor(v, 2, kSub);
+    v += __shfl_xor(v, 1, kSub);
+    return v;
+}
+
+struct Lane {
+    int i;    // solid slot served by this group
+    int sub;  // lane within the group
+};
+__device__ __forceinline__ Lane lane_of() {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    return {t / kSub, t % kSub};
+}
+
 // ---- K6: solidCoupleUpdate (src/solid.cu:262-344) ------------------------
 __global__ __launch_bounds__(kBlock) void k_couple(DevParams P, SolidArgs a) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= P.ns) return;
-    const float4 xi4 = a.spos[i];
+    const Lane L = lane_of();
+    if (L.i >= P.ns) return;  // whole groups leave together
+    const float4 xi4 = a.spos[L.i];
     const float xi = xi4.x, yi = xi4.y, zi = xi4.z;
     float nx = 0.0f, ny = 0.0f, nz = 0.0f, pressure = 0.0f, wsum = 0.0f;
-    for_each_column(P, a.scell[i], [&](int c0, int c1) {
+    for_each_column(P, a.scell[L.i], [&](int c0, int c1) {
         // Shepard pressure from fluid (solidPressureCalculate, solid.cu:275-291)
-        for (int j = a.fstart[c0], e = a.fstart[c1]; j < e; ++j) {
+        for (int j = a.fstart[c0] + L.sub, e = a.fstart[c1]; j < e; j += kSub) {
             const float4 pj = a.fpos[j];
             const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
             const float d2 = len2(dx, dy, dz);
@@ -96,7 +123,7 @@ __global__ __launch_bounds__(kBlock) void k_couple(DevParams P, SolidArgs a) {
             }
         }
         // normal from solid neighbours (solidNormalCalculate, solid.cu:262-273)
-        for (int j = a.sstart[c0], e = a.sstart[c1]; j < e; ++j) {
+        for (int j = a.sstart[c0] + L.sub, e = a.sstart[c1]; j < e; j += kSub) {
             const float4 pj = a.spos[j];
             const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
             const float d2 = len2(dx, dy, dz);
@@ -108,6 +135,12 @@ __global__ __launch_bounds__(kBlock) void k_couple(DevParams P, SolidArgs a) {
             }
         }
     });
+    nx = group_sum(nx);
+    ny = group_sum(ny);
+    nz = group_sum(nz);
+    pressure = group_sum(pressure);
+    wsum = group_sum(wsum);
+    if (L.sub != 0) return;
     const float n2 = len2(nx, ny, nz);
     if (n2 < kEpsForSum) {
         nx = ny = nz = 0.0f;
@@ -118,18 +151,19 @@ __global__ __launch_bounds__(kBlock) void k_couple(DevParams P, SolidArgs a) {
         nz *= r;
     }
     const float p = wsum > kEpsFloat ? fmaxf(pressure / wsum, P.p_min) : 0.0f;
-    a.x.np[i] = make_float4(nx, ny, nz, p);
+    a.x.np[L.i] = make_float4(nx, ny, nz, p);
 }
 
 // ---- K10: solidDeformationCalculate (src/solid.cu:10-79) -----------------
 __global__ __launch_bounds__(kBlock) void k_deformation(DevParams P, SolidArgs a) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= P.ns) return;
+    const Lane L = lane_of();
+    if (L.i >= P.ns) return;
+    const int i = L.i;
     const float4 Xi = a.sref[i];
     const float4 xi = a.spos[i];
     M3 F = mzero(), B = mzero();
     for_each_column(P, a.scell[i], [&](int c0, int c1) {
-        for (int j = a.sstart[c0], e = a.sstart[c1]; j < e; ++j) {
+        for (int j = a.sstart[c0] + L.sub, e = a.sstart[c1]; j < e; j += kSub) {
             const float4 Xj = a.sref[j];
             const float dX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
             const float d2 = len2(dX[0], dX[1], dX[2]);
@@ -147,8 +181,16 @@ __global__ __launch_bounds__(kBlock) void k_deformation(DevParams P, SolidArgs a
             }
         }
     });
-    const M3 L = mdet(B) != 0.0f ? minverse(B) : mident();
-    F = mmul(F, L);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            F.m[r][c] = group_sum(F.m[r][c]);
+            B.m[r][c] = group_sum(B.m[r][c]);
+        }
+    if (L.sub != 0) return;
+    const M3 Lc = mdet(B) != 0.0f ? minverse(B) : mident();
+    F = mmul(F, Lc);
     // St Venant-Kirchhoff (solidStressCalculation, solid.cu:27-36)
     M3 E;
 #pragma unroll
@@ -168,8 +210,8 @@ __global__ __launch_bounds__(kBlock) void k_deformation(DevParams P, SolidArgs a
     const int n = P.ns;
     store9(a.x.F, n, i, F);
     store9(a.x.P, n, i, Pk);
-    store9(a.x.L, n, i, L);
-    store9(a.x.PL, n, i, mmul(Pk, L));
+    store9(a.x.L, n, i, Lc);
+    store9(a.x.PL, n, i, mmul(Pk, Lc));
     // det(F) F^-T, the factor of the artificial-viscosity force
     const M3 Fi = minverse(F);
     const float dF = mdet(F);
@@ -183,8 +225,9 @@ __global__ __launch_bounds__(kBlock) void k_deformation(DevParams P, SolidArgs a
 
 // ---- K11: solidForceCalculate (src/solid.cu:81-207) ----------------------
 __global__ __launch_bounds__(kBlock) void k_force(DevParams P, SolidArgs a) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= P.ns) return;
+    const Lane L = lane_of();
+    if (L.i >= P.ns) return;
+    const int i = L.i;
     const int n = P.ns;
     const float4 xi = a.spos[i], ui = a.svel[i], Xi = a.sref[i];
     const float4 np = a.x.np[i];
@@ -197,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_force(DevParams P, SolidArgs a) {
     for_each_column(P, a.scell[i], [&](int c0, int c1) {
         // fluid pressure on the solid (solidFluidCouple, solid.cu:81-117):
         // the solid side takes the fluid velocity mirrored about its normal
-        for (int j = a.fstart[c0], e = a.fstart[c1]; j < e; ++j) {
+        for (int j = a.fstart[c0] + L.sub, e = a.fstart[c1]; j < e; j += kSub) {
             const float4 pj = a.fpos[j];
             const float dx = pj.x - xi.x, dy = pj.y - xi.y, dz = pj.z - xi.z;
             const float d2 = len2(dx, dy, dz);
@@ -224,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_force(DevParams P, SolidArgs a) {
             }
         }
         // stress divergence + artificial viscosity (solidSolidForce, solid.cu:119-157)
-        for (int j = a.sstart[c0], e = a.sstart[c1]; j < e; ++j) {
+        for (int j = a.sstart[c0] + L.sub, e = a.sstart[c1]; j < e; j += kSub) {
             const float4 Xj = a.sref[j];
             const float dX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
             const float d2 = len2(dX[0], dX[1], dX[2]);
@@ -265,6 +308,13 @@ __global__ __launch_bounds__(kBlock) void k_force(DevParams P, SolidArgs a) {
             }
         }
     });
+    cx = group_sum(cx);
+    cy = group_sum(cy);
+    cz = group_sum(cz);
+    ax = group_sum(ax);
+    ay = group_sum(ay);
+    az = group_sum(az);
+    if (L.sub != 0) return;
     a.x.couple[i] = make_float4(cx, cy, cz, 0.0f);
     a.x.acc[i] = make_float4(ax + cx, ay + cy, az + cz, 0.0f);
 }
@@ -291,22 +341,23 @@ __global__ __launch_bounds__(kBlock) void k_solid_update(DevParams P, SolidArgs 
 }
 
 inline dim3 grid_for(int n) { return dim3((n + kBlock - 1) / kBlock); }
+inline dim3 grid_groups(int n) { return dim3((int)(((long long)n * kSub + kBlock - 1) / kBlock)); }
 
 }  // namespace
 
 hipError_t launch_couple(const DevParams &P, const SolidArgs &a, hipStream_t s) {
     if (P.ns == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_couple, grid_for(P.ns), dim3(kBlock), 0, s, P, a);
+    hipLaunchKernelGGL(k_couple, grid_groups(P.ns), dim3(kBlock), 0, s, P, a);
     return hipGetLastError();
 }
 hipError_t launch_solid_deformation(const DevParams &P, const SolidArgs &a, hipStream_t s) {
     if (P.ns == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_deformation, grid_for(P.ns), dim3(kBlock), 0, s, P, a);
+    hipLaunchKernelGGL(k_deformation, grid_groups(P.ns), dim3(kBlock), 0, s, P, a);
     return hipGetLastError();
 }
 hipError_t launch_solid_force(const DevParams &P, const SolidArgs &a, hipStream_t s) {
     if (P.ns == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_force, grid_for(P.ns), dim3(kBlock), 0, s, P, a);
+    hipLaunchKernelGGL(k_force, grid_groups(P.ns), dim3(kBlock), 0, s, P, a);
     return hipGetLastError();
 }
 hipError_t launch_solid_update(const DevParams &P, const SolidArgs &a, hipStream_t s) {

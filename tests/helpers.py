@@ -57,6 +57,40 @@ def assert_within(report: dict, what: str = ""):
     assert not bad, f"{what}: relative errors over tolerance {bad}; all: { {k: v[0] for k, v in report.items()} }"
 
 
+def custom_case(sc: scenes.DamBreak, fluid=None, solid=None, virt=None) -> tuple:
+    """Parameters of scene `sc` with hand-placed particles.
+
+    fluid: dict(position, velocity=0, density=rho0, pressure=0)
+    solid: dict(position, velocity=0, reference_position=position, type=0)
+    virt:  dict(position)
+    """
+    def arr(x, n, cols=3, fill=0.0, dt=np.float32):
+        if x is None:
+            return np.full((n, cols) if cols > 1 else n, fill, dtype=dt)
+        return np.ascontiguousarray(np.asarray(x, dtype=dt).reshape((n, cols) if cols > 1 else n))
+
+    fluid = fluid or {}
+    solid = solid or {}
+    virt = virt or {}
+    fx = np.asarray(fluid.get("position", np.zeros((0, 3))), np.float32).reshape(-1, 3)
+    sx = np.asarray(solid.get("position", np.zeros((0, 3))), np.float32).reshape(-1, 3)
+    vx = np.asarray(virt.get("position", np.zeros((0, 3))), np.float32).reshape(-1, 3)
+    nf, ns, nv = len(fx), len(sx), len(vx)
+    a = {
+        "fluid_position": fx,
+        "fluid_velocity": arr(fluid.get("velocity"), nf),
+        "fluid_density": arr(fluid.get("density"), nf, 1, sc.fluid_density),
+        "fluid_pressure": arr(fluid.get("pressure"), nf, 1, 0.0),
+        "solid_position": sx,
+        "solid_velocity": arr(solid.get("velocity"), ns),
+        "solid_reference_position": arr(solid.get("reference_position", sx), ns),
+        "solid_type": arr(solid.get("type"), ns, 1, 0, np.int32),
+        "virt_position": vx,
+        "virt_type": np.zeros(nv, np.int32),
+    }
+    return sc.params((nf, ns, nv)), a
+
+
 def same_class(got, o32, o64, scale: float, factor: float = 4.0, rel: float = REL) -> tuple:
     """FP32 result `got` is as accurate as the FP32 oracle: |got-o64| <= factor*|o32-o64| + rel*scale."""
     e_got = max_abs(got, o64)

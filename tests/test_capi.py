@@ -1,0 +1,76 @@
+"""The C ABI libraries load without a GPU and export every declared symbol (CPU only)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+from fluidsim_amd import _native
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared(header: str, prefix: str) -> set:
+    text = (ROOT / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(" + prefix + r"\w+)\s*\(", text))
+
+
+def test_device_library_exports_every_declared_symbol():
+    names = declared("fluidsim_amd.h", "fsa_")
+    assert len(names) >= 25
+    lib = C.CDLL(str(_native.DEVICE_LIB))
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    # the ctypes table covers the whole header
+    assert names == set(_native.DEVICE_API), names ^ set(_native.DEVICE_API)
+
+
+def test_host_library_exports_every_declared_symbol():
+    names = declared("fluidsim_host.h", "fsh_")
+    lib = C.CDLL(str(_native.HOST_LIB))
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert names == set(_native.HOST_API)
+
+
+def test_params_struct_layout_matches_the_header():
+    """ctypes mirror of fsa_params has the fields of the header in order."""
+    text = (ROOT / "include" / "fluidsim_amd.h").read_text()
+    body = text[text.index("typedef struct fsa_params"):text.index("} fsa_params;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in re.findall(r"(?:float|int32_t)\s+([^;]+);", body):
+        for name in decl.split(","):
+            fields.append(re.sub(r"\[.*\]", "", name).strip())
+    assert fields == [f for f, _ in _native.FsaParams._fields_]
+
+
+def test_abi_version_and_error_paths_without_a_device():
+    lib = _native.device_lib()
+    assert lib.fsa_abi_version() == 1
+    n = C.c_int(-1)
+    assert lib.fsa_device_count(C.byref(n)) == 0
+    if n.value > 0:
+        pytest.skip("a GPU is visible; the no-device path is not reachable")
+    p = _native.FsaParams()
+    h = C.c_void_p()
+    # an invalid parameter block is rejected before any device call
+    assert lib.fsa_create(C.byref(p), None, C.byref(h)) == _native.FSA_ERR_INVALID
+    assert b"cell_number" in lib.fsa_last_error()
+    assert lib.fsa_create(None, None, C.byref(h)) == _native.FSA_ERR_INVALID
+    assert lib.fsa_step(None, 1) == _native.FSA_ERR_INVALID
+
+
+def test_host_driver_reports_missing_device(tmp_path):
+    """fsh_run on a valid case fails cleanly (no abort) where no GPU is visible."""
+    from fluidsim_amd import scenes
+    n = C.c_int(0)
+    _native.device_lib().fsa_device_count(C.byref(n))
+    if n.value > 0:
+        pytest.skip("a GPU is visible")
+    root = scenes.tiny_scene().with_(iend=2).write_case(tmp_path / "case")
+    host = _native.host_lib()
+    rc = host.fsh_run(str(root).encode(), 0)
+    assert rc != 0
+    assert b"fsa_create" in host.fsh_last_error()
