@@ -86,6 +86,28 @@ class FsaParams(C.Structure):
         return out
 
 
+class FsaObservables(C.Structure):
+    """Mirror of `fsa_observables` (include/fluidsim_amd.h)."""
+
+    _fields_ = [
+        ("fluid_mass", C.c_double),
+        ("fluid_momentum", C.c_double * 3),
+        ("fluid_kinetic_energy", C.c_double),
+        ("fluid_potential_energy", C.c_double),
+        ("front_x", C.c_double),
+        ("solid_momentum", C.c_double * 3),
+        ("gate_tip_displacement", C.c_double * 3),
+        ("gate_tip_count", C.c_int64),
+    ]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
 _P = C.POINTER
 _f = _P(C.c_float)
 _i = _P(C.c_int32)
@@ -119,6 +141,8 @@ DEVICE_API = {
     "fsa_download_fluid_rates": (C.c_int, [_vp, _f, _f]),
     "fsa_get_jump_count": (C.c_int, [_vp, _P(C.c_int64)]),
     "fsa_set_debug": (C.c_int, [_vp, C.c_uint]),
+    "fsa_observe": (C.c_int, [_vp, _P(FsaObservables)]),
+    "fsa_probe_pressure": (C.c_int, [_vp, _f, C.c_int, _f]),
     "fsa_set_timing": (C.c_int, [_vp, C.c_int]),
     "fsa_get_stage_times": (C.c_int, [_vp, _d, _P(C.c_int64)]),
     "fsa_reset_stage_times": (C.c_int, [_vp]),

@@ -46,6 +46,12 @@ struct fsa_sim {
     float4 *rates = nullptr;   // fluid (ax, ay, az, drho) when debugging
     ScanScratch scan;
     unsigned long long *jumps = nullptr;
+    // observables
+    double *obs_partial = nullptr, *obs_out = nullptr;
+    float tip_y = 3.0e38f;  // reference y of the gate tip row (none: above everything)
+    float4 *probe_pts = nullptr;
+    float *probe_out = nullptr;
+    int probe_cap = 0;
 
     bool uploaded[3] = {false, false, false};
     bool hashed = false;  // cell ids/ranks of the current order are fresh
@@ -165,7 +171,8 @@ int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_sta
     if (n == 0) return FSA_OK;
     for (int b = 0; b < 2; ++b) {
         if (b == 0 || double_state) {
-            if ((rc = s->alloc(&k.pos[b], n))) return rc;
+            if ((rc = s->alloc(&k.pos[b], (size_t)n + kPosPad))) return rc;
+            HIP_TRY(hipMemsetAsync(k.pos[b], 0, sizeof(float4) * ((size_t)n + kPosPad), s->stream));
             if ((rc = s->alloc(&k.vel[b], n))) return rc;
         }
         if ((rc = s->alloc(&k.id[b], n))) return rc;
@@ -355,6 +362,8 @@ int fsa_create(const fsa_params *params, void *stream, fsa_sim **out) {
     s->scan.ntiles_cap = (s->P.ncell + kScanTile - 1) / kScanTile;
     if ((rc = s->alloc(&s->scan.tile_sums, (size_t)s->scan.ntiles_cap))) return fail(rc);
     if ((rc = s->alloc(&s->jumps, 1))) return fail(rc);
+    if ((rc = s->alloc(&s->obs_partial, (size_t)observe_partials())) || (rc = s->alloc(&s->obs_out, kObserveSums)))
+        return fail(rc);
     if (hipMemsetAsync(s->jumps, 0, sizeof(unsigned long long), s->stream) != hipSuccess)
         return fail(set_err(FSA_ERR_HIP, "hipMemsetAsync failed"));
     if ((rc = zero_starts(s, s->fluid)) || (rc = zero_starts(s, s->solid)) || (rc = zero_starts(s, s->virt)))
@@ -444,6 +453,11 @@ int fsa_upload_solid(fsa_sim *s, const float *x, const float *u, const float *X,
         HIP_TRY(hipMemcpyAsync(s->sx.np, z4.data(), 16 * (size_t)n, hipMemcpyHostToDevice, s->stream));
         HIP_TRY(hipMemcpyAsync(s->sx.acc, z4.data(), 16 * (size_t)n, hipMemcpyHostToDevice, s->stream));
         HIP_TRY(hipMemcpyAsync(s->sx.couple, z4.data(), 16 * (size_t)n, hipMemcpyHostToDevice, s->stream));
+        // gate tip row: unconstrained solids within dp/2 of the largest reference y
+        float ymax = -3.0e38f;
+        for (int i = 0; i < n; ++i)
+            if (type[i] == FSA_UNCONSTRAINED_SOLID) ymax = std::max(ymax, X[3 * i + 1]);
+        s->tip_y = ymax > -3.0e38f ? ymax - 0.5f * s->params.particle_diameter : 3.0e38f;
         HIP_TRY(hipStreamSynchronize(s->stream));
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -832,6 +846,68 @@ int fsa_reset_stage_times(fsa_sim *s) {
         s->stage_ms[st] = 0.0;
         s->stage_n[st] = 0;
     }
+    return FSA_OK;
+}
+
+static ObserveArgs observe_args(fsa_sim *s) {
+    ObserveArgs a{};
+    a.fpos = s->fluid.pos[s->fluid.st];
+    a.fvel = s->fluid.vel[s->fluid.st];
+    a.fstart = s->fluid.start;
+    a.spos = s->solid.pos[s->solid.st];
+    a.svel = s->solid.vel[s->solid.st];
+    a.sref = s->solid.ref[s->solid.ord];
+    a.tip_y = s->tip_y;
+    return a;
+}
+
+int fsa_observe(fsa_sim *s, fsa_observables *out) {
+    CHECK_SIM(s);
+    if (!out) return set_err(FSA_ERR_INVALID, "null output");
+    int rc = require_uploaded(s);
+    if (rc) return rc;
+    HIP_TRY(launch_observe(s->P, observe_args(s), s->obs_partial, s->obs_out, s->stream));
+    double v[kObserveSums];
+    HIP_TRY(hipMemcpyAsync(v, s->obs_out, sizeof v, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const double dp = s->params.particle_diameter;
+    const double V = dp * dp * dp;
+    const double ms = (double)s->params.solid_reference_density * V;
+    *out = fsa_observables{};
+    out->fluid_mass = v[0] * V;
+    for (int a = 0; a < 3; ++a) {
+        out->fluid_momentum[a] = v[1 + a] * V;
+        out->solid_momentum[a] = v[7 + a] * ms;
+    }
+    out->fluid_kinetic_energy = v[4] * V;
+    out->fluid_potential_energy = v[5] * V;
+    out->front_x = s->P.nf > 0 ? v[6] : 0.0;
+    out->gate_tip_count = (int64_t)v[13];
+    for (int a = 0; a < 3; ++a) out->gate_tip_displacement[a] = v[13] > 0 ? v[10 + a] / v[13] : 0.0;
+    return FSA_OK;
+}
+
+int fsa_probe_pressure(fsa_sim *s, const float *points, int n, float *pressure) {
+    CHECK_SIM(s);
+    if (n < 0 || (n > 0 && (!points || !pressure))) return set_err(FSA_ERR_INVALID, "bad probe arguments");
+    int rc = require_uploaded(s);
+    if (rc) return rc;
+    if (n == 0) return FSA_OK;
+    if ((rc = check_in_grid(s, points, n, "probe"))) return rc;
+    if (n > s->probe_cap) {
+        float4 *p = nullptr;
+        float *o = nullptr;
+        if ((rc = s->alloc(&p, (size_t)n)) || (rc = s->alloc(&o, (size_t)n))) return rc;
+        s->probe_pts = p;  // the old buffers stay owned by the context until fsa_destroy
+        s->probe_out = o;
+        s->probe_cap = n;
+    }
+    std::vector<float4> pts(n);
+    for (int i = 0; i < n; ++i) pts[i] = make_float4(points[3 * i], points[3 * i + 1], points[3 * i + 2], 0.0f);
+    HIP_TRY(hipMemcpyAsync(s->probe_pts, pts.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(launch_probe(s->P, observe_args(s), s->probe_pts, n, s->probe_out, s->stream));
+    HIP_TRY(hipMemcpyAsync(pressure, s->probe_out, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return FSA_OK;
 }
 

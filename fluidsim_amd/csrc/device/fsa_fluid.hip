@@ -93,6 +93,7 @@ __device__ __forceinline__ void one_pair(const DevParams &P, const Self &me, con
 
 constexpr int kPairBatch = 4;  // partner gathers in flight in the pair loop
 constexpr int kScanBatch = 8;  // candidate loads in flight in the test loop
+static_assert(kPosPad >= kScanBatch, "candidate batches may read kScanBatch-1 entries past a range");
 
 // pair terms of list entries [0, cnt), in list order; the partner state of
 // kPairBatch entries is gathered before any of them is used
@@ -104,12 +105,11 @@ __device__ __forceinline__ void flush(const DevParams &P, const Self &me, const 
         float4 pj[kPairBatch], vj[kPairBatch], nj[kPairBatch];
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q) {
-            if (k + q < cnt) {
-                const int j = list[(k + q) * kBlock];
-                pj[q] = pos[j];
-                vj[q] = vel[j];
-                if (KIND == kFluidSolid) nj[q] = np[j];
-            }
+            // entries past cnt re-read the last one (always a valid slot)
+            const int j = list[min(k + q, cnt - 1) * kBlock];
+            pj[q] = pos[j];
+            vj[q] = vel[j];
+            if (KIND == kFluidSolid) nj[q] = np[j];
         }
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q)
@@ -125,19 +125,20 @@ __device__ __forceinline__ void run(const DevParams &P, const Self &me, int b, i
                                     Acc &acc) {
     int cnt = 0;
     for (int j = b; j < e; j += kScanBatch) {
+        // branch-free batch: loads past the run end stay inside the padded
+        // array (kPosPad >= kScanBatch) and are masked out of the test; every
+        // candidate writes list slot cnt, which only a hit keeps
+        // (cnt <= kListCap-1 here: see the flush below)
         float4 q[kScanBatch];
+        const float4 *src = pos + j;
 #pragma unroll
-        for (int k = 0; k < kScanBatch; ++k)
-            if (j + k < e) q[k] = pos[j + k];
+        for (int k = 0; k < kScanBatch; ++k) q[k] = src[k];
 #pragma unroll
         for (int k = 0; k < kScanBatch; ++k) {
-            if (j + k < e) {
-                const float d2 = len2(q[k].x - me.x, q[k].y - me.y, q[k].z - me.z);
-                if (d2 < P.R2) {
-                    list[cnt * kBlock] = j + k;
-                    ++cnt;
-                }
-            }
+            const float d2 = len2(q[k].x - me.x, q[k].y - me.y, q[k].z - me.z);
+            const bool hit = (j + k < e) & (d2 < P.R2);
+            list[cnt * kBlock] = j + k;
+            cnt += hit ? 1 : 0;
         }
         // at most kScanBatch entries are added per round
         if (cnt > kListCap - kScanBatch) {

@@ -28,6 +28,10 @@ namespace fsa {
 
 constexpr int kBlock = 256;      // 4 wave64 per workgroup
 constexpr int kScanTile = 1024;  // elements per scan workgroup (256 threads x 4)
+// position arrays carry kPosPad zeroed entries past the last particle, so a
+// batch of candidate loads may run past the end of a slot range (the
+// results past the range are masked out of every test)
+constexpr int kPosPad = 16;
 
 // the kernels' view of fsa_params
 struct DevParams {
@@ -133,5 +137,18 @@ hipError_t launch_couple(const DevParams &P, const SolidArgs &a, hipStream_t s);
 hipError_t launch_solid_deformation(const DevParams &P, const SolidArgs &a, hipStream_t s);
 hipError_t launch_solid_force(const DevParams &P, const SolidArgs &a, hipStream_t s);
 hipError_t launch_solid_update(const DevParams &P, const SolidArgs &a, hipStream_t s);
+
+// ---- launchers (fsa_observe.hip) -------------------------------------------
+constexpr int kObserveSums = 14;
+struct ObserveArgs {
+    const float4 *fpos, *fvel;
+    const int32_t *fstart;
+    const float4 *spos, *svel, *sref;
+    float tip_y;  // reference y at and above which an unconstrained solid is in the tip row
+};
+int observe_partials();  // doubles of scratch the sums need
+hipError_t launch_observe(const DevParams &P, const ObserveArgs &a, double *partial, double *out, hipStream_t s);
+hipError_t launch_probe(const DevParams &P, const ObserveArgs &a, const float4 *pts, int n, float *out,
+                        hipStream_t s);
 
 }  // namespace fsa

@@ -25,19 +25,24 @@ __global__ __launch_bounds__(kBlock) void k_virt(DevParams P, VirtArgs a) {
     float nx = 0.0f, ny = 0.0f, nz = 0.0f;
     float pressure = 0.0f, density = P.rho0, nearest = kLargeFloat;
     int best = -1;
-    float bdx = 0.0f, bdy = 0.0f, bdz = 0.0f;
 
     for_each_column(P, a.vcell[i], [&](int c0, int c1) {
-        for (int j = a.fstart[c0], e = a.fstart[c1]; j < e; ++j) {
-            const float4 pj = a.fpos[j];
-            const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-            const float d2 = len2(dx, dy, dz);
-            if (d2 < nearest) {
-                nearest = d2;
-                best = j;
-                bdx = dx;
-                bdy = dy;
-                bdz = dz;
+        // nearest fluid particle: strict <, so the first in traversal order
+        // wins ties; kScan loads in flight, reads past the run end stay in
+        // the padded array and are masked
+        constexpr int kScan = 8;
+        static_assert(kPosPad >= kScan, "padded position arrays");
+        for (int j = a.fstart[c0], e = a.fstart[c1]; j < e; j += kScan) {
+            float4 q[kScan];
+            const float4 *src = a.fpos + j;
+#pragma unroll
+            for (int k = 0; k < kScan; ++k) q[k] = src[k];
+#pragma unroll
+            for (int k = 0; k < kScan; ++k) {
+                const float d2 = len2(q[k].x - xi, q[k].y - yi, q[k].z - zi);
+                const bool better = (j + k < e) & (d2 < nearest);
+                nearest = better ? d2 : nearest;
+                best = better ? j + k : best;
             }
         }
         if (NORMALS) {
@@ -58,6 +63,7 @@ __global__ __launch_bounds__(kBlock) void k_virt(DevParams P, VirtArgs a) {
     if (best >= 0) {
         const float4 pj = a.fpos[best];
         const float4 vj = a.fvel[best];
+        const float bdx = pj.x - xi, bdy = pj.y - yi, bdz = pj.z - zi;
         density = pj.w;
         pressure = vj.w - density * (P.gx * bdx + P.gy * bdy + P.gz * bdz);
         ux = vj.x;

@@ -154,6 +154,17 @@ class DeviceSim:
         check(self.lib.fsa_get_jump_count(self.handle, C.byref(v)))
         return int(v.value)
 
+    def observe(self) -> dict:
+        o = _native.FsaObservables()
+        check(self.lib.fsa_observe(self.handle, C.byref(o)))
+        return o.as_dict()
+
+    def probe_pressure(self, points) -> np.ndarray:
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        out = np.zeros(len(pts), np.float32)
+        check(self.lib.fsa_probe_pressure(self.handle, _ptr(pts), len(pts), _ptr(out)))
+        return out
+
     def set_debug(self, flags: int):
         check(self.lib.fsa_set_debug(self.handle, flags))
 

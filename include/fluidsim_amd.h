@@ -200,6 +200,30 @@ int fsa_get_jump_count(fsa_sim *sim, int64_t *count);
 #define FSA_DEBUG_PAIR_SUMS 1u
 int fsa_set_debug(fsa_sim *sim, unsigned flags);
 
+/* ---- run-level observables ----------------------------------------------
+ * The reference has no probes (src/probe.cu is commented out) and no
+ * conservation diagnostics; these are computed on the device from the
+ * current state.  Sums are accumulated in FP64.  V = particle_diameter^3. */
+typedef struct fsa_observables {
+    double fluid_mass;                  /* sum rho V */
+    double fluid_momentum[3];           /* sum rho V u */
+    double fluid_kinetic_energy;        /* sum rho V |u|^2 / 2 */
+    double fluid_potential_energy;      /* sum rho V (-g . x) */
+    double front_x;                     /* largest fluid x (the surge front along the flow axis) */
+    double solid_momentum[3];           /* sum rho_s V u over solids */
+    double gate_tip_displacement[3];    /* mean x - X over the gate tip row */
+    int64_t gate_tip_count;             /* solids in the tip row: unconstrained solids whose
+                                           reference y is within dp/2 of the largest one */
+} fsa_observables;
+
+int fsa_observe(fsa_sim *sim, fsa_observables *out);
+
+/* Shepard-interpolated fluid pressure at n points (x,y,z triples), with the
+ * weights of solidPressureCalculate (src/solid.cu:275-291):
+ * p = sum p_j w_j / sum w_j, w = (exp(-q^2) - c1) c2 over fluid within K h;
+ * 0 where the sum of weights is <= 1e-6.  Points must lie in the cell grid. */
+int fsa_probe_pressure(fsa_sim *sim, const float *points, int n, float *pressure);
+
 /* ---- timing ------------------------------------------------------------ */
 
 enum fsa_stage {

@@ -96,3 +96,46 @@ def same_class(got, o32, o64, scale: float, factor: float = 4.0, rel: float = RE
     e_got = max_abs(got, o64)
     e_32 = max_abs(o32, o64)
     return e_got, e_32, e_got <= factor * e_32 + rel * scale
+
+
+def observables_from_state(params, fluid: dict, solid: dict, solid_ref, solid_type) -> dict:
+    """The quantities of fsa_observe, from a downloaded state, in float64."""
+    dp = float(params.particle_diameter)
+    V = dp ** 3
+    rho = np.asarray(fluid["density"], np.float64)
+    u = np.asarray(fluid["velocity"], np.float64)
+    x = np.asarray(fluid["position"], np.float64)
+    g = np.array(params.gravity, np.float64)
+    out = {
+        "fluid_mass": rho.sum() * V,
+        "fluid_momentum": list((rho[:, None] * u).sum(axis=0) * V),
+        "fluid_kinetic_energy": 0.5 * (rho * (u * u).sum(axis=1)).sum() * V,
+        "fluid_potential_energy": -(rho * (x @ g)).sum() * V,
+        "front_x": float(x[:, 0].max()) if len(x) else 0.0,
+    }
+    ns = len(solid_type)
+    if ns:
+        us = np.asarray(solid["velocity"], np.float64)
+        out["solid_momentum"] = list(us.sum(axis=0) * float(params.solid_reference_density) * V)
+        X = np.asarray(solid_ref, np.float64)
+        free = np.asarray(solid_type) == 0
+        ytop = X[free, 1].max()
+        tip = free & (X[:, 1].astype(np.float32) >= np.float32(np.float32(ytop) - np.float32(0.5) * np.float32(dp)))
+        xs = np.asarray(solid["position"], np.float64)
+        out["gate_tip_displacement"] = list((xs[tip] - X[tip]).mean(axis=0))
+        out["gate_tip_count"] = int(tip.sum())
+    return out
+
+
+def shepard_probe(params, fluid: dict, points) -> np.ndarray:
+    """Brute-force Shepard pressure over all fluid particles (solid.cu:275-291 weights)."""
+    x = np.asarray(fluid["position"], np.float64)
+    p = np.asarray(fluid["pressure"], np.float64)
+    out = []
+    for q in np.asarray(points, np.float64):
+        d2 = ((x - q) ** 2).sum(axis=1)
+        m = d2 < params.impact_distance_square
+        w = (np.exp(-d2[m] * params.smoothing_length_inverse_square) - params.kernel_coefficient_1) \
+            * params.kernel_coefficient_2
+        out.append((p[m] * w).sum() / w.sum() if w.sum() > 1e-6 else 0.0)
+    return np.array(out)

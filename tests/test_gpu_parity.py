@@ -105,3 +105,176 @@ def test_multi_step_state(gpu_available, tiny, steps):
     L = max(tiny.tank) * tiny.dp
     assert max_abs(gs["position"], os_["position"]) <= REL * L
     assert dev.jump_count() == 0
+
+
+# ---- golden fixtures ---------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["tiny_seed7", "tiny_inviscid_seed8"])
+def test_gpu_matches_golden_fixtures(gpu_available, name):
+    from test_golden import inputs, load, state
+    sc, d = load(name)
+    a = inputs(d)
+    p = sc.params((len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])))
+    dev = DeviceSim(p)
+    dev.upload(a)
+    done = 0
+    for s in (1, 10):
+        print(f"case golden {name} step {s}", flush=True)
+        dev.step(s - done)
+        done = s
+        assert_within(compare_fluid(sc, dev.fluid(), state(d, s, "fluid")), f"{name} step {s}")
+        L = max(sc.tank) * sc.dp
+        assert max_abs(dev.solid()["position"], state(d, s, "solid")["position"]) <= REL * L
+
+
+# ---- the CPU-scale scene: 43,750 fluid + 2,000 solid --------------------------
+
+@pytest.fixture(scope="module")
+def c1_pair():
+    sc = scenes.c1_cpu_scene(seed=2)
+    a = sc.arrays()
+    p = sc.params((len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])))
+    orc.set_threads(16)
+    return sc, a, p
+
+
+@pytest.mark.parametrize("steps", [1, 100])
+def test_c1_scene_parity(gpu_available, c1_pair, steps):
+    sc, a, p = c1_pair
+    print(f"case c1 steps={steps} fluid={p.n_fluid} solid={p.n_solid} walls={p.n_virt}", flush=True)
+    dev = DeviceSim(p)
+    dev.upload(a)
+    o = orc.Oracle(p, 32)
+    o.upload(a)
+    dev.step(steps)
+    o.step(steps)
+    rep = compare_fluid(sc, dev.fluid(), o.fluid())
+    print({k: v[0] for k, v in rep.items()}, flush=True)
+    assert_within(rep, f"C1 fluid after {steps} steps")
+    L = max(sc.tank) * sc.dp
+    assert max_abs(dev.solid()["position"], o.solid()["position"]) <= REL * L
+    for kind in (FLUID, SOLID):
+        np.testing.assert_array_equal(dev.cell_ids(kind), o.cell_ids(kind))
+
+
+# ---- full size: 4,000,000 fluid + 27,000 solid, size-independent checks ------
+
+def test_showcase_scale_integer_invariants(gpu_available):
+    sc = scenes.c2_showcase_scene()
+    a = sc.arrays()
+    p = sc.params((len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])))
+    print(f"case c2 fluid={p.n_fluid} solid={p.n_solid} walls={p.n_virt}", flush=True)
+    dev = DeviceSim(p)
+    dev.upload(a)
+    dev.step(10)
+    dev.particle_cell_update()  # hash the post-step positions, as the next step would
+    dev.fill_cell()
+    f = dev.fluid()
+    cmin = np.array(p.cell_min, np.float32)
+    inv = np.array(p.interval_inv, np.float32)
+    n = np.array(p.cell_number)
+    for kind, x in ((FLUID, f["position"]), (SOLID, dev.solid()["position"])):
+        idx = ((x.astype(np.float32) - cmin) * inv).astype(np.int64)
+        expect = (idx[:, 0] * n[1] + idx[:, 1]) * n[2] + idx[:, 2]
+        cells = dev.cell_ids(kind)
+        np.testing.assert_array_equal(cells, expect)
+        order = dev.sorted_ids(kind)
+        np.testing.assert_array_equal(order, np.lexsort((np.arange(len(cells)), cells)))
+        start = dev.cell_start(kind)
+        np.testing.assert_array_equal(np.diff(start), np.bincount(cells, minlength=p.cell_number_total))
+    assert np.isfinite(f["velocity"]).all() and np.isfinite(f["pressure"]).all()
+    assert f["density"].min() >= p.fluid_min_density and f["density"].max() <= p.fluid_max_density
+    lo, hi = np.array(p.domain_min), np.array(p.domain_max)
+    assert (f["position"] >= lo).all() and (f["position"] <= hi).all()
+    assert dev.jump_count() == 0
+
+
+# ---- edge cases --------------------------------------------------------------
+
+@pytest.mark.parametrize("variant", ["no_gate", "no_walls", "no_fluid", "single_fluid"])
+def test_missing_kinds(gpu_available, variant):
+    from helpers import custom_case
+    base = scenes.tiny_scene(seed=4)
+    if variant == "no_gate":
+        sc = base.with_(gate=None)
+        a = sc.arrays()
+    elif variant == "no_walls":
+        sc = base.with_(wall_layers=0)
+        a = sc.arrays()
+    elif variant == "no_fluid":
+        sc = base.with_(fluid=(0, 0, 0))
+        a = sc.arrays()
+    else:
+        sc = base
+        _, a = custom_case(sc, fluid={"position": [[0.05, 0.05, 0.03]], "velocity": [[0.1, 0.0, 0.0]]})
+    p = sc.params((len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])))
+    print(f"case {variant}: {p.n_fluid} {p.n_solid} {p.n_virt}", flush=True)
+    dev = DeviceSim(p)
+    dev.upload(a)
+    o = orc.Oracle(p, 32)
+    o.upload(a)
+    dev.step(3)
+    o.step(3)
+    if p.n_fluid:
+        assert_within(compare_fluid(sc, dev.fluid(), o.fluid()), variant)
+    if p.n_solid:
+        L = max(sc.tank) * sc.dp
+        assert max_abs(dev.solid()["position"], o.solid()["position"]) <= REL * L
+
+
+def test_teleported_particle_is_kept_and_counted(gpu_available, tiny):
+    a = tiny.arrays()
+    p = tiny.params((len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])))
+    a["fluid_velocity"][0] = [3.0 * p.interval[0] / p.dt, 0.0, 0.0]
+    dev = DeviceSim(p)
+    dev.upload(a)
+    o = orc.Oracle(p, 32)
+    o.upload(a)
+    dev.step(2)
+    o.step(2)
+    assert dev.jump_count() == 1
+    assert_within(compare_fluid(tiny, dev.fluid(), o.fluid()), "after a jump")
+    ids = dev.sorted_ids(FLUID)
+    assert sorted(ids.tolist()) == list(range(p.n_fluid))
+
+
+def test_nan_pressure_is_scrubbed_like_the_oracle(gpu_available, tiny):
+    a = tiny.arrays()
+    p = tiny.params((len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])))
+    a["fluid_pressure"][5] = np.nan
+    dev = DeviceSim(p)
+    dev.upload(a)
+    o = orc.Oracle(p, 32)
+    o.upload(a)
+    dev.particle_cell_update(); dev.fill_cell(); dev.couple_calculation(); dev.virt_update()
+    dev.fluid_calculation()
+    o.particle_cell_update(); o.fill_cell(); o.couple_calculation(); o.virt_update()
+    o.fluid_calculation()
+    g, r = dev.fluid(), o.fluid()
+    assert np.isfinite(g["position"]).all() and np.isfinite(g["density"]).all()
+    assert_within(compare_fluid(tiny, g, r), "NaN neighbour")
+
+
+def test_errors_are_status_codes(gpu_available, tiny):
+    from fluidsim_amd._native import FsaError, FSA_ERR_STATE, FSA_ERR_INVALID
+    a = tiny.arrays()
+    p = tiny.params((len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])))
+    dev = DeviceSim(p)
+    with pytest.raises(FsaError) as e:
+        dev.fluid_calculation()  # nothing uploaded
+    assert e.value.code == FSA_ERR_STATE
+    dev.upload(a)
+    with pytest.raises(FsaError) as e:
+        dev.fill_cell()  # no hash of the current order
+    assert e.value.code == FSA_ERR_STATE
+    bad = {k: v.copy() for k, v in a.items()}
+    bad["fluid_position"][3] = [-1.0, 0.0, 0.0]
+    with pytest.raises(FsaError) as e:
+        dev.upload(bad)
+    assert e.value.code == FSA_ERR_INVALID
+    dev.upload(a)  # a good upload resets the context
+    dev.step(1)
+    dev.set_timing(True)
+    dev.step(2)
+    t = dev.stage_times()
+    assert t["fluid"][1] == 2 and t["fluid"][0] > 0
