@@ -117,42 +117,57 @@ __device__ __forceinline__ void flush(const DevParams &P, const Self &me, const 
     }
 }
 
-// candidate test over slots [b, e) of one kind, then the pair terms of the
-// hits; kScanBatch candidate loads are issued before the first test
+// candidate test over slots [b, e) of one kind, appending hits to the list.
+// Control flow is uniform across the wave: the batch loop runs while any
+// lane still has candidates (a lane past its run re-tests the padded tail
+// at e, masked out), and the list is evaluated by all lanes together as soon
+// as any lane is nearly full, so the pair loop runs over dense batches.
+// kScanBatch candidate loads are issued before the first test.
 template <int KIND>
-__device__ __forceinline__ void run(const DevParams &P, const Self &me, int b, int e, const float4 *__restrict__ pos,
-                                    const float4 *__restrict__ vel, const float4 *__restrict__ np, int *list,
-                                    Acc &acc) {
-    int cnt = 0;
-    for (int j = b; j < e; j += kScanBatch) {
-        // branch-free batch: loads past the run end stay inside the padded
-        // array (kPosPad >= kScanBatch) and are masked out of the test; every
-        // candidate writes list slot cnt, which only a hit keeps
-        // (cnt <= kListCap-1 here: see the flush below)
+__device__ __forceinline__ void scan(const DevParams &P, const Self &me, int b, int e, const float4 *__restrict__ pos,
+                                     const float4 *__restrict__ vel, const float4 *__restrict__ np, int *list,
+                                     int &cnt, Acc &acc) {
+    for (int j = b; __ballot(j < e) != 0; j += kScanBatch) {
+        // branch-free batch: loads at or past the run end stay inside the
+        // padded array (e <= n, kPosPad >= kScanBatch) and are masked out of
+        // the test; every candidate writes list slot cnt, which only a hit
+        // keeps (cnt <= kListCap-1 here: see the flush below)
+        const int jj = min(j, e);
         float4 q[kScanBatch];
-        const float4 *src = pos + j;
+        const float4 *src = pos + jj;
 #pragma unroll
         for (int k = 0; k < kScanBatch; ++k) q[k] = src[k];
 #pragma unroll
         for (int k = 0; k < kScanBatch; ++k) {
             const float d2 = len2(q[k].x - me.x, q[k].y - me.y, q[k].z - me.z);
-            const bool hit = (j + k < e) & (d2 < P.R2);
-            list[cnt * kBlock] = j + k;
+            const bool hit = (jj + k < e) & (d2 < P.R2);
+            list[cnt * kBlock] = jj + k;
             cnt += hit ? 1 : 0;
         }
         // at most kScanBatch entries are added per round
-        if (cnt > kListCap - kScanBatch) {
+        if (__ballot(cnt > kListCap - kScanBatch) != 0) {
             flush<KIND>(P, me, pos, vel, np, list, cnt, acc);
             cnt = 0;
         }
     }
+}
+
+// all partners of one kind: the 9 columns in order, then the list remainder
+template <int KIND>
+__device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cell, const int32_t *__restrict__ start,
+                                     const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                                     const float4 *__restrict__ np, int *list, Acc &acc) {
+    int cnt = 0;
+    for_each_column(P, cell, [&](int c0, int c1) {
+        scan<KIND>(P, me, start[c0], start[c1], pos, vel, np, list, cnt, acc);
+    });
     flush<KIND>(P, me, pos, vel, np, list, cnt, acc);
 }
 
 template <bool RATES>
 __global__ __launch_bounds__(kBlock) void k_fluid(DevParams P, FluidStepArgs a) {
     __shared__ int s_list[kListCap * kBlock];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (i >= P.nf) return;  // lists are lane-private: no barrier follows
     int *list = s_list + threadIdx.x;
 
@@ -171,11 +186,11 @@ __global__ __launch_bounds__(kBlock) void k_fluid(DevParams P, FluidStepArgs a) 
     me.two_rho = 2.0f * me.rho;
     Acc acc{0.0f, 0.0f, 0.0f, 0.0f};
 
-    for_each_column(P, a.fcell[i], [&](int c0, int c1) {
-        run<kFluidFluid>(P, me, a.fstart[c0], a.fstart[c1], a.fpos, a.fvel, nullptr, list, acc);
-        run<kFluidSolid>(P, me, a.sstart[c0], a.sstart[c1], a.spos, a.svel, a.snp, list, acc);
-        run<kFluidVirt>(P, me, a.vstart[c0], a.vstart[c1], a.vpos, a.vvel, nullptr, list, acc);
-    });
+    // partners kind by kind: fluid, solid, wall (the canonical order)
+    const int cell = a.fcell[i];
+    pass<kFluidFluid>(P, me, cell, a.fstart, a.fpos, a.fvel, nullptr, list, acc);
+    pass<kFluidSolid>(P, me, cell, a.sstart, a.spos, a.svel, a.snp, list, acc);
+    pass<kFluidVirt>(P, me, cell, a.vstart, a.vpos, a.vvel, nullptr, list, acc);
 
     // fluidUpdate (fluid.cu:234-269): NaN scrub, explicit step with the old
     // velocity for the position, density clamp, EOS, domain clamp, speed cap

@@ -30,6 +30,16 @@ __device__ __forceinline__ int find_cell(const DevParams &P, float x, float y, f
     return (ix * P.cn[1] + iy) * P.cn[2] + iz;
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8), and each XCD has its own L2.  Renumbering so that XCD x walks one
+// contiguous range of tiles keeps neighbouring cell columns in one L2.
+// A bijection of [0, nb).
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    constexpr int kXcd = 8;
+    const int x = b % kXcd, q = nb / kXcd, r = nb % kXcd;
+    return x * q + min(x, r) + b / kXcd;
+}
+
 // The 27 neighbour cells of cell c as 9 columns (ox, oy ascending), each
 // the contiguous z-run cz-1..cz+1 clipped to the grid: `body(c0, c1)` gets
 // the first cell and one past the last cell of each column.

@@ -6,10 +6,10 @@
 // Templated on the arithmetic type T (float = the reference's FP32, double =
 // the FP64 precision oracle) and on the neighbour traversal:
 //   CANONICAL  cells sorted by (cell id, .dat index); for a particle in cell
-//              (cx,cy,cz) the 27 neighbour cells are walked as 9 columns
+//              (cx,cy,cz) the partners are walked kind by kind (fluid, solid,
+//              virt), and for each kind the 27 neighbour cells as 9 columns
 //              (ox,oy ascending), each column as the contiguous z-run
-//              cz-1..cz+1, kinds in the order fluid, solid, virt.  This is
-//              the order the HIP kernels use.
+//              cz-1..cz+1.  This is the order the HIP kernels use.
 //   EXACT      the reference's per-cell singly linked lists: host build of
 //              src/sim.cu:256-302 (append, ascending index), per-step rebuild
 //              of src/pair.cu:72-207 (walk previous lists of the 27
@@ -333,15 +333,15 @@ template <class T> struct Oracle : OracleBase {
             const int ny = P.cn[1], nz = P.cn[2];
             const int cx = c / (ny * nz), cy = (c / nz) % ny, cz = c % nz;
             const int zlo = std::max(cz - 1, 0), zhi = std::min(cz + 1, nz - 1);
-            for (int ox = -1; ox <= 1; ++ox) {
-                const int gx = cx + ox;
-                if (gx < 0 || gx >= P.cn[0]) continue;
-                for (int oy = -1; oy <= 1; ++oy) {
-                    const int gy = cy + oy;
-                    if (gy < 0 || gy >= ny) continue;
-                    const int base = (gx * ny + gy) * nz;
-                    for (int k = 0; k < 3; ++k) {
-                        if (!(mask & (1u << k))) continue;
+            for (int k = 0; k < 3; ++k) {
+                if (!(mask & (1u << k))) continue;
+                for (int ox = -1; ox <= 1; ++ox) {
+                    const int gx = cx + ox;
+                    if (gx < 0 || gx >= P.cn[0]) continue;
+                    for (int oy = -1; oy <= 1; ++oy) {
+                        const int gy = cy + oy;
+                        if (gy < 0 || gy >= ny) continue;
+                        const int base = (gx * ny + gy) * nz;
                         const int b = start[k][base + zlo], e = start[k][base + zhi + 1];
                         for (int t = b; t < e; ++t) f(k, sorted[k][t]);
                     }

@@ -75,5 +75,6 @@ def test_driver_true_pressure_key(gpu_available, tmp_path):
     sc = scenes.tiny_scene(seed=9).with_(iend=1, result_interval=1)
     root = sc.write_case(tmp_path / "case", extra={"output_fluid_pressure": "true"})
     assert _native.host_lib().fsh_run(str(root).encode(), 0) == 0
-    _, _, p = read_vtk(root / "out" / "fluid" / "fluid_pressure-1.vtk")
-    assert np.abs(p).max() > 0
+    _, _, p0 = read_vtk(root / "out" / "fluid" / "fluid_pressure-1.vtk")  # start of step 1: p = 0 as uploaded
+    _, _, p = read_vtk(root / "out" / "fluid" / "fluid_pressure-2.vtk")   # after the step
+    assert np.all(p0 == 0) and np.abs(p).max() > 0
