@@ -67,6 +67,22 @@ def active_walls(params, fluid_pos: np.ndarray, virt_pos: np.ndarray) -> int:
     return int(dil[vc[:, 0], vc[:, 1], vc[:, 2]].sum())
 
 
+def measured_traffic(scene: str):
+    """HBM bytes per k_fluid launch from the committed rocprofv3 --pmc summary of this
+    workload (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled by what the
+    counter reads for a plain copy of known size); None when there is none."""
+    path = ROOT / "profiles" / f"k_fluid_{scene}_pmc.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic:"):
+            kv = dict(tok.split("=") for tok in line.split()[1:])
+            b = (float(kv["fetch_kib"]) * float(kv["fetch_scale"]) +
+                 float(kv["write_kib"]) * float(kv["write_scale"])) * 1024
+            return int(b)
+    return None
+
+
 def sync_all(dev):
     dev.synchronize()
     try:
@@ -182,17 +198,33 @@ def main():
     fluid_avg_s = fluid_ms / max(fluid_n, 1) / 1e3
     bytes_per_launch = 68 * nf + 32 * n_active + 40 * ns + 8 * params.cell_number_total
     achieved = bytes_per_launch / fluid_avg_s / 1e9
+    # the pass is VALU-bound: price it in flops too (8 per candidate test,
+    # 90 per in-range Riemann pair) from an exact census of this state
+    census = dev.pair_census()
+    cand = sum(census["candidates"].values())
+    pairs = sum(census["pairs"].values())
+    flops = 8 * cand + 90 * pairs
+    tflops = flops / fluid_avg_s / 1e12
+    traffic = measured_traffic(args.scene)
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": None,
+        "traffic": traffic,
         "kernel": "k_fluid (fused pair sums + update)",
         "kernel_avg_ms": round(fluid_avg_s * 1e3, 5),
         "algorithmic_bytes_per_launch": int(bytes_per_launch),
         "active_walls": n_active,
+        "valu": {
+            "achieved_tflops": round(tflops, 3),
+            "peak_tflops": FP32_PEAK_TFLOPS,
+            "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
+            "flops_per_launch": int(flops),
+            "candidates_per_fluid_particle": round(cand / nf, 2),
+            "pairs_per_fluid_particle": round(pairs / nf, 2),
+        },
     }
 
     result = {

@@ -143,6 +143,7 @@ DEVICE_API = {
     "fsa_set_debug": (C.c_int, [_vp, C.c_uint]),
     "fsa_observe": (C.c_int, [_vp, _P(FsaObservables)]),
     "fsa_probe_pressure": (C.c_int, [_vp, _f, C.c_int, _f]),
+    "fsa_pair_census": (C.c_int, [_vp, _P(C.c_int64)]),
     "fsa_set_timing": (C.c_int, [_vp, C.c_int]),
     "fsa_get_stage_times": (C.c_int, [_vp, _d, _P(C.c_int64)]),
     "fsa_reset_stage_times": (C.c_int, [_vp]),

@@ -49,6 +49,7 @@ struct fsa_sim {
     // observables
     double *obs_partial = nullptr, *obs_out = nullptr;
     float tip_y = 3.0e38f;  // reference y of the gate tip row (none: above everything)
+    unsigned long long *census = nullptr;
     float4 *probe_pts = nullptr;
     float *probe_out = nullptr;
     int probe_cap = 0;
@@ -884,6 +885,30 @@ int fsa_observe(fsa_sim *s, fsa_observables *out) {
     out->front_x = s->P.nf > 0 ? v[6] : 0.0;
     out->gate_tip_count = (int64_t)v[13];
     for (int a = 0; a < 3; ++a) out->gate_tip_displacement[a] = v[13] > 0 ? v[10 + a] / v[13] : 0.0;
+    return FSA_OK;
+}
+
+int fsa_pair_census(fsa_sim *s, int64_t counts[6]) {
+    CHECK_SIM(s);
+    if (!counts) return set_err(FSA_ERR_INVALID, "null output");
+    int rc = require_uploaded(s);
+    if (rc) return rc;
+    if (!s->census) {
+        if ((rc = s->alloc(&s->census, 6))) return rc;
+    }
+    CensusArgs a{};
+    a.fpos = s->fluid.pos[s->fluid.st];
+    a.spos = s->solid.pos[s->solid.st];
+    a.vpos = s->virt.pos[0];
+    a.fcell = s->fluid.cell[s->fluid.ord];
+    a.fstart = s->fluid.start;
+    a.sstart = s->solid.start;
+    a.vstart = s->virt.start;
+    HIP_TRY(launch_census(s->P, a, s->census, s->stream));
+    unsigned long long v[6];
+    HIP_TRY(hipMemcpyAsync(v, s->census, sizeof v, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < 6; ++k) counts[k] = (int64_t)v[k];
     return FSA_OK;
 }
 

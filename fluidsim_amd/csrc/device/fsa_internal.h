@@ -146,6 +146,12 @@ struct ObserveArgs {
     const float4 *spos, *svel, *sref;
     float tip_y;  // reference y at and above which an unconstrained solid is in the tip row
 };
+struct CensusArgs {
+    const float4 *fpos, *spos, *vpos;
+    const int32_t *fcell, *fstart, *sstart, *vstart;
+};
+__global__ void k_fill_u64(unsigned long long *p, int n);
+hipError_t launch_census(const DevParams &P, const CensusArgs &a, unsigned long long *out, hipStream_t s);
 int observe_partials();  // doubles of scratch the sums need
 hipError_t launch_observe(const DevParams &P, const ObserveArgs &a, double *partial, double *out, hipStream_t s);
 hipError_t launch_probe(const DevParams &P, const ObserveArgs &a, const float4 *pts, int n, float *out,

@@ -13,6 +13,14 @@ namespace {
 
 constexpr int kSumBlocks = 256;
 
+}  // namespace
+
+__global__ void k_fill_u64(unsigned long long *p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+namespace {
+
 template <int N>
 __device__ __forceinline__ void block_reduce(double (&v)[N], double (*sh)[kBlock]) {
 #pragma unroll
@@ -125,6 +133,51 @@ hipError_t launch_probe(const DevParams &P, const ObserveArgs &a, const float4 *
                         hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_probe, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, a, pts, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace fsa
+
+namespace fsa {
+
+namespace {
+
+// candidates and cut-off hits of the fluid pass, per partner kind
+__global__ __launch_bounds__(kBlock) void k_census(DevParams P, CensusArgs a, unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long sh[6];
+    if (threadIdx.x < 6) sh[threadIdx.x] = 0ull;
+    __syncthreads();
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < P.nf) {
+        const float4 xi = a.fpos[i];
+        unsigned long long cand[3] = {0, 0, 0}, hit[3] = {0, 0, 0};
+        const int32_t *starts[3] = {a.fstart, a.sstart, a.vstart};
+        const float4 *pos[3] = {a.fpos, a.spos, a.vpos};
+        for_each_column(P, a.fcell[i], [&](int c0, int c1) {
+            for (int k = 0; k < 3; ++k) {
+                const int b = starts[k][c0], e = starts[k][c1];
+                cand[k] += (unsigned long long)(e - b);
+                for (int j = b; j < e; ++j) {
+                    const float4 pj = pos[k][j];
+                    hit[k] += len2(pj.x - xi.x, pj.y - xi.y, pj.z - xi.z) < P.R2 ? 1ull : 0ull;
+                }
+            }
+        });
+        for (int k = 0; k < 3; ++k) {
+            atomicAdd(&sh[k], cand[k]);
+            atomicAdd(&sh[3 + k], hit[k]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) atomicAdd(&out[threadIdx.x], sh[threadIdx.x]);
+}
+
+}  // namespace
+
+hipError_t launch_census(const DevParams &P, const CensusArgs &a, unsigned long long *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, out, 6);
+    if (P.nf > 0)
+        hipLaunchKernelGGL(k_census, dim3((P.nf + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, a, out);
     return hipGetLastError();
 }
 

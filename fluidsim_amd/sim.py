@@ -165,6 +165,12 @@ class DeviceSim:
         check(self.lib.fsa_probe_pressure(self.handle, _ptr(pts), len(pts), _ptr(out)))
         return out
 
+    def pair_census(self) -> dict:
+        c = (C.c_int64 * 6)()
+        check(self.lib.fsa_pair_census(self.handle, c))
+        kinds = ("fluid", "solid", "virt")
+        return {"candidates": dict(zip(kinds, c[:3])), "pairs": dict(zip(kinds, c[3:]))}
+
     def set_debug(self, flags: int):
         check(self.lib.fsa_set_debug(self.handle, flags))
 

@@ -224,6 +224,12 @@ int fsa_observe(fsa_sim *sim, fsa_observables *out);
  * 0 where the sum of weights is <= 1e-6.  Points must lie in the cell grid. */
 int fsa_probe_pressure(fsa_sim *sim, const float *points, int n, float *pressure);
 
+/* Work census of the fluid pass on the current state: counts[k] = candidate
+ * tests and counts[3+k] = partners inside the cut-off, for partner kind k
+ * (fluid, solid, virt), summed over all fluid particles (self included).
+ * Used to price the pass in flops; the fluid pass itself is not run. */
+int fsa_pair_census(fsa_sim *sim, int64_t counts[6]);
+
 /* ---- timing ------------------------------------------------------------ */
 
 enum fsa_stage {
