@@ -148,7 +148,14 @@ DEVICE_API = {
     "fsa_get_stage_times": (C.c_int, [_vp, _d, _P(C.c_int64)]),
     "fsa_reset_stage_times": (C.c_int, [_vp]),
     "fsa_memory_used": (C.c_int, [_vp, _P(C.c_int64)]),
+    "fsa_batch_create": (C.c_int, [_vp, C.c_uint, _P(_vp)]),
+    "fsa_batch_destroy": (C.c_int, [_vp]),
+    "fsa_batch_step": (C.c_int, [_vp, _P(_vp), C.c_int, C.c_int]),
+    "fsa_batch_synchronize": (C.c_int, [_vp]),
+    "fsa_batch_stats": (C.c_int, [_vp, _P(C.c_int64), _P(C.c_int64)]),
 }
+
+FSA_BATCH_NO_GRAPH = 1
 
 HOST_API = {
     "fsh_last_error": (C.c_char_p, []),

@@ -1,0 +1,409 @@
+// fsa_batch.hip -- many scenes advanced by one launch per stage.
+//
+// The reference runs one scene per process (process-global parameter and
+// pointer blocks, src/sim.cu:9-10, src/memory_copy.cu:5-6) and its step is a
+// fixed sequence of stages (src/calculation.cu:23-33).  A batch keeps that
+// sequence but launches every stage once for all of its scenes: the grid of
+// a launch is the concatenation of the scenes' grids, each scene's range
+// rounded up to whole XCD rounds (8 blocks), and a block finds its scene by a
+// binary search in the launch's prefix table (k_many).  Each scene runs the
+// same operation code as when it is stepped alone, so results are
+// bit-identical to fsa_step on each scene.
+//
+// Scene-dependent launch arguments change with the double-buffer indices,
+// which repeat every two steps (the fluid order buffer and the solid state
+// and order buffers flip once per step; the fluid state buffer flips twice).
+// A plan therefore holds the job and prefix tables of two consecutive steps
+// in device memory, each recorded as one hipGraph, and a step is one graph
+// launch.  The plan is keyed on the scene list (context serials) and on
+// every scene's buffer indices; any mismatch -- another scene list, or a
+// scene stepped alone in between -- rebuilds it.
+//
+// Solid sub-steps: the scenes' solid_sub_step counts differ (a stiffer gate
+// needs more).  Sub-step k launches the three solid stages over the scenes
+// with more than k sub-steps; the others get empty ranges.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "fluidsim_amd.h"
+#include "fsa_context.h"
+#include "fsa_internal.h"
+
+using namespace fsa;
+
+#define HIP_TRY(expr) FSA_HIP_TRY(expr)
+
+namespace {
+
+constexpr int kXcdRound = 8;  // blocks per round of the XCD dispatcher
+constexpr size_t kAlign = 256;
+
+inline int round_xcd(int blocks) { return (blocks + kXcdRound - 1) / kXcdRound * kXcdRound; }
+inline size_t align_up(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
+
+// indices of the double buffers of one scene at the start of a step
+struct Phase {
+    int fst, ford, sst, sord;
+    bool operator==(const Phase &o) const {
+        return fst == o.fst && ford == o.ford && sst == o.sst && sord == o.sord;
+    }
+};
+Phase phase_of(const fsa_sim *s) { return {s->fluid.st, s->fluid.ord, s->solid.st, s->solid.ord}; }
+// the indices one step later: the sort flips state and order of every kind
+// with particles, the fluid pass flips the fluid state back
+Phase next_phase(const fsa_sim *s, Phase p) {
+    if (s->P.nf > 0) p.ford ^= 1;
+    if (s->P.ns > 0) {
+        p.sst ^= 1;
+        p.sord ^= 1;
+    }
+    return p;
+}
+void apply_phase(fsa_sim *s, const Phase &p) {
+    s->fluid.st = p.fst;
+    s->fluid.ord = p.ford;
+    s->solid.st = p.sst;
+    s->solid.ord = p.sord;
+}
+
+// one launch of a step: which op, which job table, its prefix table
+struct Launch {
+    enum Family { kCells, kVirt, kFluid, kSolid } family;
+    BatchOp op;
+    int njobs;
+    size_t prefix_off;  // byte offset of the prefix table in the blob
+    int nblocks;
+};
+
+// the launches and tables of one step
+struct StepTables {
+    size_t cells_off = 0, virt_off = 0, fluid_off = 0, solid_off = 0;
+    int ncells = 0, nscenes = 0;
+    std::vector<Launch> launches;
+};
+
+}  // namespace
+
+struct fsa_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    unsigned flags = 0;
+    hipEvent_t ev_sync = nullptr;
+
+    // the plan
+    std::vector<uint64_t> serials;
+    std::vector<Phase> phase0;  // buffer indices at the start of step 0 of the plan
+    StepTables tables[2];
+    char *dev_blob = nullptr;
+    size_t blob_cap = 0;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    bool planned = false;
+
+    int64_t plans_built = 0;
+    int64_t steps_run = 0;
+};
+
+namespace {
+
+void drop_graphs(fsa_batch *b) {
+    for (int p = 0; p < 2; ++p) {
+        if (b->exec[p]) (void)hipGraphExecDestroy(b->exec[p]);
+        if (b->graph[p]) (void)hipGraphDestroy(b->graph[p]);
+        b->exec[p] = nullptr;
+        b->graph[p] = nullptr;
+    }
+}
+
+// host image of one step's tables, appended to `blob`
+template <class T> size_t put(std::vector<char> &blob, const T *data, size_t count) {
+    const size_t off = align_up(blob.size());
+    blob.resize(off + sizeof(T) * count);
+    if (count) std::memcpy(blob.data() + off, data, sizeof(T) * count);
+    return off;
+}
+
+void add_launch(std::vector<char> &blob, StepTables &t, Launch::Family fam, BatchOp op,
+                const std::vector<int> &blocks) {
+    std::vector<int> prefix(blocks.size() + 1, 0);
+    for (size_t i = 0; i < blocks.size(); ++i) prefix[i + 1] = prefix[i] + round_xcd(blocks[i]);
+    if (prefix.back() == 0) return;  // no scene has work in this stage
+    Launch l{};
+    l.family = fam;
+    l.op = op;
+    l.njobs = (int)blocks.size();
+    l.nblocks = prefix.back();
+    l.prefix_off = put(blob, prefix.data(), prefix.size());
+    t.launches.push_back(l);
+}
+
+// tables of one step for scenes whose buffer indices are `ph`
+void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph, std::vector<char> &blob,
+                StepTables &t) {
+    const int n = (int)sims.size();
+    std::vector<CellJob> cells;
+    std::vector<VirtJob> virt(n);
+    std::vector<FluidJob> fluid(n);
+    std::vector<SolidJob> solid(n);
+    int smax = 0;
+    for (int i = 0; i < n; ++i) {
+        const fsa_sim *s = sims[i];
+        KindBuffers f = s->fluid, so = s->solid;
+        f.st = ph[i].fst;
+        f.ord = ph[i].ford;
+        so.st = ph[i].sst;
+        so.ord = ph[i].sord;
+        cells.push_back(make_cell_job(s->P, f, s->jumps, true));
+        cells.push_back(make_cell_job(s->P, so, s->jumps, true));
+        // after the sort
+        if (f.n) {
+            f.st ^= 1;
+            f.ord ^= 1;
+        }
+        if (so.n) {
+            so.st ^= 1;
+            so.ord ^= 1;
+        }
+        virt[i] = VirtJob{s->P, virt_args(s, f, 0)};
+        fluid[i] = FluidJob{s->P, fluid_args(s, f, so)};
+        solid[i] = SolidJob{s->P, solid_args(s, f, so)};
+        smax = std::max(smax, s->P.ns > 0 ? s->P.sub : 0);
+    }
+    t = StepTables{};
+    t.nscenes = n;
+    t.ncells = (int)cells.size();
+    t.cells_off = put(blob, cells.data(), cells.size());
+    t.virt_off = put(blob, virt.data(), virt.size());
+    t.fluid_off = put(blob, fluid.data(), fluid.size());
+    t.solid_off = put(blob, solid.data(), solid.size());
+
+    std::vector<int> blocks;
+    for (BatchOp op : {kBFillCounts, kBHash, kBScanTiles, kBScanSums, kBScanAdd, kBPlace, kBGather}) {
+        blocks.clear();
+        for (const CellJob &j : cells) blocks.push_back(blocks_cells(op, j));
+        add_launch(blob, t, Launch::kCells, op, blocks);
+    }
+    // couple (src/solid.cu:346-359: only with both fluid and solid)
+    blocks.clear();
+    for (const SolidJob &j : solid) blocks.push_back(j.P.nf > 0 ? blocks_solid(kBCouple, j) : 0);
+    add_launch(blob, t, Launch::kSolid, kBCouple, blocks);
+    blocks.clear();
+    for (const VirtJob &j : virt) blocks.push_back(blocks_virt(j));
+    add_launch(blob, t, Launch::kVirt, kBFillCounts, blocks);
+    blocks.clear();
+    for (const FluidJob &j : fluid) blocks.push_back(blocks_fluid(j));
+    add_launch(blob, t, Launch::kFluid, kBFillCounts, blocks);
+    for (int sub = 0; sub < smax; ++sub)
+        for (BatchOp op : {kBDeform, kBForce, kBSolidUpdate}) {
+            blocks.clear();
+            for (const SolidJob &j : solid) blocks.push_back(sub < j.P.sub ? blocks_solid(op, j) : 0);
+            add_launch(blob, t, Launch::kSolid, op, blocks);
+        }
+}
+
+hipError_t enqueue_step(fsa_batch *b, const StepTables &t) {
+    const char *base = b->dev_blob;
+    for (const Launch &l : t.launches) {
+        const int *prefix = reinterpret_cast<const int *>(base + l.prefix_off);
+        hipError_t e = hipSuccess;
+        switch (l.family) {
+        case Launch::kCells:
+            e = launch_many_cells(l.op, reinterpret_cast<const CellJob *>(base + t.cells_off), prefix, l.njobs,
+                                  l.nblocks, b->stream);
+            break;
+        case Launch::kVirt:
+            e = launch_many_virt(reinterpret_cast<const VirtJob *>(base + t.virt_off), prefix, l.njobs, l.nblocks,
+                                 b->stream);
+            break;
+        case Launch::kFluid:
+            e = launch_many_fluid(reinterpret_cast<const FluidJob *>(base + t.fluid_off), prefix, l.njobs,
+                                  l.nblocks, b->stream);
+            break;
+        case Launch::kSolid:
+            e = launch_many_solid(l.op, reinterpret_cast<const SolidJob *>(base + t.solid_off), prefix, l.njobs,
+                                  l.nblocks, b->stream);
+            break;
+        }
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+int build_plan(fsa_batch *b, const std::vector<fsa_sim *> &sims) {
+    drop_graphs(b);
+    b->planned = false;
+    const int n = (int)sims.size();
+    std::vector<Phase> ph(n), ph1(n);
+    for (int i = 0; i < n; ++i) {
+        ph[i] = phase_of(sims[i]);
+        ph1[i] = next_phase(sims[i], ph[i]);
+    }
+    std::vector<char> blob;
+    build_step(sims, ph, blob, b->tables[0]);
+    build_step(sims, ph1, blob, b->tables[1]);
+    if (blob.size() > b->blob_cap) {
+        HIP_TRY(hipStreamSynchronize(b->stream));  // the old tables may still be read
+        if (b->dev_blob) (void)hipFree(b->dev_blob);
+        b->dev_blob = nullptr;
+        b->blob_cap = 0;
+        const size_t cap = std::max(blob.size(), 2 * b->blob_cap);
+        HIP_TRY(hipMalloc(&b->dev_blob, cap));
+        b->blob_cap = cap;
+    }
+    HIP_TRY(hipMemcpyAsync(b->dev_blob, blob.data(), blob.size(), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));  // `blob` is pageable host memory and goes out of scope
+    if (!(b->flags & FSA_BATCH_NO_GRAPH)) {
+        for (int p = 0; p < 2; ++p) {
+            HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            const hipError_t e = enqueue_step(b, b->tables[p]);
+            hipGraph_t g = nullptr;
+            const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return set_err(FSA_ERR_HIP, std::string("batch step capture: ") +
+                                                hipGetErrorString(e != hipSuccess ? e : e2));
+            }
+            b->graph[p] = g;
+            HIP_TRY(hipGraphInstantiate(&b->exec[p], g, nullptr, nullptr, 0));
+        }
+    }
+    b->serials.resize(n);
+    for (int i = 0; i < n; ++i) b->serials[i] = sims[i]->serial;
+    b->phase0 = ph;
+    b->planned = true;
+    b->plans_built += 1;
+    return FSA_OK;
+}
+
+// which recorded step matches the scenes' current buffer indices (-1: none)
+int plan_phase(const fsa_batch *b, const std::vector<fsa_sim *> &sims) {
+    if (!b->planned || sims.size() != b->serials.size()) return -1;
+    for (size_t i = 0; i < sims.size(); ++i)
+        if (sims[i]->serial != b->serials[i]) return -1;
+    for (int p = 0; p < 2; ++p) {
+        bool all = true;
+        for (size_t i = 0; i < sims.size() && all; ++i) {
+            const Phase want = p == 0 ? b->phase0[i] : next_phase(sims[i], b->phase0[i]);
+            all = phase_of(sims[i]) == want;
+        }
+        if (all) return p;
+    }
+    return -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsa_batch_create(void *stream, unsigned flags, fsa_batch **out) {
+    if (!out) return set_err(FSA_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (flags & ~(unsigned)FSA_BATCH_NO_GRAPH) return set_err(FSA_ERR_INVALID, "unknown batch flag");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_err(FSA_ERR_NODEVICE, "no HIP device visible");
+    fsa_batch *b = new (std::nothrow) fsa_batch;
+    if (!b) return set_err(FSA_ERR_NOMEM, "host allocation failed");
+    b->flags = flags;
+    (void)hipGetDevice(&b->device);
+    if (stream) {
+        b->stream = static_cast<hipStream_t>(stream);
+    } else {
+        if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete b;
+            return set_err(FSA_ERR_HIP, "hipStreamCreate failed");
+        }
+        b->own_stream = true;
+    }
+    if (hipEventCreateWithFlags(&b->ev_sync, hipEventDisableTiming) != hipSuccess) {
+        fsa_batch_destroy(b);
+        return set_err(FSA_ERR_HIP, "hipEventCreate failed");
+    }
+    *out = b;
+    return FSA_OK;
+}
+
+int fsa_batch_destroy(fsa_batch *b) {
+    if (!b) return FSA_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graphs(b);
+    if (b->dev_blob) (void)hipFree(b->dev_blob);
+    if (b->ev_sync) (void)hipEventDestroy(b->ev_sync);
+    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return FSA_OK;
+}
+
+int fsa_batch_step(fsa_batch *b, fsa_sim *const *scenes, int n_scenes, int n_steps) {
+    if (!b) return set_err(FSA_ERR_INVALID, "null batch");
+    if (n_scenes < 0 || n_steps < 0) return set_err(FSA_ERR_INVALID, "negative scene or step count");
+    if (n_scenes > 0 && !scenes) return set_err(FSA_ERR_INVALID, "null scene list");
+    if (n_scenes == 0 || n_steps == 0) return FSA_OK;
+    std::vector<fsa_sim *> sims(scenes, scenes + n_scenes);
+    for (int i = 0; i < n_scenes; ++i) {
+        fsa_sim *s = sims[i];
+        if (!s) return set_err(FSA_ERR_INVALID, "null scene in the list");
+        if (s->device != b->device) return set_err(FSA_ERR_INVALID, "scene lives on another device than the batch");
+        int rc = require_uploaded(s);
+        if (rc) return rc;
+        for (int j = 0; j < i; ++j)
+            if (sims[j] == s) return set_err(FSA_ERR_INVALID, "scene listed twice");
+    }
+    // the batch stream waits for work already queued on the scenes' streams
+    for (fsa_sim *s : sims) {
+        if (s->stream == b->stream) continue;
+        HIP_TRY(hipEventRecord(b->ev_sync, s->stream));
+        HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_sync, 0));
+    }
+    int phase = plan_phase(b, sims);
+    if (phase < 0) {
+        int rc = build_plan(b, sims);
+        if (rc) return rc;
+        phase = 0;
+    }
+    // wall normals are computed once, on a scene's first update (the walls do
+    // not move); the step's own wall pass then reads them
+    for (fsa_sim *s : sims) {
+        if (s->P.nv == 0 || s->vx.normals_ready) continue;
+        HIP_TRY(launch_virt(s->P, virt_args(s, 1), b->stream));
+        s->vx.normals_ready = true;
+    }
+    for (int k = 0; k < n_steps; ++k) {
+        if (b->exec[phase]) {
+            HIP_TRY(hipGraphLaunch(b->exec[phase], b->stream));
+        } else {
+            HIP_TRY(enqueue_step(b, b->tables[phase]));
+        }
+        for (fsa_sim *s : sims) {
+            apply_phase(s, next_phase(s, phase_of(s)));
+            s->hashed = false;
+        }
+        phase ^= 1;
+        b->steps_run += 1;
+    }
+    // the scenes' streams wait for the batch before their next work
+    HIP_TRY(hipEventRecord(b->ev_sync, b->stream));
+    for (fsa_sim *s : sims) {
+        if (s->stream == b->stream) continue;
+        HIP_TRY(hipStreamWaitEvent(s->stream, b->ev_sync, 0));
+    }
+    return FSA_OK;
+}
+
+int fsa_batch_synchronize(fsa_batch *b) {
+    if (!b) return set_err(FSA_ERR_INVALID, "null batch");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return FSA_OK;
+}
+
+int fsa_batch_stats(fsa_batch *b, int64_t *plans_built, int64_t *steps_run) {
+    if (!b) return set_err(FSA_ERR_INVALID, "null batch");
+    if (plans_built) *plans_built = b->plans_built;
+    if (steps_run) *steps_run = b->steps_run;
+    return FSA_OK;
+}
+
+}  // extern "C"

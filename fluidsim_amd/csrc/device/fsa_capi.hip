@@ -2,6 +2,7 @@
 // uploads/downloads, and the step stages in the reference's order
 // (src/calculation.cu:23-33).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -9,84 +10,21 @@
 #include <vector>
 
 #include "fluidsim_amd.h"
+#include "fsa_context.h"
 #include "fsa_internal.h"
 
 using namespace fsa;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int set_err(int code, const std::string &msg) {
+int fsa::set_err(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
 
-#define HIP_TRY(expr)                                                                             \
-    do {                                                                                          \
-        hipError_t e_ = (expr);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
-            return set_err(e_ == hipErrorOutOfMemory ? FSA_ERR_NOMEM : FSA_ERR_HIP,               \
-                           std::string(#expr) + ": " + hipGetErrorString(e_));                    \
-    } while (0)
-
-}  // namespace
-
-struct fsa_sim {
-    fsa_params params{};
-    DevParams P{};
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    int device = 0;
-
-    KindBuffers fluid, solid, virt;
-    SolidExtra sx;
-    VirtExtra vx;
-    float4 *virt_x = nullptr;  // constant wall positions (x, y, z, 0)
-    float4 *rates = nullptr;   // fluid (ax, ay, az, drho) when debugging
-    ScanScratch scan;
-    unsigned long long *jumps = nullptr;
-    // observables
-    double *obs_partial = nullptr, *obs_out = nullptr;
-    float tip_y = 3.0e38f;  // reference y of the gate tip row (none: above everything)
-    unsigned long long *census = nullptr;
-    float4 *probe_pts = nullptr;
-    float *probe_out = nullptr;
-    int probe_cap = 0;
-
-    bool uploaded[3] = {false, false, false};
-    bool hashed = false;  // cell ids/ranks of the current order are fresh
-    unsigned debug = 0;
-
-    // stage timing: a ring of event pairs per stage, read back lazily so the
-    // host never waits on the GPU while it enqueues steps
-    static constexpr int kRing = 256;
-    bool timing = false;
-    hipEvent_t ev_begin[FSA_STAGE_COUNT][kRing] = {};
-    hipEvent_t ev_end[FSA_STAGE_COUNT][kRing] = {};
-    int ev_head[FSA_STAGE_COUNT] = {};     // next ring slot
-    int ev_pending[FSA_STAGE_COUNT] = {};  // recorded, not yet read back
-    bool ev_open[FSA_STAGE_COUNT] = {};
-    double stage_ms[FSA_STAGE_COUNT] = {};
-    int64_t stage_n[FSA_STAGE_COUNT] = {};
-
-    std::vector<void *> allocations;
-    int64_t bytes = 0;
-
-    template <class T> int alloc(T **p, size_t count) {
-        *p = nullptr;
-        if (count == 0) return FSA_OK;
-        void *q = nullptr;
-        hipError_t e = hipMalloc(&q, count * sizeof(T));
-        if (e != hipSuccess)
-            return set_err(FSA_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
-                                              " bytes: " + hipGetErrorString(e));
-        allocations.push_back(q);
-        bytes += (int64_t)(count * sizeof(T));
-        *p = static_cast<T *>(q);
-        return FSA_OK;
-    }
-};
+#define HIP_TRY(expr) FSA_HIP_TRY(expr)
 
 namespace fsa {
 DevParams make_dev_params(const fsa_params &p) {
@@ -184,6 +122,7 @@ int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_sta
     if ((rc = s->alloc(&k.rank, n))) return rc;
     if ((rc = s->alloc(&k.tmp_slot, n))) return rc;
     if ((rc = s->alloc(&k.tmp_id, n))) return rc;
+    if ((rc = s->alloc(&k.tile_sums, (size_t)scan_tiles(ncell)))) return rc;
     return FSA_OK;
 }
 
@@ -211,12 +150,17 @@ int check_in_grid(const fsa_sim *s, const float *x, int n, const char *what) {
     return FSA_OK;
 }
 
+// the sort gathered into the other buffers: they now hold the state
+void flip(KindBuffers &k) {
+    k.st ^= 1;
+    k.ord ^= 1;
+}
+
 // full cell build of one kind: hash + count, scan, place + rank + gather
-int build_cells(fsa_sim *s, KindBuffers &k, bool has_ref, bool count_jumps) {
+int build_cells(fsa_sim *s, KindBuffers &k, bool count_jumps) {
     if (k.n == 0) return FSA_OK;
-    HIP_TRY(launch_hash_count(s->P, k, s->jumps, count_jumps, s->stream));
-    HIP_TRY(launch_scan(k.count, k.start, s->P.ncell, s->scan, s->stream));
-    HIP_TRY(launch_sort_gather(s->P, k, has_ref, s->stream));
+    HIP_TRY(launch_cells_one(make_cell_job(s->P, k, s->jumps, count_jumps), s->stream));
+    flip(k);
     return FSA_OK;
 }
 
@@ -271,7 +215,9 @@ void stage_end(fsa_sim *s, int st) {
         if (!(s)) return set_err(FSA_ERR_INVALID, "null context");                     \
     } while (0)
 
-int require_uploaded(fsa_sim *s) {
+}  // namespace
+
+int fsa::require_uploaded(fsa_sim *s) {
     const int n[3] = {s->P.nf, s->P.ns, s->P.nv};
     for (int k = 0; k < 3; ++k)
         if (n[k] > 0 && !s->uploaded[k])
@@ -279,21 +225,53 @@ int require_uploaded(fsa_sim *s) {
     return FSA_OK;
 }
 
-SolidArgs solid_args(fsa_sim *s) {
+SolidArgs fsa::solid_args(const fsa_sim *s, const KindBuffers &f, const KindBuffers &so) {
     SolidArgs a{};
-    a.spos = s->solid.pos[s->solid.st];
-    a.svel = s->solid.vel[s->solid.st];
-    a.sref = s->solid.ref[s->solid.ord];
-    a.scell = s->solid.cell[s->solid.ord];
-    a.fstart = s->fluid.start;
-    a.sstart = s->solid.start;
-    a.fpos = s->fluid.pos[s->fluid.st];
-    a.fvel = s->fluid.vel[s->fluid.st];
+    a.spos = so.pos[so.st];
+    a.svel = so.vel[so.st];
+    a.sref = so.ref[so.ord];
+    a.scell = so.cell[so.ord];
+    a.fstart = f.start;
+    a.sstart = so.start;
+    a.fpos = f.pos[f.st];
+    a.fvel = f.vel[f.st];
     a.x = s->sx;
     return a;
 }
 
-}  // namespace
+VirtArgs fsa::virt_args(const fsa_sim *s, const KindBuffers &f, int compute_normals) {
+    VirtArgs a{};
+    a.vpos_in = s->virt_x;
+    a.vpos = s->virt.pos[0];
+    a.vvel = s->virt.vel[0];
+    a.vnrm = s->vx.nrm;
+    a.vcell = s->virt.cell[s->virt.ord];
+    a.fstart = f.start;
+    a.vstart = s->virt.start;
+    a.fpos = f.pos[f.st];
+    a.fvel = f.vel[f.st];
+    a.compute_normals = compute_normals;
+    return a;
+}
+
+FluidStepArgs fsa::fluid_args(const fsa_sim *s, const KindBuffers &f, const KindBuffers &so) {
+    FluidStepArgs a{};
+    a.fpos = f.pos[f.st];
+    a.fvel = f.vel[f.st];
+    a.fcell = f.cell[f.ord];
+    a.fstart = f.start;
+    a.sstart = so.start;
+    a.vstart = s->virt.start;
+    a.spos = so.pos[so.st];
+    a.svel = so.vel[so.st];
+    a.snp = s->sx.np;
+    a.vpos = s->virt.pos[0];
+    a.vvel = s->virt.vel[0];
+    a.fpos_out = f.pos[1 - f.st];
+    a.fvel_out = f.vel[1 - f.st];
+    a.rates = nullptr;
+    return a;
+}
 
 extern "C" {
 
@@ -328,6 +306,8 @@ int fsa_create(const fsa_params *params, void *stream, fsa_sim **out) {
     if (!s) return set_err(FSA_ERR_NOMEM, "host allocation failed");
     s->params = *params;
     s->P = make_dev_params(*params);
+    static std::atomic<uint64_t> next_serial{1};
+    s->serial = next_serial.fetch_add(1);
     hipGetDevice(&s->device);
     {
         hipDeviceProp_t prop;
@@ -360,8 +340,6 @@ int fsa_create(const fsa_params *params, void *stream, fsa_sim **out) {
         (rc = s->alloc(&s->sx.A, 9 * (size_t)ns)))
         return fail(rc);
     if ((rc = s->alloc(&s->vx.nrm, s->P.nv)) || (rc = s->alloc(&s->virt_x, s->P.nv))) return fail(rc);
-    s->scan.ntiles_cap = (s->P.ncell + kScanTile - 1) / kScanTile;
-    if ((rc = s->alloc(&s->scan.tile_sums, (size_t)s->scan.ntiles_cap))) return fail(rc);
     if ((rc = s->alloc(&s->jumps, 1))) return fail(rc);
     if ((rc = s->alloc(&s->obs_partial, (size_t)observe_partials())) || (rc = s->alloc(&s->obs_out, kObserveSums)))
         return fail(rc);
@@ -408,7 +386,7 @@ int fsa_upload_fluid(fsa_sim *s, const float *x, const float *u, const float *rh
         HIP_TRY(hipMemcpyAsync(k.pos[0], pos.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
         HIP_TRY(hipMemcpyAsync(k.vel[0], vel.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
         HIP_TRY(hipMemcpyAsync(k.id[0], id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s->stream));
-        if ((rc = build_cells(s, k, false, false))) return rc;
+        if ((rc = build_cells(s, k, false))) return rc;
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->uploaded[FSA_FLUID] = true;
@@ -441,7 +419,7 @@ int fsa_upload_solid(fsa_sim *s, const float *x, const float *u, const float *X,
         HIP_TRY(hipMemcpyAsync(k.vel[0], vel.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
         HIP_TRY(hipMemcpyAsync(k.ref[0], ref.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
         HIP_TRY(hipMemcpyAsync(k.id[0], id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s->stream));
-        if ((rc = build_cells(s, k, true, false))) return rc;
+        if ((rc = build_cells(s, k, false))) return rc;
         // Solid_Host_List::addList defaults (include/host_data.cuh:92-99):
         // F = I, P = 0, L = I, n = 0, p = 0, du_dt = 0
         std::vector<float> eye(9 * (size_t)n, 0.0f), zero9(9 * (size_t)n, 0.0f);
@@ -498,7 +476,7 @@ int fsa_upload_virt(fsa_sim *s, const float *x, const int32_t *type) {
             hipMemcpyAsync(vel_tmp, vel.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
             hipMemcpyAsync(k.id[0], id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s->stream) != hipSuccess)
             err = set_err(FSA_ERR_HIP, "wall upload copy failed");
-        if (!err) err = build_cells(s, k, false, false);  // gathers into pos[0]/vel[0]
+        if (!err) err = build_cells(s, k, false);  // gathers into pos[0]/vel[0]
         if (!err && hipMemcpyAsync(s->virt_x, k.pos[0], sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream) !=
                         hipSuccess)
             err = set_err(FSA_ERR_HIP, "wall position copy failed");
@@ -626,8 +604,8 @@ int fsa_particle_cell_update(fsa_sim *s) {
     if (rc) return rc;
     if ((rc = stage_open(s, FSA_STAGE_CELL))) return rc;
     // wall cell ids never change (the walls do not move): only fluid and solid
-    if (s->fluid.n) HIP_TRY(launch_hash_count(s->P, s->fluid, s->jumps, true, s->stream));
-    if (s->solid.n) HIP_TRY(launch_hash_count(s->P, s->solid, s->jumps, true, s->stream));
+    if (s->fluid.n) HIP_TRY(launch_hash_one(make_cell_job(s->P, s->fluid, s->jumps, true), s->stream));
+    if (s->solid.n) HIP_TRY(launch_hash_one(make_cell_job(s->P, s->solid, s->jumps, true), s->stream));
     s->hashed = true;
     return FSA_OK;
 }
@@ -640,13 +618,10 @@ int fsa_fill_cell(fsa_sim *s) {
     // current order only until the next sort
     if (!s->hashed) return set_err(FSA_ERR_STATE, "fill_cell needs particle_cell_update first");
     s->hashed = false;
-    if (s->fluid.n) {
-        HIP_TRY(launch_scan(s->fluid.count, s->fluid.start, s->P.ncell, s->scan, s->stream));
-        HIP_TRY(launch_sort_gather(s->P, s->fluid, false, s->stream));
-    }
-    if (s->solid.n) {
-        HIP_TRY(launch_scan(s->solid.count, s->solid.start, s->P.ncell, s->scan, s->stream));
-        HIP_TRY(launch_sort_gather(s->P, s->solid, true, s->stream));
+    for (KindBuffers *k : {&s->fluid, &s->solid}) {
+        if (!k->n) continue;
+        HIP_TRY(launch_sort_one(make_cell_job(s->P, *k, s->jumps, false), s->stream));
+        flip(*k);
     }
     stage_end(s, FSA_STAGE_CELL);
     return FSA_OK;
@@ -670,17 +645,7 @@ int fsa_virt_update(fsa_sim *s) {
     if (rc) return rc;
     if (s->P.nv == 0) return FSA_OK;
     if ((rc = stage_open(s, FSA_STAGE_VIRT))) return rc;
-    VirtArgs a{};
-    a.vpos_in = s->virt_x;
-    a.vpos = s->virt.pos[0];
-    a.vvel = s->virt.vel[0];
-    a.vnrm = s->vx.nrm;
-    a.vcell = s->virt.cell[s->virt.ord];
-    a.fstart = s->fluid.start;
-    a.vstart = s->virt.start;
-    a.fpos = s->fluid.pos[s->fluid.st];
-    a.fvel = s->fluid.vel[s->fluid.st];
-    a.compute_normals = !s->vx.normals_ready;
+    const VirtArgs a = virt_args(s, s->vx.normals_ready ? 0 : 1);
     HIP_TRY(launch_virt(s->P, a, s->stream));
     s->vx.normals_ready = true;
     stage_end(s, FSA_STAGE_VIRT);
@@ -694,20 +659,7 @@ int fsa_fluid_calculation(fsa_sim *s) {
     if (s->P.nf == 0) return FSA_OK;
     if ((rc = stage_open(s, FSA_STAGE_FLUID))) return rc;
     KindBuffers &f = s->fluid;
-    FluidStepArgs a{};
-    a.fpos = f.pos[f.st];
-    a.fvel = f.vel[f.st];
-    a.fcell = f.cell[f.ord];
-    a.fstart = f.start;
-    a.sstart = s->solid.start;
-    a.vstart = s->virt.start;
-    a.spos = s->solid.pos[s->solid.st];
-    a.svel = s->solid.vel[s->solid.st];
-    a.snp = s->sx.np;
-    a.vpos = s->virt.pos[0];
-    a.vvel = s->virt.vel[0];
-    a.fpos_out = f.pos[1 - f.st];
-    a.fvel_out = f.vel[1 - f.st];
+    FluidStepArgs a = fluid_args(s);
     a.rates = (s->debug & FSA_DEBUG_PAIR_SUMS) ? s->rates : nullptr;
     HIP_TRY(launch_fluid(s->P, a, s->stream));
     f.st = 1 - f.st;

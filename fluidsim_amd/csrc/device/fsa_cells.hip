@@ -4,17 +4,18 @@
 // step by walking the previous lists of the 27 neighbour cells with one
 // thread per cell (src/pair.cu:72-207): a latency-bound pointer chase.  Here
 // the cell list is a counting sort of the particles by cell id:
-//   1. hash+count  one thread per particle: cell id (findPositionCell,
-//                  src/pair.cu:8-24) and an atomic arrival rank in its cell
+//   1. fill+hash   zero the per-cell counts; one thread per particle: cell id
+//                  (findPositionCell, src/pair.cu:8-24) and an arrival rank
+//                  in its cell (atomics aggregated per wave)
 //   2. scan        exclusive prefix sum of the per-cell counts -> start[]
 //   3. place       the particle's slot and .dat id at start[c] + rank
-//   4. rank+gather inside each cell the final rank is the number of cell
+//   4. gather      inside each cell the final rank is the number of cell
 //                  members with a smaller .dat id, which makes the order
 //                  deterministic (ascending .dat id) whatever order the
 //                  atomics came in; the state is gathered into the other
 //                  buffer at start[c] + rank.
-// All passes are streaming over HBM except the small per-cell rank loop,
-// which reads the cell's ids from L1/L2.
+// All passes stream over HBM except the small per-cell rank loop, which
+// reads the cell's ids from L1/L2.
 #include "fsa_internal.h"
 #include "fsa_math.h"
 
@@ -27,45 +28,52 @@ __global__ __launch_bounds__(kBlock) void k_fill_i32(int32_t *__restrict__ p, in
     if (i < n) p[i] = v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_hash_count(DevParams P, int n, const float4 *__restrict__ pos,
-                                                       const int32_t *__restrict__ prev_cell,
-                                                       int32_t *__restrict__ key, int32_t *__restrict__ rank,
-                                                       int32_t *__restrict__ count, unsigned long long *jumps,
-                                                       int count_jumps) {
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
-    const float4 x = pos[s];
-    const int c = find_cell(P, x.x, x.y, x.z);
-    key[s] = c;
-    // Particles arrive sorted by their previous cell, so the lanes of a wave
-    // share few cells: one atomic per distinct cell and wave, the lanes of a
-    // cell take consecutive ranks (the order is fixed later by the rank pass).
-    const int lane = threadIdx.x & 63;
-    const unsigned long long below = (1ull << lane) - 1ull;
-    unsigned long long todo = __ballot(1);
-    int r = 0;
-    while (todo) {
-        const int leader = __ffsll((long long)todo) - 1;
-        const int lc = __shfl(c, leader, 64);
-        const unsigned long long same = __ballot(c == lc);
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&count[lc], __popcll(same));
-        base = __shfl(base, leader, 64);
-        if (c == lc) r = base + __popcll(same & below);
-        todo &= ~same;
+struct OpFillCountsImpl {
+    static __device__ void run(const CellJob &j, int blk, int) {
+        const int i = blk * kBlock + threadIdx.x;
+        if (i < padded_cells(j.P.ncell)) j.count[i] = 0;
     }
-    rank[s] = r;
-    if (count_jumps) {
-        // more than one cell in any direction since the last build: the
-        // reference's list rebuild would have lost this particle
-        const int o = prev_cell[s];
-        const int nyz = P.cn[1] * P.cn[2];
-        const int ax = c / nyz, bx = o / nyz;
-        const int ay = (c - ax * nyz) / P.cn[2], by = (o - bx * nyz) / P.cn[2];
-        const int az = c % P.cn[2], bz = o % P.cn[2];
-        if (abs(ax - bx) > 1 || abs(ay - by) > 1 || abs(az - bz) > 1) atomicAdd(jumps, 1ull);
+};
+
+struct OpHashImpl {
+    static __device__ void run(const CellJob &j, int blk, int) {
+        const int s = blk * kBlock + threadIdx.x;
+        if (s >= j.n) return;
+        const DevParams &P = j.P;
+        const float4 x = j.pos_in[s];
+        const int c = find_cell(P, x.x, x.y, x.z);
+        j.key[s] = c;
+        // Particles arrive sorted by their previous cell, so the lanes of a
+        // wave share few cells: one atomic per distinct cell and wave, the
+        // lanes of a cell take consecutive ranks (the order is fixed later by
+        // the gather pass).
+        const int lane = threadIdx.x & 63;
+        const unsigned long long below = (1ull << lane) - 1ull;
+        unsigned long long todo = __ballot(1);
+        int r = 0;
+        while (todo) {
+            const int leader = __ffsll((long long)todo) - 1;
+            const int lc = __shfl(c, leader, 64);
+            const unsigned long long same = __ballot(c == lc);
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&j.count[lc], __popcll(same));
+            base = __shfl(base, leader, 64);
+            if (c == lc) r = base + __popcll(same & below);
+            todo &= ~same;
+        }
+        j.rank[s] = r;
+        if (j.count_jumps) {
+            // more than one cell in any direction since the last build: the
+            // reference's list rebuild would have lost this particle
+            const int o = j.prev_cell[s];
+            const int nyz = P.cn[1] * P.cn[2];
+            const int ax = c / nyz, bx = o / nyz;
+            const int ay = (c - ax * nyz) / P.cn[2], by = (o - bx * nyz) / P.cn[2];
+            const int az = c % P.cn[2], bz = o % P.cn[2];
+            if (abs(ax - bx) > 1 || abs(ay - by) > 1 || abs(az - bz) > 1) atomicAdd(j.jumps, 1ull);
+        }
     }
-}
+};
 
 // inclusive scan over the 64 lanes of a wavefront
 __device__ __forceinline__ int wave_inclusive_scan(int v) {
@@ -92,133 +100,180 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *wave_tot, int &b
 }
 
 // pass 1: per-tile inclusive scan into start[1 + i], tile totals
-__global__ __launch_bounds__(kBlock) void k_scan_tiles(const int32_t *__restrict__ count, int32_t *__restrict__ start,
-                                                       int32_t *__restrict__ tile_sums, int n) {
-    __shared__ int wave_tot[kBlock / 64];
-    const int base = blockIdx.x * kScanTile + threadIdx.x * 4;
-    // count is padded to a multiple of kScanTile with zeros
-    const int4 v = *reinterpret_cast<const int4 *>(count + base);
-    const int t0 = v.x, t1 = t0 + v.y, t2 = t1 + v.z, t3 = t2 + v.w;
-    int total;
-    const int ex = block_exclusive_scan(t3, wave_tot, total);
-    const int vals[4] = {ex + t0, ex + t1, ex + t2, ex + t3};
+struct OpScanTilesImpl {
+    static __device__ void run(const CellJob &j, int blk, int) {
+        __shared__ int wave_tot[kBlock / 64];
+        const int n = j.P.ncell;
+        // a batched grid rounds each scene's block range up: whole blocks past
+        // the last tile leave before the first barrier
+        if (blk >= scan_tiles(n)) return;
+        const int base = blk * kScanTile + threadIdx.x * 4;
+        // count is padded to a multiple of kScanTile with zeros
+        const int4 v = *reinterpret_cast<const int4 *>(j.count + base);
+        const int t0 = v.x, t1 = t0 + v.y, t2 = t1 + v.z, t3 = t2 + v.w;
+        int total;
+        const int ex = block_exclusive_scan(t3, wave_tot, total);
+        const int vals[4] = {ex + t0, ex + t1, ex + t2, ex + t3};
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (base + k < n) start[1 + base + k] = vals[k];
-    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
-}
+        for (int k = 0; k < 4; ++k)
+            if (base + k < n) j.start[1 + base + k] = vals[k];
+        if (threadIdx.x == 0) j.tile_sums[blk] = total;
+    }
+};
 
 // pass 2: one workgroup scans the tile totals (exclusive, in place)
-__global__ __launch_bounds__(kBlock) void k_scan_tile_sums(int32_t *__restrict__ tile_sums, int ntiles) {
-    __shared__ int wave_tot[kBlock / 64];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int b = 0; b < ntiles; b += kBlock) {
-        const int i = b + threadIdx.x;
-        const int v = i < ntiles ? tile_sums[i] : 0;
-        int total;
-        const int ex = block_exclusive_scan(v, wave_tot, total);
-        const int c = carry;
-        if (i < ntiles) tile_sums[i] = c + ex;
+struct OpScanSumsImpl {
+    static __device__ void run(const CellJob &j, int blk, int) {
+        __shared__ int wave_tot[kBlock / 64];
+        __shared__ int carry;
+        if (blk != 0) return;  // padding blocks of a batched grid
+        const int ntiles = scan_tiles(j.P.ncell);
+        if (threadIdx.x == 0) carry = 0;
         __syncthreads();
-        if (threadIdx.x == 0) carry = c + total;
-        __syncthreads();
+        for (int b = 0; b < ntiles; b += kBlock) {
+            const int i = b + threadIdx.x;
+            const int v = i < ntiles ? j.tile_sums[i] : 0;
+            int total;
+            const int ex = block_exclusive_scan(v, wave_tot, total);
+            const int c = carry;
+            if (i < ntiles) j.tile_sums[i] = c + ex;
+            __syncthreads();
+            if (threadIdx.x == 0) carry = c + total;
+            __syncthreads();
+        }
     }
-}
+};
 
 // pass 3: add tile offsets; start[0] = 0
-__global__ __launch_bounds__(kBlock) void k_scan_add(int32_t *__restrict__ start,
-                                                     const int32_t *__restrict__ tile_sums, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i == 0) start[0] = 0;
-    if (i < n) start[1 + i] += tile_sums[i / kScanTile];
-}
+struct OpScanAddImpl {
+    static __device__ void run(const CellJob &j, int blk, int) {
+        const int i = blk * kBlock + threadIdx.x;
+        if (i == 0) j.start[0] = 0;
+        if (i < j.P.ncell) j.start[1 + i] += j.tile_sums[i / kScanTile];
+    }
+};
 
-__global__ __launch_bounds__(kBlock) void k_place(int n, const int32_t *__restrict__ key,
-                                                  const int32_t *__restrict__ rank,
-                                                  const int32_t *__restrict__ start, const int32_t *__restrict__ id,
-                                                  int32_t *__restrict__ tmp_slot, int32_t *__restrict__ tmp_id) {
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
-    const int t = start[key[s]] + rank[s];
-    tmp_slot[t] = s;
-    tmp_id[t] = id[s];
-}
+struct OpPlaceImpl {
+    static __device__ void run(const CellJob &j, int blk, int) {
+        const int s = blk * kBlock + threadIdx.x;
+        if (s >= j.n) return;
+        const int t = j.start[j.key[s]] + j.rank[s];
+        j.tmp_slot[t] = s;
+        j.tmp_id[t] = j.id_in[s];
+    }
+};
 
-template <bool HAS_REF>
-__global__ __launch_bounds__(kBlock) void k_rank_gather(int n, const int32_t *__restrict__ key,
-                                                        const int32_t *__restrict__ start,
-                                                        const int32_t *__restrict__ tmp_slot,
-                                                        const int32_t *__restrict__ tmp_id,
-                                                        const float4 *__restrict__ pos_in,
-                                                        const float4 *__restrict__ vel_in,
-                                                        const float4 *__restrict__ ref_in, float4 *__restrict__ pos_out,
-                                                        float4 *__restrict__ vel_out, float4 *__restrict__ ref_out,
-                                                        int32_t *__restrict__ id_out, int32_t *__restrict__ cell_out) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= n) return;
-    const int s = tmp_slot[t];
-    const int my = tmp_id[t];
-    const int c = key[s];
-    const int b = start[c], e = start[c + 1];
-    int r = 0;
-    for (int u = b; u < e; ++u) r += tmp_id[u] < my ? 1 : 0;
-    const int d = b + r;
-    pos_out[d] = pos_in[s];
-    vel_out[d] = vel_in[s];
-    if (HAS_REF) ref_out[d] = ref_in[s];
-    id_out[d] = my;
-    cell_out[d] = c;
-}
+struct OpGatherImpl {
+    static __device__ void run(const CellJob &j, int blk, int) {
+        const int t = blk * kBlock + threadIdx.x;
+        if (t >= j.n) return;
+        const int s = j.tmp_slot[t];
+        const int my = j.tmp_id[t];
+        const int c = j.key[s];
+        const int b = j.start[c], e = j.start[c + 1];
+        int r = 0;
+        for (int u = b; u < e; ++u) r += j.tmp_id[u] < my ? 1 : 0;
+        const int d = b + r;
+        j.pos_out[d] = j.pos_in[s];
+        j.vel_out[d] = j.vel_in[s];
+        if (j.has_ref) j.ref_out[d] = j.ref_in[s];
+        j.id_out[d] = my;
+        j.cell_out[d] = c;
+    }
+};
 
-inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
+template <class Op>
+hipError_t one(const CellJob &j, int blocks, hipStream_t s) {
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_one<Op, CellJob>), dim3(blocks), dim3(kBlock), 0, s, j);
+    return hipGetLastError();
+}
 
 }  // namespace
 
+int blocks_cells(BatchOp op, const CellJob &j) {
+    if (j.n == 0) return 0;
+    switch (op) {
+    case kBFillCounts: return blocks_for(padded_cells(j.P.ncell));
+    case kBHash: return blocks_for(j.n);
+    case kBScanTiles: return scan_tiles(j.P.ncell);
+    case kBScanSums: return 1;
+    case kBScanAdd: return blocks_for(j.P.ncell);
+    case kBPlace: return blocks_for(j.n);
+    case kBGather: return blocks_for(j.n);
+    default: return 0;
+    }
+}
+
+hipError_t launch_many_cells(BatchOp op, const CellJob *jobs, const int *prefix, int njobs, int nblocks,
+                             hipStream_t s) {
+    if (nblocks <= 0 || njobs <= 0) return hipSuccess;
+    const dim3 g(nblocks), b(kBlock);
+    switch (op) {
+    case kBFillCounts: hipLaunchKernelGGL((k_many<OpFillCountsImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    case kBHash: hipLaunchKernelGGL((k_many<OpHashImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    case kBScanTiles: hipLaunchKernelGGL((k_many<OpScanTilesImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    case kBScanSums: hipLaunchKernelGGL((k_many<OpScanSumsImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    case kBScanAdd: hipLaunchKernelGGL((k_many<OpScanAddImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    case kBPlace: hipLaunchKernelGGL((k_many<OpPlaceImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    case kBGather: hipLaunchKernelGGL((k_many<OpGatherImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+CellJob make_cell_job(const DevParams &P, const KindBuffers &k, unsigned long long *jumps, bool count_jumps) {
+    CellJob j{};
+    j.P = P;
+    j.n = k.n;
+    j.has_ref = k.ref[0] != nullptr ? 1 : 0;
+    j.count_jumps = count_jumps ? 1 : 0;
+    const int so = k.st, sn = 1 - k.st, oo = k.ord, on = 1 - k.ord;
+    j.pos_in = k.pos[so];
+    j.vel_in = k.vel[so];
+    j.ref_in = k.ref[oo];
+    j.id_in = k.id[oo];
+    j.prev_cell = k.cell[oo];
+    j.pos_out = k.pos[sn];
+    j.vel_out = k.vel[sn];
+    j.ref_out = k.ref[on];
+    j.id_out = k.id[on];
+    j.cell_out = k.cell[on];
+    j.key = k.key;
+    j.rank = k.rank;
+    j.count = k.count;
+    j.start = k.start;
+    j.tmp_slot = k.tmp_slot;
+    j.tmp_id = k.tmp_id;
+    j.tile_sums = k.tile_sums;
+    j.jumps = jumps;
+    return j;
+}
+
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(kBlock), 0, s, p, v, n);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n)), dim3(kBlock), 0, s, p, v, n);
     return hipGetLastError();
 }
 
-hipError_t launch_hash_count(const DevParams &P, KindBuffers &k, unsigned long long *jumps, bool count_jumps,
-                             hipStream_t s) {
-    const int padded = (P.ncell + kScanTile - 1) / kScanTile * kScanTile;
-    hipError_t e = launch_fill_i32(k.count, 0, padded, s);
-    if (e != hipSuccess || k.n == 0) return e;
-    hipLaunchKernelGGL(k_hash_count, dim3(grid_for(k.n)), dim3(kBlock), 0, s, P, k.n, k.pos[k.st], k.cell[k.ord],
-                       k.key, k.rank, k.count, jumps, count_jumps ? 1 : 0);
-    return hipGetLastError();
+hipError_t launch_hash_one(const CellJob &j, hipStream_t s) {
+    hipError_t e;
+    if ((e = one<OpFillCountsImpl>(j, blocks_cells(kBFillCounts, j), s)) != hipSuccess) return e;
+    return one<OpHashImpl>(j, blocks_cells(kBHash, j), s);
 }
 
-hipError_t launch_scan(const int32_t *count, int32_t *start, int ncell, ScanScratch &scr, hipStream_t s) {
-    const int ntiles = (ncell + kScanTile - 1) / kScanTile;
-    if (ntiles > scr.ntiles_cap) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, count, start, scr.tile_sums, ncell);
-    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kBlock), 0, s, scr.tile_sums, ntiles);
-    hipLaunchKernelGGL(k_scan_add, dim3(grid_for(ncell)), dim3(kBlock), 0, s, start, scr.tile_sums, ncell);
-    return hipGetLastError();
+hipError_t launch_sort_one(const CellJob &j, hipStream_t s) {
+    hipError_t e;
+    if ((e = one<OpScanTilesImpl>(j, blocks_cells(kBScanTiles, j), s)) != hipSuccess) return e;
+    if ((e = one<OpScanSumsImpl>(j, blocks_cells(kBScanSums, j), s)) != hipSuccess) return e;
+    if ((e = one<OpScanAddImpl>(j, blocks_cells(kBScanAdd, j), s)) != hipSuccess) return e;
+    if ((e = one<OpPlaceImpl>(j, blocks_cells(kBPlace, j), s)) != hipSuccess) return e;
+    return one<OpGatherImpl>(j, blocks_cells(kBGather, j), s);
 }
 
-hipError_t launch_sort_gather(const DevParams &P, KindBuffers &k, bool has_ref, hipStream_t s) {
-    (void)P;
-    if (k.n == 0) return hipSuccess;
-    const int g = grid_for(k.n);
-    hipLaunchKernelGGL(k_place, dim3(g), dim3(kBlock), 0, s, k.n, k.key, k.rank, k.start, k.id[k.ord], k.tmp_slot,
-                       k.tmp_id);
-    const int so = k.st, sn = 1 - k.st, oo = k.ord, on = 1 - k.ord;
-    if (has_ref)
-        hipLaunchKernelGGL(k_rank_gather<true>, dim3(g), dim3(kBlock), 0, s, k.n, k.key, k.start, k.tmp_slot,
-                           k.tmp_id, k.pos[so], k.vel[so], k.ref[oo], k.pos[sn], k.vel[sn], k.ref[on], k.id[on],
-                           k.cell[on]);
-    else
-        hipLaunchKernelGGL(k_rank_gather<false>, dim3(g), dim3(kBlock), 0, s, k.n, k.key, k.start, k.tmp_slot,
-                           k.tmp_id, k.pos[so], k.vel[so], (const float4 *)nullptr, k.pos[sn], k.vel[sn],
-                           (float4 *)nullptr, k.id[on], k.cell[on]);
-    k.st = sn;
-    k.ord = on;
-    return hipGetLastError();
+hipError_t launch_cells_one(const CellJob &j, hipStream_t s) {
+    hipError_t e = launch_hash_one(j, s);
+    return e != hipSuccess ? e : launch_sort_one(j, s);
 }
 
 }  // namespace fsa

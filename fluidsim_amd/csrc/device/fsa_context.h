@@ -1,0 +1,102 @@
+// fsa_context.h -- the per-scene context behind the opaque fsa_sim handle and
+// the helpers the C ABI files share.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fluidsim_amd.h"
+#include "fsa_internal.h"
+
+namespace fsa {
+
+// records the message for fsa_last_error() and returns `code`
+int set_err(int code, const std::string &msg);
+
+}  // namespace fsa
+
+#define FSA_HIP_TRY(expr)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return ::fsa::set_err(e_ == hipErrorOutOfMemory ? FSA_ERR_NOMEM : FSA_ERR_HIP,         \
+                                  std::string(#expr) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+struct fsa_sim {
+    fsa_params params{};
+    fsa::DevParams P{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+
+    fsa::KindBuffers fluid, solid, virt;
+    fsa::SolidExtra sx;
+    fsa::VirtExtra vx;
+    float4 *virt_x = nullptr;  // constant wall positions (x, y, z, 0)
+    float4 *rates = nullptr;   // fluid (ax, ay, az, drho) when debugging
+    unsigned long long *jumps = nullptr;
+    // observables
+    double *obs_partial = nullptr, *obs_out = nullptr;
+    float tip_y = 3.0e38f;  // reference y of the gate tip row (none: above everything)
+    unsigned long long *census = nullptr;
+    float4 *probe_pts = nullptr;
+    float *probe_out = nullptr;
+    int probe_cap = 0;
+
+    bool uploaded[3] = {false, false, false};
+    bool hashed = false;  // cell ids/ranks of the current order are fresh
+    unsigned debug = 0;
+    // unique per context over the process lifetime: a batch keys its recorded
+    // launches on it (a new context may reuse a freed one's address)
+    uint64_t serial = 0;
+
+    // stage timing: a ring of event pairs per stage, read back lazily so the
+    // host never waits on the GPU while it enqueues steps
+    static constexpr int kRing = 256;
+    bool timing = false;
+    hipEvent_t ev_begin[FSA_STAGE_COUNT][kRing] = {};
+    hipEvent_t ev_end[FSA_STAGE_COUNT][kRing] = {};
+    int ev_head[FSA_STAGE_COUNT] = {};     // next ring slot
+    int ev_pending[FSA_STAGE_COUNT] = {};  // recorded, not yet read back
+    bool ev_open[FSA_STAGE_COUNT] = {};
+    double stage_ms[FSA_STAGE_COUNT] = {};
+    int64_t stage_n[FSA_STAGE_COUNT] = {};
+
+    std::vector<void *> allocations;
+    int64_t bytes = 0;
+
+    template <class T> int alloc(T **p, size_t count) {
+        *p = nullptr;
+        if (count == 0) return FSA_OK;
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, count * sizeof(T));
+        if (e != hipSuccess)
+            return fsa::set_err(FSA_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                                                   " bytes: " + hipGetErrorString(e));
+        allocations.push_back(q);
+        bytes += (int64_t)(count * sizeof(T));
+        *p = static_cast<T *>(q);
+        return FSA_OK;
+    }
+};
+
+namespace fsa {
+
+// FSA_ERR_STATE unless every kind with particles has been uploaded
+int require_uploaded(fsa_sim *s);
+
+// stage arguments over given fluid/solid buffer indices (a batch plans a
+// step ahead of the context's own indices) ...
+SolidArgs solid_args(const fsa_sim *s, const KindBuffers &f, const KindBuffers &so);
+VirtArgs virt_args(const fsa_sim *s, const KindBuffers &f, int compute_normals);
+FluidStepArgs fluid_args(const fsa_sim *s, const KindBuffers &f, const KindBuffers &so);  // rates off
+// ... and over the context's current ones
+inline SolidArgs solid_args(const fsa_sim *s) { return solid_args(s, s->fluid, s->solid); }
+inline VirtArgs virt_args(const fsa_sim *s, int compute_normals) { return virt_args(s, s->fluid, compute_normals); }
+inline FluidStepArgs fluid_args(const fsa_sim *s) { return fluid_args(s, s->fluid, s->solid); }
+
+}  // namespace fsa

@@ -165,9 +165,9 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cel
 }
 
 template <bool RATES>
-__global__ __launch_bounds__(kBlock) void k_fluid(DevParams P, FluidStepArgs a) {
+__device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepArgs &a, int blk, int nblk) {
     __shared__ int s_list[kListCap * kBlock];
-    const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const int i = xcd_tile(blk, nblk) * kBlock + threadIdx.x;
     if (i >= P.nf) return;  // lists are lane-private: no barrier follows
     int *list = s_list + threadIdx.x;
 
@@ -219,15 +219,29 @@ __global__ __launch_bounds__(kBlock) void k_fluid(DevParams P, FluidStepArgs a) 
     a.fvel_out[i] = make_float4(ux, uy, uz, p);
 }
 
+template <bool RATES>
+struct OpFluid {
+    static __device__ void run(const FluidJob &j, int blk, int nblk) { fluid_body<RATES>(j.P, j.a, blk, nblk); }
+};
+
 }  // namespace
+
+int blocks_fluid(const FluidJob &j) { return blocks_for(j.P.nf); }
 
 hipError_t launch_fluid(const DevParams &P, const FluidStepArgs &a, hipStream_t s) {
     if (P.nf == 0) return hipSuccess;
-    const dim3 grid((P.nf + kBlock - 1) / kBlock);
+    const FluidJob j{P, a};
+    const dim3 grid(blocks_fluid(j));
     if (a.rates)
-        hipLaunchKernelGGL(k_fluid<true>, grid, dim3(kBlock), 0, s, P, a);
+        hipLaunchKernelGGL((k_one<OpFluid<true>, FluidJob>), grid, dim3(kBlock), 0, s, j);
     else
-        hipLaunchKernelGGL(k_fluid<false>, grid, dim3(kBlock), 0, s, P, a);
+        hipLaunchKernelGGL((k_one<OpFluid<false>, FluidJob>), grid, dim3(kBlock), 0, s, j);
+    return hipGetLastError();
+}
+
+hipError_t launch_many_fluid(const FluidJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s) {
+    if (nblocks <= 0 || njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_many<OpFluid<false>, FluidJob>), dim3(nblocks), dim3(kBlock), 0, s, jobs, prefix, njobs);
     return hipGetLastError();
 }
 

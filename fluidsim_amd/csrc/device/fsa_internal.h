@@ -15,6 +15,14 @@
 // other buffer, and the fused fluid pair+update kernel reads one buffer and
 // writes the other, so every kernel is a Jacobi sweep with one writer per
 // output element.
+//
+// Every kernel is written as an operation `Op::run(job, block, nblocks)`
+// over one scene's job descriptor.  k_one<Op> launches it for one scene
+// (the job travels as the kernel argument); k_many<Op> launches it for many
+// scenes at once: the grid is the concatenation of the scenes' block ranges,
+// each block finds its scene in a prefix table and reads the scene's job
+// from device memory.  Both run the same code, so a batched step gives the
+// same bits as stepping every scene on its own.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -32,6 +40,7 @@ constexpr int kScanTile = 1024;  // elements per scan workgroup (256 threads x 4
 // batch of candidate loads may run past the end of a slot range (the
 // results past the range are masked out of every test)
 constexpr int kPosPad = 16;
+constexpr int kSolidLanes = 16;  // lanes serving one solid particle
 
 // the kernels' view of fsa_params
 struct DevParams {
@@ -72,6 +81,7 @@ struct KindBuffers {
     int32_t *rank = nullptr;   // arrival slot inside the cell (atomic)
     int32_t *tmp_slot = nullptr;
     int32_t *tmp_id = nullptr;
+    int32_t *tile_sums = nullptr;  // one per scan tile
 };
 
 struct SolidExtra {
@@ -86,22 +96,55 @@ struct VirtExtra {
     bool normals_ready = false;
 };
 
-struct ScanScratch {
-    int32_t *tile_sums = nullptr;
-    int ntiles_cap = 0;
+__host__ __device__ inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
+__host__ __device__ inline int scan_tiles(int ncell) { return (ncell + kScanTile - 1) / kScanTile; }
+__host__ __device__ inline int padded_cells(int ncell) { return scan_tiles(ncell) * kScanTile; }
+
+// ---- generic wrappers -------------------------------------------------------
+
+template <class Op, class Job>
+__global__ __launch_bounds__(kBlock) void k_one(Job job) {
+    Op::run(job, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// prefix[s] is the first block of job s, prefix[n] the grid size
+template <class Op, class Job>
+__global__ __launch_bounds__(kBlock) void k_many(const Job *__restrict__ jobs, const int *__restrict__ prefix, int n) {
+    const int b = (int)blockIdx.x;
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const Job job = jobs[lo];  // a private copy: fields stay in registers across the body's stores
+    Op::run(job, b - prefix[lo], prefix[lo + 1] - prefix[lo]);
+}
+
+// ---- cell build (fsa_cells.hip) ---------------------------------------------
+// One kind of one scene.  Grids: Fill/ScanAdd over cells, Hash/Place/Gather
+// over particles, ScanTiles over scan tiles, ScanSums a single block.
+struct CellJob {
+    DevParams P;
+    int n;
+    int has_ref;
+    int count_jumps;
+    const float4 *pos_in, *vel_in, *ref_in;
+    const int32_t *id_in, *prev_cell;
+    float4 *pos_out, *vel_out, *ref_out;
+    int32_t *id_out, *cell_out;
+    int32_t *key, *rank, *count, *start, *tmp_slot, *tmp_id, *tile_sums;
+    unsigned long long *jumps;
 };
-
-// ---- launchers (fsa_cells.hip) -------------------------------------------
+// the job of kind k reading the current buffers and writing the other ones
+CellJob make_cell_job(const DevParams &P, const KindBuffers &k, unsigned long long *jumps, bool count_jumps);
+// the whole build (fill, hash, scan, place, gather) for one job
+hipError_t launch_cells_one(const CellJob &j, hipStream_t s);
+hipError_t launch_hash_one(const CellJob &j, hipStream_t s);     // fill + hash
+hipError_t launch_sort_one(const CellJob &j, hipStream_t s);     // scan + place + gather
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int n, hipStream_t s);
-// cell id of every slot, arrival ranks, per-cell counts, jump diagnostic
-hipError_t launch_hash_count(const DevParams &P, KindBuffers &k, unsigned long long *jumps, bool count_jumps,
-                             hipStream_t s);
-// exclusive scan count[0..ncell) -> start[0..ncell]
-hipError_t launch_scan(const int32_t *count, int32_t *start, int ncell, ScanScratch &scr, hipStream_t s);
-// place + rank + gather into the other buffers; flips ord/st
-hipError_t launch_sort_gather(const DevParams &P, KindBuffers &k, bool has_ref, hipStream_t s);
 
-// ---- launchers (fsa_fluid.hip) -------------------------------------------
+// ---- fluid pass (fsa_fluid.hip) -----------------------------------------------
 struct FluidStepArgs {
     const float4 *fpos, *fvel;
     const int32_t *fcell;
@@ -111,20 +154,28 @@ struct FluidStepArgs {
     float4 *fpos_out, *fvel_out;
     float4 *rates;  // (ax, ay, az, drho) or null
 };
+struct FluidJob {
+    DevParams P;
+    FluidStepArgs a;
+};
 hipError_t launch_fluid(const DevParams &P, const FluidStepArgs &a, hipStream_t s);
 
-// ---- launchers (fsa_virt.hip) --------------------------------------------
+// ---- walls (fsa_virt.hip) ------------------------------------------------------
 struct VirtArgs {
     const float4 *vpos_in;
     float4 *vpos, *vvel, *vnrm;
     const int32_t *vcell;
     const int32_t *fstart, *vstart;
     const float4 *fpos, *fvel;
-    bool compute_normals;
+    int compute_normals;
+};
+struct VirtJob {
+    DevParams P;
+    VirtArgs a;
 };
 hipError_t launch_virt(const DevParams &P, const VirtArgs &a, hipStream_t s);
 
-// ---- launchers (fsa_solid.hip) -------------------------------------------
+// ---- solid (fsa_solid.hip) -------------------------------------------------------
 struct SolidArgs {
     float4 *spos, *svel;
     const float4 *sref;
@@ -133,12 +184,36 @@ struct SolidArgs {
     const float4 *fpos, *fvel;
     SolidExtra x;  // device pointers, passed by value
 };
+struct SolidJob {
+    DevParams P;
+    SolidArgs a;
+};
 hipError_t launch_couple(const DevParams &P, const SolidArgs &a, hipStream_t s);
 hipError_t launch_solid_deformation(const DevParams &P, const SolidArgs &a, hipStream_t s);
 hipError_t launch_solid_force(const DevParams &P, const SolidArgs &a, hipStream_t s);
 hipError_t launch_solid_update(const DevParams &P, const SolidArgs &a, hipStream_t s);
 
-// ---- launchers (fsa_observe.hip) -------------------------------------------
+// ---- batched launches ----------------------------------------------------------
+// Each launches one grid over all jobs of a table: `jobs` and `prefix` are
+// device arrays, prefix[s] the first block of job s and prefix[njobs] = nblocks.
+// blocks_* give the grid size of one job (0 when it has no work); the host
+// rounds every job's range up to whole XCD rounds (multiples of 8 blocks) so
+// a block's XCD is the same in the batched grid as in a single-scene grid.
+enum BatchOp {
+    kBFillCounts, kBHash, kBScanTiles, kBScanSums, kBScanAdd, kBPlace, kBGather,  // CellJob
+    kBCouple, kBDeform, kBForce, kBSolidUpdate                                     // SolidJob
+};
+hipError_t launch_many_cells(BatchOp op, const CellJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s);
+int blocks_cells(BatchOp op, const CellJob &j);
+hipError_t launch_many_fluid(const FluidJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s);
+int blocks_fluid(const FluidJob &j);
+hipError_t launch_many_virt(const VirtJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s);
+int blocks_virt(const VirtJob &j);
+hipError_t launch_many_solid(BatchOp op, const SolidJob *jobs, const int *prefix, int njobs, int nblocks,
+                             hipStream_t s);
+int blocks_solid(BatchOp op, const SolidJob &j);
+
+// ---- observables (fsa_observe.hip) ------------------------------------------------
 constexpr int kObserveSums = 14;
 struct ObserveArgs {
     const float4 *fpos, *fvel;
@@ -150,7 +225,6 @@ struct CensusArgs {
     const float4 *fpos, *spos, *vpos;
     const int32_t *fcell, *fstart, *sstart, *vstart;
 };
-__global__ void k_fill_u64(unsigned long long *p, int n);
 hipError_t launch_census(const DevParams &P, const CensusArgs &a, unsigned long long *out, hipStream_t s);
 int observe_partials();  // doubles of scratch the sums need
 hipError_t launch_observe(const DevParams &P, const ObserveArgs &a, double *partial, double *out, hipStream_t s);

@@ -22,7 +22,7 @@ namespace fsa {
 
 namespace {
 
-constexpr int kSub = 16;
+constexpr int kSub = kSolidLanes;
 
 struct M3 {
     float m[3][3];  // row-major: m[r][c]
@@ -98,14 +98,14 @@ struct Lane {
     int i;    // solid slot served by this group
     int sub;  // lane within the group
 };
-__device__ __forceinline__ Lane lane_of() {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ Lane lane_of(int blk) {
+    const int t = blk * kBlock + threadIdx.x;
     return {t / kSub, t % kSub};
 }
 
 // ---- K6: solidCoupleUpdate (src/solid.cu:262-344) ------------------------
-__global__ __launch_bounds__(kBlock) void k_couple(DevParams P, SolidArgs a) {
-    const Lane L = lane_of();
+__device__ __forceinline__ void couple_body(const DevParams &P, const SolidArgs &a, int blk) {
+    const Lane L = lane_of(blk);
     if (L.i >= P.ns) return;  // whole groups leave together
     const float4 xi4 = a.spos[L.i];
     const float xi = xi4.x, yi = xi4.y, zi = xi4.z;
@@ -155,8 +155,8 @@ __global__ __launch_bounds__(kBlock) void k_couple(DevParams P, SolidArgs a) {
 }
 
 // ---- K10: solidDeformationCalculate (src/solid.cu:10-79) -----------------
-__global__ __launch_bounds__(kBlock) void k_deformation(DevParams P, SolidArgs a) {
-    const Lane L = lane_of();
+__device__ __forceinline__ void deformation_body(const DevParams &P, const SolidArgs &a, int blk) {
+    const Lane L = lane_of(blk);
     if (L.i >= P.ns) return;
     const int i = L.i;
     const float4 Xi = a.sref[i];
@@ -224,8 +224,8 @@ __global__ __launch_bounds__(kBlock) void k_deformation(DevParams P, SolidArgs a
 }
 
 // ---- K11: solidForceCalculate (src/solid.cu:81-207) ----------------------
-__global__ __launch_bounds__(kBlock) void k_force(DevParams P, SolidArgs a) {
-    const Lane L = lane_of();
+__device__ __forceinline__ void force_body(const DevParams &P, const SolidArgs &a, int blk) {
+    const Lane L = lane_of(blk);
     if (L.i >= P.ns) return;
     const int i = L.i;
     const int n = P.ns;
@@ -320,8 +320,8 @@ __global__ __launch_bounds__(kBlock) void k_force(DevParams P, SolidArgs a) {
 }
 
 // ---- K12: solidUpdate (src/solid.cu:209-239) -----------------------------
-__global__ __launch_bounds__(kBlock) void k_solid_update(DevParams P, SolidArgs a) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void update_body(const DevParams &P, const SolidArgs &a, int blk) {
+    const int i = blk * kBlock + threadIdx.x;
     if (i >= P.ns) return;
     const int type = __float_as_int(a.sref[i].w);
     const float dt = P.dt_s;
@@ -340,29 +340,59 @@ __global__ __launch_bounds__(kBlock) void k_solid_update(DevParams P, SolidArgs 
     }
 }
 
-inline dim3 grid_for(int n) { return dim3((n + kBlock - 1) / kBlock); }
-inline dim3 grid_groups(int n) { return dim3((int)(((long long)n * kSub + kBlock - 1) / kBlock)); }
+struct OpCouple {
+    static __device__ void run(const SolidJob &j, int blk, int) { couple_body(j.P, j.a, blk); }
+};
+struct OpDeform {
+    static __device__ void run(const SolidJob &j, int blk, int) { deformation_body(j.P, j.a, blk); }
+};
+struct OpForce {
+    static __device__ void run(const SolidJob &j, int blk, int) { force_body(j.P, j.a, blk); }
+};
+struct OpSolidUpdate {
+    static __device__ void run(const SolidJob &j, int blk, int) { update_body(j.P, j.a, blk); }
+};
+
+template <class Op>
+hipError_t one(const DevParams &P, const SolidArgs &a, int blocks, hipStream_t s) {
+    if (blocks <= 0) return hipSuccess;
+    const SolidJob j{P, a};
+    hipLaunchKernelGGL((k_one<Op, SolidJob>), dim3(blocks), dim3(kBlock), 0, s, j);
+    return hipGetLastError();
+}
 
 }  // namespace
 
+int blocks_solid(BatchOp op, const SolidJob &j) {
+    if (j.P.ns == 0) return 0;
+    if (op == kBSolidUpdate) return blocks_for(j.P.ns);
+    return blocks_for((long long)j.P.ns * kSub);  // couple, deformation, force: kSub lanes per particle
+}
+
 hipError_t launch_couple(const DevParams &P, const SolidArgs &a, hipStream_t s) {
-    if (P.ns == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_couple, grid_groups(P.ns), dim3(kBlock), 0, s, P, a);
-    return hipGetLastError();
+    return one<OpCouple>(P, a, blocks_solid(kBCouple, SolidJob{P, a}), s);
 }
 hipError_t launch_solid_deformation(const DevParams &P, const SolidArgs &a, hipStream_t s) {
-    if (P.ns == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_deformation, grid_groups(P.ns), dim3(kBlock), 0, s, P, a);
-    return hipGetLastError();
+    return one<OpDeform>(P, a, blocks_solid(kBDeform, SolidJob{P, a}), s);
 }
 hipError_t launch_solid_force(const DevParams &P, const SolidArgs &a, hipStream_t s) {
-    if (P.ns == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_force, grid_groups(P.ns), dim3(kBlock), 0, s, P, a);
-    return hipGetLastError();
+    return one<OpForce>(P, a, blocks_solid(kBForce, SolidJob{P, a}), s);
 }
 hipError_t launch_solid_update(const DevParams &P, const SolidArgs &a, hipStream_t s) {
-    if (P.ns == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_solid_update, grid_for(P.ns), dim3(kBlock), 0, s, P, a);
+    return one<OpSolidUpdate>(P, a, blocks_solid(kBSolidUpdate, SolidJob{P, a}), s);
+}
+
+hipError_t launch_many_solid(BatchOp op, const SolidJob *jobs, const int *prefix, int njobs, int nblocks,
+                             hipStream_t s) {
+    if (nblocks <= 0 || njobs <= 0) return hipSuccess;
+    const dim3 g(nblocks), b(kBlock);
+    switch (op) {
+    case kBCouple: hipLaunchKernelGGL((k_many<OpCouple, SolidJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    case kBDeform: hipLaunchKernelGGL((k_many<OpDeform, SolidJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    case kBForce: hipLaunchKernelGGL((k_many<OpForce, SolidJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    case kBSolidUpdate: hipLaunchKernelGGL((k_many<OpSolidUpdate, SolidJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

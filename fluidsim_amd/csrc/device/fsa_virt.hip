@@ -16,8 +16,8 @@ namespace fsa {
 namespace {
 
 template <bool NORMALS>
-__global__ __launch_bounds__(kBlock) void k_virt(DevParams P, VirtArgs a) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void virt_body(const DevParams &P, const VirtArgs &a, int blk) {
+    const int i = blk * kBlock + threadIdx.x;
     if (i >= P.nv) return;
     const float4 xi4 = a.vpos_in[i];
     const float xi = xi4.x, yi = xi4.y, zi = xi4.z;
@@ -92,15 +92,27 @@ __global__ __launch_bounds__(kBlock) void k_virt(DevParams P, VirtArgs a) {
     a.vvel[i] = make_float4(ux - m * nx, uy - m * ny, uz - m * nz, pressure);
 }
 
+struct OpVirt {
+    static __device__ void run(const VirtJob &j, int blk, int) {
+        if (j.a.compute_normals) virt_body<true>(j.P, j.a, blk);
+        else virt_body<false>(j.P, j.a, blk);
+    }
+};
+
 }  // namespace
+
+int blocks_virt(const VirtJob &j) { return blocks_for(j.P.nv); }
 
 hipError_t launch_virt(const DevParams &P, const VirtArgs &a, hipStream_t s) {
     if (P.nv == 0) return hipSuccess;
-    const dim3 grid((P.nv + kBlock - 1) / kBlock);
-    if (a.compute_normals)
-        hipLaunchKernelGGL(k_virt<true>, grid, dim3(kBlock), 0, s, P, a);
-    else
-        hipLaunchKernelGGL(k_virt<false>, grid, dim3(kBlock), 0, s, P, a);
+    const VirtJob j{P, a};
+    hipLaunchKernelGGL((k_one<OpVirt, VirtJob>), dim3(blocks_virt(j)), dim3(kBlock), 0, s, j);
+    return hipGetLastError();
+}
+
+hipError_t launch_many_virt(const VirtJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s) {
+    if (nblocks <= 0 || njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_many<OpVirt, VirtJob>), dim3(nblocks), dim3(kBlock), 0, s, jobs, prefix, njobs);
     return hipGetLastError();
 }
 

@@ -190,3 +190,45 @@ class DeviceSim:
         v = C.c_int64()
         check(self.lib.fsa_memory_used(self.handle, C.byref(v)))
         return int(v.value)
+
+
+class DeviceBatch:
+    """Executor that advances many contexts with one launch per stage
+    (fsa_batch_* of include/fluidsim_amd.h).  The list of contexts is given on
+    every call and may change between calls."""
+
+    def __init__(self, stream: int | None = None, graphs: bool = True):
+        self.lib = _native.device_lib()
+        h = C.c_void_p()
+        flags = 0 if graphs else _native.FSA_BATCH_NO_GRAPH
+        check(self.lib.fsa_batch_create(C.c_void_p(stream) if stream else None, flags, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.fsa_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self, scenes, n: int = 1):
+        arr = (C.c_void_p * max(1, len(scenes)))(*[s.handle for s in scenes])
+        check(self.lib.fsa_batch_step(self.handle, arr, len(scenes), n))
+
+    def synchronize(self):
+        check(self.lib.fsa_batch_synchronize(self.handle))
+
+    def stats(self) -> dict:
+        plans, steps = C.c_int64(), C.c_int64()
+        check(self.lib.fsa_batch_stats(self.handle, C.byref(plans), C.byref(steps)))
+        return {"plans_built": int(plans.value), "steps_run": int(steps.value)}

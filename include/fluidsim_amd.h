@@ -251,6 +251,38 @@ int fsa_reset_stage_times(fsa_sim *sim);
 /* Device bytes held by the context. */
 int fsa_memory_used(fsa_sim *sim, int64_t *bytes);
 
+/* ---- batches: many scenes per launch ------------------------------------
+ *
+ * The reference runs one scene per process: its parameter and pointer
+ * blocks are process-global (src/sim.cu:9-10, src/memory_copy.cu:5-6) and its
+ * step loop is DeviceCalculation::calculation (src/calculation.cu:23-33).  A
+ * batch advances any number of independent contexts through that same stage
+ * sequence with one launch per stage for all of them, recorded as a HIP graph
+ * per step.  Every context's result is bit-identical to fsa_step on it alone;
+ * contexts may differ in every parameter, particle count and solid_sub_step.
+ *
+ * A batch is an executor, not an owner: the caller names the contexts on
+ * every fsa_batch_step call and may change the list between calls
+ * (continuous batching).  The recorded launches are reused while the list
+ * and the contexts' buffers stay the same and rebuilt otherwise (a new list,
+ * or a context stepped on its own in between).  All contexts must live on
+ * the batch's device.  The batch stream first waits for work queued on each
+ * context's stream, and each context's stream then waits for the batch, so
+ * stage calls and downloads on a context stay ordered after the batch step.
+ * Per-stage timing and FSA_DEBUG_PAIR_SUMS are not recorded in a batch. */
+typedef struct fsa_batch fsa_batch;
+
+#define FSA_BATCH_NO_GRAPH 1u /* launch every stage directly instead of replaying a HIP graph */
+
+/* stream: hipStream_t to run on, or NULL for a batch-owned stream. */
+int fsa_batch_create(void *stream, unsigned flags, fsa_batch **out);
+int fsa_batch_destroy(fsa_batch *batch);
+/* n_steps full steps of every listed context (no context listed twice). */
+int fsa_batch_step(fsa_batch *batch, fsa_sim *const *scenes, int n_scenes, int n_steps);
+int fsa_batch_synchronize(fsa_batch *batch);
+/* How often the launches were (re)recorded, and steps run, since creation. */
+int fsa_batch_stats(fsa_batch *batch, int64_t *plans_built, int64_t *steps_run);
+
 #ifdef __cplusplus
 }
 #endif
