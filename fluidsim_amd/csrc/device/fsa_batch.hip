@@ -70,7 +70,7 @@ void apply_phase(fsa_sim *s, const Phase &p) {
 
 // one launch of a step: which op, which job table, its prefix table
 struct Launch {
-    enum Family { kCells, kVirt, kFluid, kSolid } family;
+    enum Family { kCells, kCouple, kVirt, kFluid, kSolid } family;
     BatchOp op;
     int njobs;
     size_t prefix_off;  // byte offset of the prefix table in the blob
@@ -79,7 +79,7 @@ struct Launch {
 
 // the launches and tables of one step
 struct StepTables {
-    size_t cells_off = 0, virt_off = 0, fluid_off = 0, solid_off = 0;
+    size_t cells_off = 0, couple_off = 0, virt_off = 0, fluid_off = 0, solid_off = 0;
     int ncells = 0, nscenes = 0;
     std::vector<Launch> launches;
 };
@@ -147,7 +147,7 @@ void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph
     std::vector<CellJob> cells;
     std::vector<VirtJob> virt(n);
     std::vector<FluidJob> fluid(n);
-    std::vector<SolidJob> solid(n);
+    std::vector<SolidJob> couple(n), solid(n);
     int smax = 0;
     for (int i = 0; i < n; ++i) {
         const fsa_sim *s = sims[i];
@@ -167,8 +167,11 @@ void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph
             so.st ^= 1;
             so.ord ^= 1;
         }
+        couple[i] = SolidJob{s->P, solid_args(s, f, so)};
         virt[i] = VirtJob{s->P, virt_args(s, f, 0)};
         fluid[i] = FluidJob{s->P, fluid_args(s, f, so)};
+        // the solid sub-steps see the fluid after its pass (src/calculation.cu:31-32)
+        if (f.n) f.st ^= 1;
         solid[i] = SolidJob{s->P, solid_args(s, f, so)};
         smax = std::max(smax, s->P.ns > 0 ? s->P.sub : 0);
     }
@@ -176,6 +179,7 @@ void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph
     t.nscenes = n;
     t.ncells = (int)cells.size();
     t.cells_off = put(blob, cells.data(), cells.size());
+    t.couple_off = put(blob, couple.data(), couple.size());
     t.virt_off = put(blob, virt.data(), virt.size());
     t.fluid_off = put(blob, fluid.data(), fluid.size());
     t.solid_off = put(blob, solid.data(), solid.size());
@@ -188,8 +192,8 @@ void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph
     }
     // couple (src/solid.cu:346-359: only with both fluid and solid)
     blocks.clear();
-    for (const SolidJob &j : solid) blocks.push_back(j.P.nf > 0 ? blocks_solid(kBCouple, j) : 0);
-    add_launch(blob, t, Launch::kSolid, kBCouple, blocks);
+    for (const SolidJob &j : couple) blocks.push_back(j.P.nf > 0 ? blocks_solid(kBCouple, j) : 0);
+    add_launch(blob, t, Launch::kCouple, kBCouple, blocks);
     blocks.clear();
     for (const VirtJob &j : virt) blocks.push_back(blocks_virt(j));
     add_launch(blob, t, Launch::kVirt, kBFillCounts, blocks);
@@ -212,6 +216,10 @@ hipError_t enqueue_step(fsa_batch *b, const StepTables &t) {
         switch (l.family) {
         case Launch::kCells:
             e = launch_many_cells(l.op, reinterpret_cast<const CellJob *>(base + t.cells_off), prefix, l.njobs,
+                                  l.nblocks, b->stream);
+            break;
+        case Launch::kCouple:
+            e = launch_many_solid(l.op, reinterpret_cast<const SolidJob *>(base + t.couple_off), prefix, l.njobs,
                                   l.nblocks, b->stream);
             break;
         case Launch::kVirt:

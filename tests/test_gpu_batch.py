@@ -1,0 +1,128 @@
+"""Batched stepping: many contexts advanced with one launch per stage.
+
+Every context's state after a batched step must be bit-identical to the same
+context stepped on its own with fsa_step (integer and float fields alike:
+both run the same per-scene operation code, only the grid differs).  The
+scenes differ in size, particle kinds present, stiffness and solid
+sub-step count; the batch list changes between calls, a scene is stepped on
+its own in between (which must force new recorded launches), and the step
+counts are odd so both recorded steps of a plan are exercised.
+"""
+import numpy as np
+import pytest
+
+from fluidsim_amd import scenes
+from fluidsim_amd._native import FLUID, SOLID, VIRT, FsaError
+from fluidsim_amd.sim import DeviceBatch, DeviceSim
+
+pytestmark = pytest.mark.gpu
+
+
+def _variants():
+    base = scenes.tiny_scene(seed=11)
+    return [
+        ("base", base),
+        ("stiff_s6", base.with_(seed=12, youngs_modulus=8.0e6, solid_sub_step=6)),
+        ("soft_s2", base.with_(seed=13, youngs_modulus=1.0e6, solid_sub_step=2, fluid=(10, 8, 6))),
+        ("no_gate", base.with_(seed=14, gate=None)),
+        ("no_walls", base.with_(seed=15, wall_layers=0)),
+        ("no_fluid", base.with_(seed=16, fluid=(0, 0, 0))),
+        ("bigger", scenes.tiny_scene(seed=17, fluid=(12, 12, 8), tank=(22, 16, 8), gate=(2, 10, 8), gate_x0=16,
+                                     solid_sub_step=3)),
+    ]
+
+
+def _make(sc):
+    a = sc.arrays()
+    p = sc.params((len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])))
+    d = DeviceSim(p)
+    d.upload(a)
+    return d
+
+
+def _state(d: DeviceSim) -> dict:
+    out = {}
+    if d.nf:
+        out.update({"f_" + k: v for k, v in d.fluid().items()})
+    if d.ns:
+        out.update({"s_" + k: v for k, v in d.solid().items()})
+    if d.nv:
+        out.update({"v_" + k: v for k, v in d.virt().items()})
+    for kind, name in ((FLUID, "fluid"), (SOLID, "solid"), (VIRT, "virt")):
+        out["cells_" + name] = d.cell_ids(kind)
+        out["order_" + name] = d.sorted_ids(kind)
+        out["start_" + name] = d.cell_start(kind)
+    out["jumps"] = np.array([d.jump_count()])
+    return out
+
+
+def _assert_same(name, got, want):
+    assert got.keys() == want.keys()
+    for k in want:
+        g, w = got[k], want[k]
+        assert g.shape == w.shape, (name, k)
+        # bit-identical, NaN patterns included
+        assert np.array_equal(g.view(np.uint32) if g.dtype == np.float32 else g,
+                              w.view(np.uint32) if w.dtype == np.float32 else w), f"{name}: {k} differs"
+
+
+@pytest.mark.parametrize("graphs", [True, False], ids=["graph", "direct"])
+def test_batched_steps_are_bit_identical_to_single_steps(gpu_available, graphs):
+    cases = _variants()
+    alone = {n: _make(sc) for n, sc in cases}
+    mixed = {n: _make(sc) for n, sc in cases}
+    names = [n for n, _ in cases]
+    print("case batch", "graphs" if graphs else "direct", names, flush=True)
+    with DeviceBatch(graphs=graphs) as b:
+        # 1) all scenes, odd count: both recorded steps of the plan run
+        b.step([mixed[n] for n in names], 3)
+        for n in names:
+            alone[n].step(3)
+        assert b.stats() == {"plans_built": 1, "steps_run": 3}
+        # 2) same list again: the plan continues at the right recorded step
+        b.step([mixed[n] for n in names], 2)
+        for n in names:
+            alone[n].step(2)
+        assert b.stats()["plans_built"] == 1
+        # 3) one scene stepped on its own: its buffers moved, the plan is stale
+        mixed["base"].step(1)
+        alone["base"].step(1)
+        b.step([mixed[n] for n in names], 1)
+        for n in names:
+            alone[n].step(1)
+        assert b.stats()["plans_built"] == 2
+        # 4) continuous batching: a scene leaves, the order changes
+        sub = ["bigger", "no_walls", "stiff_s6", "base", "no_fluid"]
+        b.step([mixed[n] for n in sub], 3)
+        for n in sub:
+            alone[n].step(3)
+        assert b.stats()["plans_built"] == 3
+        b.synchronize()
+    for n in names:
+        _assert_same(n, _state(mixed[n]), _state(alone[n]))
+
+
+def test_batch_of_one_matches_and_reuses_its_plan(gpu_available):
+    sc = scenes.tiny_scene(seed=21)
+    one, ref = _make(sc), _make(sc)
+    with DeviceBatch() as b:
+        for _ in range(4):
+            b.step([one], 1)
+        ref.step(4)
+        assert b.stats() == {"plans_built": 1, "steps_run": 4}
+    _assert_same("single", _state(one), _state(ref))
+
+
+def test_batch_errors_are_status_codes(gpu_available):
+    sc = scenes.tiny_scene(seed=22)
+    d = _make(sc)
+    a = sc.arrays()
+    p = sc.params((len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])))
+    not_uploaded = DeviceSim(p)
+    with DeviceBatch() as b:
+        with pytest.raises(FsaError, match="listed twice"):
+            b.step([d, d], 1)
+        with pytest.raises(FsaError, match="uploaded"):
+            b.step([d, not_uploaded], 1)
+        b.step([], 1)  # nothing to do
+        assert b.stats()["steps_run"] == 0
