@@ -2,6 +2,7 @@
 """Throughput of the FluidSim step pipeline on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--scene c2|c1|tiny]
+    python bench.py --workload batch [--batch-scenes 64] [--steps K]
 
 One step is one pass of the reference's step loop body (src/calculation.cu:23-33:
 cell hash, cell lists, solid coupling fields, walls, fluid pair+update,
@@ -126,15 +127,93 @@ def cpu_baseline(params, arrays, budget_s: float, threads: int) -> dict:
     }
 
 
+def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: float) -> dict:
+    """Many scenes of uneven size and stiffness: one batched step of all of them
+    (one launch per stage, replayed as a HIP graph) against stepping them one
+    after another, each on its own (the plain per-scene solve).  Scenes/hour
+    is for solves of `sim_time` seconds of simulated time, from the measured
+    per-step times."""
+    from fluidsim_amd.sim import DeviceBatch
+
+    t = time.perf_counter()
+    family = scenes.batch_scenes(n_scenes, seed=seed)
+    sims, counts = [], []
+    for sc in family:
+        a = sc.arrays()
+        c = (len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"]))
+        d = DeviceSim(sc.params(c))
+        d.upload(a)
+        sims.append(d)
+        counts.append(c)
+    nf = sum(c[0] for c in counts)
+    ns = sum(c[1] for c in counts)
+    nv = sum(c[2] for c in counts)
+    mem = sum(d.memory_used() for d in sims)
+    subs = [sc.solid_sub_step for sc in family]
+    log(f"batch: {n_scenes} scenes, {nf} fluid + {ns} solid + {nv} walls, S {min(subs)}..{max(subs)}, "
+        f"{mem / 2**30:.2f} GiB (built in {time.perf_counter() - t:.1f}s)")
+    dt = family[0].time_step
+    solve_steps = math.ceil(sim_time / dt)
+
+    b = DeviceBatch()
+    b.step(sims, warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(sims, steps)
+    b.synchronize()
+    batched = (time.perf_counter() - t0) / steps
+    log(f"batch: batched {1e3 * batched:.3f} ms/step for all {n_scenes} scenes")
+
+    for d in sims:
+        d.step(warmup)
+    for d in sims:
+        d.synchronize()
+    t0 = time.perf_counter()
+    for d in sims:
+        d.step(steps)
+        d.synchronize()  # one plain solve after another
+    sequential = (time.perf_counter() - t0) / steps
+    log(f"batch: sequential {1e3 * sequential:.3f} ms/step summed over the scenes")
+    plans = b.stats()["plans_built"]
+    b.close()
+    for d in sims:
+        d.close()
+    per_hour = lambda step_s: n_scenes * 3600.0 / (solve_steps * step_s)  # noqa: E731
+    return {
+        "workload": f"{n_scenes} dam breaks (seed {seed}): fluid log-uniform in [20k, 500k] on columns of "
+                    "aspect ratio U[0.5, 2], gate E U[5, 50] MPa, S from the solid CFL, dp 4 mm, c0 30 m/s",
+        "scenes": n_scenes, "fluid": nf, "solid": ns, "walls": nv,
+        "solid_sub_steps": [min(subs), max(subs)],
+        "solve": f"{sim_time} s simulated = {solve_steps} steps of dt {dt:.4g} s",
+        "steps_timed": steps, "warmup": warmup,
+        "batched_ms_per_step": round(1e3 * batched, 4),
+        "sequential_ms_per_step": round(1e3 * sequential, 4),
+        "scenes_per_hour_batched": round(per_hour(batched), 1),
+        "scenes_per_hour_sequential": round(per_hour(sequential), 1),
+        "speedup": round(sequential / batched, 3),
+        "particle_steps_per_s_batched": (nf + ns) / batched,
+        "plans_built": plans,
+        "bit_identical_to_single_scene": "tests/test_gpu_batch.py",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 1000; batch workload 30)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 20; batch workload 3)")
     ap.add_argument("--scene", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--workload", choices=["step", "batch"], default="step",
+                    help="step: one scene's step throughput (the headline line, plus a short batched "
+                         "measurement); batch: only the many-scene batched measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched measurement of the step workload")
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=30)
+    ap.add_argument("--batch-seed", type=int, default=0)
+    ap.add_argument("--sim-time", type=float, default=0.5, help="simulated seconds per solve (scenes/hour)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (3 times): the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -156,6 +235,31 @@ def main():
                                    "(a scene is one GPU's share; see DESIGN.md for the sharding plan)"}))
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
+    if args.workload == "step":
+        args.steps = 1000 if args.steps is None else args.steps
+        args.warmup = 20 if args.warmup is None else args.warmup
+
+    if args.workload == "batch":
+        r = batch_bench(args.batch_scenes, args.steps or args.batch_steps,
+                        3 if args.warmup is None else args.warmup, args.batch_seed, args.sim_time)
+        out = {
+            "metric": "batched scenes/h",
+            "value": r["scenes_per_hour_batched"],
+            "unit": "scenes/h",
+            "n_gpus": 1,
+            "steps": r["steps_timed"],
+            "warmup": r["warmup"],
+            "ms_per_step": r["batched_ms_per_step"],
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic (seeded dam-break family, uploaded once)",
+            "config": {"workload": r["workload"]},
+            "batched": r,
+        }
+        print(json.dumps(out), flush=True)
+        return
 
     sc = scene_for(args.scene)
     t = time.perf_counter()
@@ -249,11 +353,13 @@ def main():
         "stage_ms": {k: (ms / n if n else None) for k, (ms, n) in stage.items()},
         "roofline": roofline,
     }
+    dev.close()
+    if not args.no_batch:
+        result["batched"] = batch_bench(args.batch_scenes, args.batch_steps, 3, args.batch_seed, args.sim_time)
     if rank == 0 and not args.no_cpu_baseline:
         log(f"cpu baseline: oracle FP32 on {args.cpu_threads} threads, ~{args.cpu_budget:.0f}s budget")
         result["cpu_baseline"] = cpu_baseline(params, arrays, args.cpu_budget, args.cpu_threads)
         log(f"cpu baseline: {result['cpu_baseline']['value']:.4g} particle-steps/s")
-    dev.close()
     if rank == 0:
         print(json.dumps(result), flush=True)
 

@@ -276,3 +276,71 @@ def tiny_scene(seed: int = 0, **kw) -> DamBreak:
                 wall_layers=2, seed=seed, sound_speed=20.0, youngs_modulus=2.0e6, solid_density=1100.0)
     base.update(kw)
     return DamBreak(**base)
+
+
+# ---- scene families: many scenes of uneven size and stiffness ----------------
+
+def solid_sound_speed(youngs_modulus: float, poisson_ratio: float, density: float) -> float:
+    """c_s = sqrt(K / rho) with K = E / (3 (1 - 2 nu)), as the host layer derives it
+    (src/properties.cpp:59-66)."""
+    return math.sqrt(youngs_modulus / (3.0 * (1.0 - 2.0 * poisson_ratio)) / density)
+
+
+def solid_sub_steps_for(sc: DamBreak, cfl: float = 0.3) -> int:
+    """Smallest sub-step count with dt / S <= cfl * h / c_s.
+
+    At cfl = 0.3 this gives the 4 sub-steps of the 12 MPa gate of the 43,750-particle
+    dam break (dt c_s / h = 1.12); a 50 MPa gate needs 8, a 5 MPa gate 3."""
+    cs = solid_sound_speed(sc.youngs_modulus, sc.poisson_ratio, sc.solid_density)
+    return max(1, math.ceil(sc.time_step * cs / (cfl * sc.h) - 1e-9))
+
+
+def column_scene(n_fluid: float, flow_ratio: float, width_ratio: float, youngs_modulus: float, seed: int,
+                 dp: float = 0.004, sound_speed: float | None = None) -> DamBreak:
+    """A dam break with about `n_fluid` fluid particles in a column of aspect
+    ratios flow/height = flow_ratio and width/height = width_ratio, a gate of a
+    given stiffness at 1.5 column lengths, and the sub-step count its solid CFL
+    needs.  sound_speed None: 10x the dam-break velocity bound sqrt(2 g H)."""
+    ny = max(4, round((n_fluid / (flow_ratio * width_ratio)) ** (1.0 / 3.0)))
+    nx = max(4, round(flow_ratio * ny))
+    nz = max(4, round(width_ratio * ny))
+    tank = (2 * nx + 10, ny + max(6, ny // 2), nz)
+    gate_h = max(8, ny // 2)
+    gate_w = max(2, nz - 2 * max(1, nz // 8))
+    c0 = sound_speed if sound_speed is not None else 10.0 * math.sqrt(2.0 * 9.81 * ny * dp)
+    sc = DamBreak(dp=dp, fluid=(nx, ny, nz), tank=tank, gate=(4, gate_h, gate_w), gate_x0=(3 * nx) // 2,
+                  gate_clamp_rows=3, wall_layers=3, seed=seed, sound_speed=float(c0),
+                  youngs_modulus=float(youngs_modulus))
+    return sc.with_(solid_sub_step=solid_sub_steps_for(sc))
+
+
+def batch_scenes(n: int = 64, seed: int = 0, fluid_range=(20_000, 500_000), e_range=(5.0e6, 50.0e6),
+                 dp: float = 0.004, sound_speed: float = 30.0) -> list:
+    """`n` independent dam breaks: fluid counts log-uniform in `fluid_range`,
+    column aspect ratios uniform in [0.5, 2], gate E uniform in `e_range`
+    (nu 0.4, rho 1100), one sound speed (so one dt) for all, S per scene from
+    the solid CFL."""
+    rng = np.random.default_rng(seed)
+    lo, hi = math.log(fluid_range[0]), math.log(fluid_range[1])
+    out = []
+    for i in range(n):
+        nf = math.exp(rng.uniform(lo, hi))
+        ax, az = rng.uniform(0.5, 2.0, size=2)
+        E = rng.uniform(*e_range)
+        out.append(column_scene(nf, ax, az, E, seed=seed * 1000 + i, dp=dp, sound_speed=sound_speed))
+    return out
+
+
+def serving_scenes(n: int, seed: int = 0, fluid_range=(10_000, 2_000_000), e_range=(5.0e6, 50.0e6),
+                   dp: float = 0.004) -> list:
+    """Scene-solve requests: fluid counts log-uniform in `fluid_range`, gate E
+    uniform in `e_range`, c0 = 10 sqrt(2 g H) per scene (so dt differs)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = math.log(fluid_range[0]), math.log(fluid_range[1])
+    out = []
+    for i in range(n):
+        nf = math.exp(rng.uniform(lo, hi))
+        ax, az = rng.uniform(0.5, 2.0, size=2)
+        E = rng.uniform(*e_range)
+        out.append(column_scene(nf, ax, az, E, seed=seed * 1000 + i, dp=dp))
+    return out
