@@ -3,6 +3,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--scene c2|c1|tiny]
     python bench.py --workload batch [--batch-scenes 64] [--steps K]
+    python bench.py --workload serve [--requests 16] [--load 0.8] [--sim-time 0.5]
 
 One step is one pass of the reference's step loop body (src/calculation.cu:23-33:
 cell hash, cell lists, solid coupling fields, walls, fluid pair+update,
@@ -197,6 +198,109 @@ def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: flo
     }
 
 
+def serve_bench(n_requests: int, seed: int, sim_time: float, load: float, chunk: int) -> dict:
+    """Scene-solve requests with Poisson arrivals served by continuous batching
+    (fluidsim_amd.serving.Server), against the same requests solved one at a
+    time in arrival order (each scene's solve time measured alone, the FIFO
+    queue then computed for the same arrival times)."""
+    from fluidsim_amd.serving import Request, Server, percentile
+    from fluidsim_amd.sim import DeviceBatch
+
+    rng = np.random.default_rng(seed + 7)
+    t = time.perf_counter()
+    family = scenes.serving_scenes(n_requests, seed=seed)
+    reqs, work = [], []
+    for i, sc in enumerate(family):
+        a = sc.arrays()
+        nf, ns = len(a["fluid_position"]), len(a["solid_position"])
+        steps = math.ceil(sim_time / sc.time_step)
+        reqs.append(Request(i, sc, steps, 0.0, a))
+        work.append((nf + ns) * steps)
+    log(f"serve: {n_requests} requests, fluid {min(len(r.arrays['fluid_position']) for r in reqs)}.."
+        f"{max(len(r.arrays['fluid_position']) for r in reqs)}, {sum(work) / 1e9:.2f} G particle-steps "
+        f"(built in {time.perf_counter() - t:.1f}s)")
+
+    # each request alone: upload + per-step time over a few steps
+    alone = []
+    for r in reqs:
+        t0 = time.perf_counter()
+        d = DeviceSim(r.scene.params(tuple(len(r.arrays[k]) for k in
+                                          ("fluid_position", "solid_position", "virt_position"))))
+        d.upload(r.arrays)
+        d.synchronize()
+        up = time.perf_counter() - t0
+        d.step(2)
+        d.synchronize()
+        t1 = time.perf_counter()
+        d.step(8)
+        d.synchronize()
+        per_step = (time.perf_counter() - t1) / 8
+        alone.append(up + per_step * r.steps)
+        d.close()
+        log(f"serve: request {r.rid} alone: upload {up:.3f}s + {r.steps} x {1e3 * per_step:.3f} ms")
+    # batched throughput on the whole set, to set the arrival rate for the target load
+    sims = []
+    for r in reqs[: min(len(reqs), 12)]:
+        d = DeviceSim(r.scene.params(tuple(len(r.arrays[k]) for k in
+                                          ("fluid_position", "solid_position", "virt_position"))))
+        d.upload(r.arrays)
+        sims.append(d)
+    b = DeviceBatch()
+    b.step(sims, 2)
+    b.synchronize()
+    t1 = time.perf_counter()
+    b.step(sims, 8)
+    b.synchronize()
+    rate = sum((d.nf + d.ns) for d in sims) * 8 / (time.perf_counter() - t1)
+    b.close()
+    for d in sims:
+        d.close()
+    mean_service = float(np.mean(work)) / rate
+    lam = load / mean_service
+    gaps = rng.exponential(1.0 / lam, size=n_requests)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    for r, a in zip(reqs, arrivals):
+        r.arrival = float(a)
+    log(f"serve: batched rate {rate:.3g} particle-steps/s, mean service {mean_service:.2f}s, "
+        f"lambda {lam:.3f}/s for load {load}")
+
+    srv = Server(chunk_steps=chunk, on_event=lambda kind, rid, t: log(f"serve: t={t:7.2f}s {kind} {rid}"))
+    t0 = time.perf_counter()
+    res = srv.run(reqs)
+    wall = time.perf_counter() - t0
+    lat = [r.latency for r in res if r.status == "done"]
+    makespan = max(r.finished for r in res) - min(r.arrival for r in res)
+    # one at a time in arrival order, same arrivals
+    free_at, seq_lat, seq_end = 0.0, [], 0.0
+    for r, s_alone in sorted(zip(reqs, alone), key=lambda x: x[0].arrival):
+        start = max(free_at, r.arrival)
+        free_at = start + s_alone
+        seq_lat.append(free_at - r.arrival)
+        seq_end = free_at
+    seq_makespan = seq_end - min(r.arrival for r in reqs)
+    log(f"serve: done in {wall:.1f}s; p50 {percentile(lat, 50):.2f}s p99 {percentile(lat, 99):.2f}s; "
+        f"sequential p50 {percentile(seq_lat, 50):.2f}s p99 {percentile(seq_lat, 99):.2f}s")
+    return {
+        "workload": f"{n_requests} scene-solve requests (seed {seed}): fluid log-uniform in [10k, 2M], solid "
+                    f"1-3 %, gate E U[5, 50] MPa, c0 = 10 sqrt(2 g H), {sim_time} s simulated each, "
+                    f"Poisson arrivals at {load:.0%} of the batched device rate",
+        "requests": n_requests,
+        "done": sum(r.status == "done" for r in res),
+        "particle_steps": int(sum(work)),
+        "arrival_rate_per_s": lam,
+        "p50_s": percentile(lat, 50), "p99_s": percentile(lat, 99),
+        "scenes_per_hour": 3600.0 * len(lat) / makespan,
+        "makespan_s": makespan,
+        "sequential_p50_s": percentile(seq_lat, 50), "sequential_p99_s": percentile(seq_lat, 99),
+        "sequential_scenes_per_hour": 3600.0 * n_requests / seq_makespan,
+        "sequential_capacity_scenes_per_hour": 3600.0 * n_requests / sum(alone),
+        "max_resident_scenes": srv.stats.max_active,
+        "max_resident_gib": srv.stats.max_resident_bytes / 2**30,
+        "chunks": srv.stats.chunks, "chunk_steps": chunk, "plans_built": srv.stats.plans_built,
+        "same_alone_and_in_mix": "tests/test_gpu_serving.py (bit-identical)",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,7 +310,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--workload", choices=["step", "batch"], default="step",
+    ap.add_argument("--workload", choices=["step", "batch", "serve"], default="step",
                     help="step: one scene's step throughput (the headline line, plus a short batched "
                          "measurement); batch: only the many-scene batched measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched measurement of the step workload")
@@ -214,6 +318,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=30)
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--sim-time", type=float, default=0.5, help="simulated seconds per solve (scenes/hour)")
+    ap.add_argument("--requests", type=int, default=16, help="serve workload: number of requests")
+    ap.add_argument("--load", type=float, default=0.8, help="serve workload: offered load")
+    ap.add_argument("--chunk", type=int, default=64, help="serve workload: steps between admission points")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (3 times): the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -238,6 +345,27 @@ def main():
     if args.workload == "step":
         args.steps = 1000 if args.steps is None else args.steps
         args.warmup = 20 if args.warmup is None else args.warmup
+
+    if args.workload == "serve":
+        r = serve_bench(args.requests, args.batch_seed, args.sim_time, args.load, args.chunk)
+        out = {
+            "metric": "p99 latency",
+            "value": r["p99_s"],
+            "unit": "s",
+            "n_gpus": 1,
+            "steps": None,
+            "warmup": None,
+            "ms_per_step": None,
+            "higher_is_better": False,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic (seeded dam-break requests, Poisson arrivals)",
+            "config": {"workload": r["workload"]},
+            "serving": r,
+        }
+        print(json.dumps(out), flush=True)
+        return
 
     if args.workload == "batch":
         r = batch_bench(args.batch_scenes, args.steps or args.batch_steps,

@@ -148,6 +148,8 @@ DEVICE_API = {
     "fsa_get_stage_times": (C.c_int, [_vp, _d, _P(C.c_int64)]),
     "fsa_reset_stage_times": (C.c_int, [_vp]),
     "fsa_memory_used": (C.c_int, [_vp, _P(C.c_int64)]),
+    "fsa_footprint": (C.c_int, [_P(FsaParams), _P(C.c_int64), _P(C.c_int64)]),
+    "fsa_device_memory": (C.c_int, [_P(C.c_int64), _P(C.c_int64)]),
     "fsa_batch_create": (C.c_int, [_vp, C.c_uint, _P(_vp)]),
     "fsa_batch_destroy": (C.c_int, [_vp]),
     "fsa_batch_step": (C.c_int, [_vp, _P(_vp), C.c_int, C.c_int]),

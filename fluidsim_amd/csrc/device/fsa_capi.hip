@@ -111,7 +111,7 @@ int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_sta
     for (int b = 0; b < 2; ++b) {
         if (b == 0 || double_state) {
             if ((rc = s->alloc(&k.pos[b], (size_t)n + kPosPad))) return rc;
-            HIP_TRY(hipMemsetAsync(k.pos[b], 0, sizeof(float4) * ((size_t)n + kPosPad), s->stream));
+            if (!s->dry) HIP_TRY(hipMemsetAsync(k.pos[b], 0, sizeof(float4) * ((size_t)n + kPosPad), s->stream));
             if ((rc = s->alloc(&k.vel[b], n))) return rc;
         }
         if ((rc = s->alloc(&k.id[b], n))) return rc;
@@ -123,6 +123,25 @@ int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_sta
     if ((rc = s->alloc(&k.tmp_slot, n))) return rc;
     if ((rc = s->alloc(&k.tmp_id, n))) return rc;
     if ((rc = s->alloc(&k.tile_sums, (size_t)scan_tiles(ncell)))) return rc;
+    return FSA_OK;
+}
+
+// every device buffer of a context (dry: only count the bytes)
+int alloc_all(fsa_sim *s) {
+    int rc;
+    if ((rc = alloc_kind(s, s->fluid, s->P.nf, false, true))) return rc;
+    if ((rc = alloc_kind(s, s->solid, s->P.ns, true, true))) return rc;
+    if ((rc = alloc_kind(s, s->virt, s->P.nv, false, false))) return rc;
+    const int ns = s->P.ns;
+    if ((rc = s->alloc(&s->sx.np, ns)) || (rc = s->alloc(&s->sx.acc, ns)) || (rc = s->alloc(&s->sx.couple, ns)) ||
+        (rc = s->alloc(&s->sx.F, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.P, 9 * (size_t)ns)) ||
+        (rc = s->alloc(&s->sx.L, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.PL, 9 * (size_t)ns)) ||
+        (rc = s->alloc(&s->sx.A, 9 * (size_t)ns)))
+        return rc;
+    if ((rc = s->alloc(&s->vx.nrm, s->P.nv)) || (rc = s->alloc(&s->virt_x, s->P.nv))) return rc;
+    if ((rc = s->alloc(&s->jumps, 1))) return rc;
+    if ((rc = s->alloc(&s->obs_partial, (size_t)observe_partials())) || (rc = s->alloc(&s->obs_out, kObserveSums)))
+        return rc;
     return FSA_OK;
 }
 
@@ -330,25 +349,37 @@ int fsa_create(const fsa_params *params, void *stream, fsa_sim **out) {
         fsa_destroy(s);
         return code;
     };
-    if ((rc = alloc_kind(s, s->fluid, s->P.nf, false, true))) return fail(rc);
-    if ((rc = alloc_kind(s, s->solid, s->P.ns, true, true))) return fail(rc);
-    if ((rc = alloc_kind(s, s->virt, s->P.nv, false, false))) return fail(rc);
-    const int ns = s->P.ns;
-    if ((rc = s->alloc(&s->sx.np, ns)) || (rc = s->alloc(&s->sx.acc, ns)) || (rc = s->alloc(&s->sx.couple, ns)) ||
-        (rc = s->alloc(&s->sx.F, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.P, 9 * (size_t)ns)) ||
-        (rc = s->alloc(&s->sx.L, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.PL, 9 * (size_t)ns)) ||
-        (rc = s->alloc(&s->sx.A, 9 * (size_t)ns)))
-        return fail(rc);
-    if ((rc = s->alloc(&s->vx.nrm, s->P.nv)) || (rc = s->alloc(&s->virt_x, s->P.nv))) return fail(rc);
-    if ((rc = s->alloc(&s->jumps, 1))) return fail(rc);
-    if ((rc = s->alloc(&s->obs_partial, (size_t)observe_partials())) || (rc = s->alloc(&s->obs_out, kObserveSums)))
-        return fail(rc);
+    if ((rc = alloc_all(s))) return fail(rc);
     if (hipMemsetAsync(s->jumps, 0, sizeof(unsigned long long), s->stream) != hipSuccess)
         return fail(set_err(FSA_ERR_HIP, "hipMemsetAsync failed"));
     if ((rc = zero_starts(s, s->fluid)) || (rc = zero_starts(s, s->solid)) || (rc = zero_starts(s, s->virt)))
         return fail(rc);
     if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(set_err(FSA_ERR_HIP, "stream sync failed"));
     *out = s;
+    return FSA_OK;
+}
+
+int fsa_footprint(const fsa_params *params, int64_t *resident, int64_t *upload_peak) {
+    if (!params || !resident) return set_err(FSA_ERR_INVALID, "null argument");
+    int rc = check_params(*params);
+    if (rc) return rc;
+    fsa_sim s;
+    s.dry = true;
+    s.params = *params;
+    s.P = make_dev_params(*params);
+    if ((rc = alloc_all(&s))) return rc;
+    *resident = s.bytes;
+    // fsa_upload_virt sorts the walls through two temporary float4 arrays
+    if (upload_peak) *upload_peak = s.bytes + 2 * (int64_t)sizeof(float4) * s.P.nv;
+    return FSA_OK;
+}
+
+int fsa_device_memory(int64_t *free_bytes, int64_t *total_bytes) {
+    if (!free_bytes || !total_bytes) return set_err(FSA_ERR_INVALID, "null argument");
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t));
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
     return FSA_OK;
 }
 

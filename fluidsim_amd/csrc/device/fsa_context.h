@@ -68,10 +68,15 @@ struct fsa_sim {
 
     std::vector<void *> allocations;
     int64_t bytes = 0;
+    bool dry = false;  // only count the bytes (fsa_footprint)
 
     template <class T> int alloc(T **p, size_t count) {
         *p = nullptr;
         if (count == 0) return FSA_OK;
+        if (dry) {
+            bytes += (int64_t)(count * sizeof(T));
+            return FSA_OK;
+        }
         void *q = nullptr;
         hipError_t e = hipMalloc(&q, count * sizeof(T));
         if (e != hipSuccess)

@@ -232,3 +232,20 @@ class DeviceBatch:
         plans, steps = C.c_int64(), C.c_int64()
         check(self.lib.fsa_batch_stats(self.handle, C.byref(plans), C.byref(steps)))
         return {"plans_built": int(plans.value), "steps_run": int(steps.value)}
+
+
+def footprint(params: FsaParams) -> tuple:
+    """(resident, upload_peak) device bytes of a context for these parameters;
+    allocates nothing (fsa_footprint)."""
+    lib = _native.device_lib()
+    r, p = C.c_int64(), C.c_int64()
+    check(lib.fsa_footprint(C.byref(params), C.byref(r), C.byref(p)))
+    return int(r.value), int(p.value)
+
+
+def device_memory() -> tuple:
+    """(free, total) bytes of the current device."""
+    lib = _native.device_lib()
+    f, t = C.c_int64(), C.c_int64()
+    check(lib.fsa_device_memory(C.byref(f), C.byref(t)))
+    return int(f.value), int(t.value)

@@ -251,6 +251,18 @@ int fsa_reset_stage_times(fsa_sim *sim);
 /* Device bytes held by the context. */
 int fsa_memory_used(fsa_sim *sim, int64_t *bytes);
 
+/* Device bytes fsa_create would allocate for these parameters (= what
+ * fsa_memory_used reports after creation, before optional debug, census and
+ * probe buffers), and the peak while fsa_upload_virt runs (it sorts the walls
+ * through temporary arrays).  Allocates nothing and needs no device: a
+ * scheduler admits scenes by this figure.  The reference sizes its device
+ * arrays from the .dat counts the same way (Sim::copyData, src/sim.cu:304-308;
+ * the list initiate functions, include/particle.cuh:73, 155, 218). */
+int fsa_footprint(const fsa_params *params, int64_t *resident, int64_t *upload_peak);
+
+/* Free and total bytes of the current device (hipMemGetInfo). */
+int fsa_device_memory(int64_t *free_bytes, int64_t *total_bytes);
+
 /* ---- batches: many scenes per launch ------------------------------------
  *
  * The reference runs one scene per process: its parameter and pointer

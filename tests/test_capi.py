@@ -74,3 +74,25 @@ def test_host_driver_reports_missing_device(tmp_path):
     rc = host.fsh_run(str(root).encode(), 0)
     assert rc != 0
     assert b"fsa_create" in host.fsh_last_error()
+
+
+def test_footprint_matches_a_hand_count_of_the_buffers():
+    """fsa_footprint needs no device: per fluid particle the context holds
+    position + velocity double buffers (4 x 16 B), id and cell double buffers
+    (4 x 4 B) and four sort scratch ints (16 B) = 96 B; per solid particle
+    additionally the reference position double buffer (32 B), np/acc/couple
+    (48 B) and five 3x3 planes (180 B) = 356 B; per wall particle one
+    position/velocity buffer (32 B), id and cell double buffers (16 B), sort
+    scratch (16 B), normal and constant position (32 B) = 96 B."""
+    from fluidsim_amd import scenes
+    from fluidsim_amd.sim import footprint
+    sc = scenes.tiny_scene(seed=1)
+    base = sc.params((1000, 100, 2000))
+    r0, p0 = footprint(base)
+    r1, _ = footprint(sc.params((1500, 100, 2000)))
+    r2, _ = footprint(sc.params((1000, 150, 2000)))
+    r3, p3 = footprint(sc.params((1000, 100, 2400)))
+    assert r1 - r0 == 500 * 96
+    assert r2 - r0 == 50 * 356
+    assert r3 - r0 == 400 * 96
+    assert p0 - r0 == 2000 * 32 and p3 - r3 == 2400 * 32
