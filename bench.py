@@ -182,7 +182,8 @@ def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: flo
     per_hour = lambda step_s: n_scenes * 3600.0 / (solve_steps * step_s)  # noqa: E731
     return {
         "workload": f"{n_scenes} dam breaks (seed {seed}): fluid log-uniform in [20k, 500k] on columns of "
-                    "aspect ratio U[0.5, 2], gate E U[5, 50] MPa, S from the solid CFL, dp 4 mm, c0 30 m/s",
+                    "aspect ratio U[0.5, 2], gate 1-3 % of the fluid count, gate E U[5, 50] MPa, S from the solid CFL, "
+                    "dp 4 mm, c0 30 m/s",
         "scenes": n_scenes, "fluid": nf, "solid": ns, "walls": nv,
         "solid_sub_steps": [min(subs), max(subs)],
         "solve": f"{sim_time} s simulated = {solve_steps} steps of dt {dt:.4g} s",
@@ -269,6 +270,12 @@ def serve_bench(n_requests: int, seed: int, sim_time: float, load: float, chunk:
     res = srv.run(reqs)
     wall = time.perf_counter() - t0
     lat = [r.latency for r in res if r.status == "done"]
+    busy = sum(e - s0 for s0, e, *_ in srv.stats.chunk_log)
+    done_work = sum(n * k for *_, k, n in srv.stats.chunk_log)
+    log(f"serve: chunks busy {busy:.1f}s of {wall:.1f}s, {done_work / max(busy, 1e-9):.3g} particle-steps/s in chunks")
+    for s0, e, na, k, n in srv.stats.chunk_log[:: max(1, len(srv.stats.chunk_log) // 25)]:
+        log(f"serve: chunk t={s0:7.2f} active {na:2d} particles {n:8d} steps {k:3d}: "
+            f"{1e3 * (e - s0) / k:.3f} ms/step")
     makespan = max(r.finished for r in res) - min(r.arrival for r in res)
     # one at a time in arrival order, same arrivals
     free_at, seq_lat, seq_end = 0.0, [], 0.0

@@ -302,25 +302,24 @@ def column_scene(n_fluid: float, flow_ratio: float, width_ratio: float, youngs_m
     ratios flow/height = flow_ratio and width/height = width_ratio, a gate of a
     given stiffness at 1.5 column lengths, and the sub-step count its solid CFL
     needs.  sound_speed None: 10x the dam-break velocity bound sqrt(2 g H).
-    solid_fraction: gate particles as a fraction of the fluid count (4 layers
-    thick, at most the column height); None: half the column height and 3/4
-    of the width."""
+    solid_fraction: gate particles as a fraction of the fluid count, in a gate
+    of 3/4 the column width whose height is 4x its thickness (a slender plate
+    under a large column deflects until it tears apart); None: 4 layers thick,
+    half the column height."""
     ny = max(4, round((n_fluid / (flow_ratio * width_ratio)) ** (1.0 / 3.0)))
     nx = max(4, round(flow_ratio * ny))
     nz = max(4, round(width_ratio * ny))
     tank = (2 * nx + 10, ny + max(6, ny // 2), nz)
     if solid_fraction is None:
+        gate_t = 4
         gate_h = max(8, ny // 2)
         gate_w = max(2, nz - 2 * max(1, nz // 8))
     else:
-        face = solid_fraction * nx * ny * nz / 4.0  # gate sites in the (height, width) face
         gate_w = max(2, min(nz, round(0.75 * nz)))
-        gate_h = max(5, round(face / gate_w))
-        if gate_h > ny:
-            gate_h = ny
-            gate_w = max(2, min(nz, round(face / gate_h)))
+        gate_t = max(2, round(math.sqrt(solid_fraction * nx * ny * nz / (4.0 * gate_w))))
+        gate_h = 4 * gate_t
     c0 = sound_speed if sound_speed is not None else 10.0 * math.sqrt(2.0 * 9.81 * ny * dp)
-    sc = DamBreak(dp=dp, fluid=(nx, ny, nz), tank=tank, gate=(4, gate_h, gate_w), gate_x0=(3 * nx) // 2,
+    sc = DamBreak(dp=dp, fluid=(nx, ny, nz), tank=tank, gate=(gate_t, gate_h, gate_w), gate_x0=(3 * nx) // 2,
                   gate_clamp_rows=3, wall_layers=3, seed=seed, sound_speed=float(c0),
                   youngs_modulus=float(youngs_modulus))
     return sc.with_(solid_sub_step=solid_sub_steps_for(sc))
@@ -329,9 +328,9 @@ def column_scene(n_fluid: float, flow_ratio: float, width_ratio: float, youngs_m
 def batch_scenes(n: int = 64, seed: int = 0, fluid_range=(20_000, 500_000), e_range=(5.0e6, 50.0e6),
                  dp: float = 0.004, sound_speed: float = 30.0) -> list:
     """`n` independent dam breaks: fluid counts log-uniform in `fluid_range`,
-    column aspect ratios uniform in [0.5, 2], gate E uniform in `e_range`
-    (nu 0.4, rho 1100), one sound speed (so one dt) for all, S per scene from
-    the solid CFL."""
+    column aspect ratios uniform in [0.5, 2], gate particles 1-3 % of the
+    fluid count, gate E uniform in `e_range` (nu 0.4, rho 1100), one sound
+    speed (so one dt) for all, S per scene from the solid CFL."""
     rng = np.random.default_rng(seed)
     lo, hi = math.log(fluid_range[0]), math.log(fluid_range[1])
     out = []
@@ -339,7 +338,9 @@ def batch_scenes(n: int = 64, seed: int = 0, fluid_range=(20_000, 500_000), e_ra
         nf = math.exp(rng.uniform(lo, hi))
         ax, az = rng.uniform(0.5, 2.0, size=2)
         E = rng.uniform(*e_range)
-        out.append(column_scene(nf, ax, az, E, seed=seed * 1000 + i, dp=dp, sound_speed=sound_speed))
+        frac = rng.uniform(0.01, 0.03)
+        out.append(column_scene(nf, ax, az, E, seed=seed * 1000 + i, dp=dp, sound_speed=sound_speed,
+                                solid_fraction=frac))
     return out
 
 

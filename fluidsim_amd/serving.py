@@ -72,6 +72,7 @@ class ServerStats:
     max_active: int = 0
     plans_built: int = 0
     events: list = field(default_factory=list)
+    chunk_log: list = field(default_factory=list)  # (start, end, active scenes, steps, particles)
 
 
 class Server:
@@ -160,6 +161,7 @@ class Server:
                     continue
                 k = min(self.chunk_steps, min(x.remaining for x in active))
                 sims = [x.sim for x in active]
+                t_chunk = clock()
                 batch.step(sims, k)
                 # arrivals during the chunk are prepared while the device works
                 now = clock()
@@ -168,6 +170,7 @@ class Server:
                     a, counts, params = self._inputs(req)
                     queue.append((req, a, counts, params, footprint(params)))
                 batch.synchronize()
+                st.chunk_log.append((t_chunk, clock(), len(active), k, sum(x.counts[0] + x.counts[1] for x in active)))
                 st.chunks += 1
                 st.steps += k
                 st.scene_steps += k * len(active)
