@@ -9,11 +9,19 @@
 // contiguous slot range per kind.
 //
 // Only ~1 candidate in 6 lies inside the cut-off, and in SIMT a wave pays
-// for the pair math whenever any lane takes that branch.  So every run of
-// candidates is done in two phases: the candidate test appends the in-range
+// for the pair math whenever any lane takes that branch.  So each kind's
+// partners are done in two phases: the candidate test appends the in-range
 // slots to a lane-private list in LDS (entry k of lane t at [k][t], so lanes
 // hit distinct banks), then a dense loop evaluates the pair terms of the
-// list.  Pairs are summed in exactly the candidate order.
+// list.  Pairs are summed in exactly the candidate order.  The list holds all
+// of a particle's partners of one kind (~80 at h = 1.3 dp), so the pair loop
+// runs once per kind with every lane busy: a lane's partner count varies
+// little across a wave, while its count at any point of the candidate walk
+// varies a lot (the walk visits the columns in a fixed order and the lanes'
+// particles sit at different places in their cells).  Entries are 16 bits:
+// a run-base slot (4 bits, the run's first slot is kept in LDS beside the
+// list) and the offset inside the run (12 bits); a run longer than 4096 slots
+// is split into several.
 //
 // Pair arithmetic is FP32 with the hardware reciprocal, reciprocal square
 // root and exp2, and the algebraically reduced forms noted inline; the
@@ -32,7 +40,6 @@ namespace {
 
 enum PairKind { kFluidFluid = 0, kFluidSolid = 1, kFluidVirt = 2 };
 
-constexpr int kListCap = 32;  // list entries per lane (32 KB of LDS per 256-thread block)
 
 struct Self {
     float x, y, z, rho, ux, uy, uz, p;
@@ -54,10 +61,12 @@ __device__ __forceinline__ float exp2_hw(float x) { return __builtin_amdgcn_exp2
 //   a  += -2 p*/rho_i dx k + [viscid] 10 mu ((u_j-u_i).dx)/(|dx|^2+0.01h^2)/rho_i dx k
 //   rho' += 2 rho_i (u_i - v*).dx k
 // with (u_i - v*).dx reduced to rho_j/(rho_i+rho_j) (u_i-u_j).dx - (p_i-p_j)/(rho_i+rho_j)/c0 |dx|.
+// w is 1 for a partner and 0 for a padding slot of the pair loop (whose
+// terms are then exact zeros).
 template <int KIND>
 __device__ __forceinline__ void pair_term(const DevParams &P, const Self &me, float dx, float dy, float dz, float d2,
-                                          float ujx, float ujy, float ujz, float rhoj, float pj, Acc &acc) {
-    const float kc = exp2_hw(-d2 * P.kexp2) * P.kd;
+                                          float ujx, float ujy, float ujz, float rhoj, float pj, float w, Acc &acc) {
+    const float kc = exp2_hw(-d2 * P.kexp2) * P.kd * w;
     const float r = d2 != 0.0f ? rsq(d2) : 0.0f;  // 1/|dx|, e = dx r
     const float inv_rsum = KIND == kFluidSolid ? 0.5f * me.inv_rho : rcp(me.rho + rhoj);
     const float udx = (me.ux - ujx) * dx + (me.uy - ujy) * dy + (me.uz - ujz) * dz;  // (u_i-u_j).dx
@@ -79,97 +88,142 @@ __device__ __forceinline__ void pair_term(const DevParams &P, const Self &me, fl
 
 template <int KIND>
 __device__ __forceinline__ void one_pair(const DevParams &P, const Self &me, const float4 &pj, const float4 &vj,
-                                         const float4 &n, Acc &acc) {
+                                         const float4 &n, float w, Acc &acc) {
     const float dx = pj.x - me.x, dy = pj.y - me.y, dz = pj.z - me.z;
     const float d2 = len2(dx, dy, dz);
     if (KIND == kFluidSolid) {
         // mirrored solid velocity (fluid.cu:79-83); rho_j = rho_i, p_j = p_s
         const float m = 2.0f * ((me.ux - vj.x) * n.x + (me.uy - vj.y) * n.y + (me.uz - vj.z) * n.z);
-        pair_term<KIND>(P, me, dx, dy, dz, d2, me.ux - m * n.x, me.uy - m * n.y, me.uz - m * n.z, me.rho, n.w, acc);
+        pair_term<KIND>(P, me, dx, dy, dz, d2, me.ux - m * n.x, me.uy - m * n.y, me.uz - m * n.z, me.rho, n.w, w,
+                        acc);
     } else {
-        pair_term<KIND>(P, me, dx, dy, dz, d2, vj.x, vj.y, vj.z, pj.w, vj.w, acc);
+        pair_term<KIND>(P, me, dx, dy, dz, d2, vj.x, vj.y, vj.z, pj.w, vj.w, w, acc);
     }
 }
 
-constexpr int kPairBatch = 4;  // partner gathers in flight in the pair loop
-constexpr int kScanBatch = 8;  // candidate loads in flight in the test loop
+constexpr int kThreads = 128;   // threads per workgroup (LDS per wave bounds the occupancy)
+constexpr int kListCap = 96;    // 16-bit list entries per lane
+constexpr int kRunBits = 12;    // offset of an entry inside its run
+constexpr int kRunMax = 1 << kRunBits;
+constexpr int kBaseSlots = 12;  // run bases per lane between two evaluations (4-bit ids)
+constexpr int kPairBatch = 8;   // partner gathers in flight in the pair loop
+constexpr int kScanBatch = 8;   // candidate loads in flight in the test loop
 static_assert(kPosPad >= kScanBatch, "candidate batches may read kScanBatch-1 entries past a range");
+static_assert(kBaseSlots <= 16 && kBaseSlots * kRunMax <= 65536, "entries are 16 bits");
+
+// lane-private LDS views: entry k at list[k * kThreads], base b at base[b * kThreads]
+struct Lists {
+    uint16_t *list;
+    int *base;
+};
 
 // pair terms of list entries [0, cnt), in list order; the partner state of
-// kPairBatch entries is gathered before any of them is used
+// kPairBatch entries is gathered before any of them is used, and all of them
+// are evaluated (entries past cnt re-read the last one and add nothing)
 template <int KIND>
-__device__ __forceinline__ void flush(const DevParams &P, const Self &me, const float4 *__restrict__ pos,
-                                      const float4 *__restrict__ vel, const float4 *__restrict__ np,
-                                      const int *list, int cnt, Acc &acc) {
+__device__ __forceinline__ void evaluate(const DevParams &P, const Self &me, const float4 *__restrict__ pos,
+                                         const float4 *__restrict__ vel, const float4 *__restrict__ np,
+                                         const Lists &L, int cnt, Acc &acc) {
     for (int k = 0; k < cnt; k += kPairBatch) {
         float4 pj[kPairBatch], vj[kPairBatch], nj[kPairBatch];
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q) {
-            // entries past cnt re-read the last one (always a valid slot)
-            const int j = list[min(k + q, cnt - 1) * kBlock];
+            const int v = L.list[min(k + q, cnt - 1) * kThreads];
+            const int j = L.base[(v >> kRunBits) * kThreads] + (v & (kRunMax - 1));
             pj[q] = pos[j];
             vj[q] = vel[j];
             if (KIND == kFluidSolid) nj[q] = np[j];
         }
 #pragma unroll
-        for (int q = 0; q < kPairBatch; ++q)
-            if (k + q < cnt) one_pair<KIND>(P, me, pj[q], vj[q], nj[q], acc);
+        for (int q = 0; q < kPairBatch; ++q) one_pair<KIND>(P, me, pj[q], vj[q], nj[q], k + q < cnt ? 1.0f : 0.0f, acc);
     }
 }
 
 // candidate test over slots [b, e) of one kind, appending hits to the list.
 // Control flow is uniform across the wave: the batch loop runs while any
 // lane still has candidates (a lane past its run re-tests the padded tail
-// at e, masked out), and the list is evaluated by all lanes together as soon
-// as any lane is nearly full, so the pair loop runs over dense batches.
-// kScanBatch candidate loads are issued before the first test.
+// at the run end, masked out), and a full list or full base table is
+// evaluated by all lanes together.  kScanBatch candidate loads are issued
+// before the first test.
 template <int KIND>
 __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int b, int e, const float4 *__restrict__ pos,
-                                     const float4 *__restrict__ vel, const float4 *__restrict__ np, int *list,
-                                     int &cnt, Acc &acc) {
-    for (int j = b; __ballot(j < e) != 0; j += kScanBatch) {
-        // branch-free batch: loads at or past the run end stay inside the
-        // padded array (e <= n, kPosPad >= kScanBatch) and are masked out of
-        // the test; every candidate writes list slot cnt, which only a hit
-        // keeps (cnt <= kListCap-1 here: see the flush below)
-        const int jj = min(j, e);
-        float4 q[kScanBatch];
-        const float4 *src = pos + jj;
-#pragma unroll
-        for (int k = 0; k < kScanBatch; ++k) q[k] = src[k];
-#pragma unroll
-        for (int k = 0; k < kScanBatch; ++k) {
-            const float d2 = len2(q[k].x - me.x, q[k].y - me.y, q[k].z - me.z);
-            const bool hit = (jj + k < e) & (d2 < P.R2);
-            list[cnt * kBlock] = jj + k;
-            cnt += hit ? 1 : 0;
-        }
-        // at most kScanBatch entries are added per round
-        if (__ballot(cnt > kListCap - kScanBatch) != 0) {
-            flush<KIND>(P, me, pos, vel, np, list, cnt, acc);
+                                     const float4 *__restrict__ vel, const float4 *__restrict__ np, const Lists &L,
+                                     int &cnt, int &nbase, Acc &acc) {
+    for (int sb = b; __ballot(sb < e) != 0; sb += kRunMax) {
+        const bool has = sb < e;
+        if (__ballot(has && nbase == kBaseSlots) != 0) {
+            evaluate<KIND>(P, me, pos, vel, np, L, cnt, acc);
             cnt = 0;
+            nbase = 0;
+        }
+        const int se = min(sb + kRunMax, e);  // <= sb for a lane without work
+        int tag = 0;                          // entry = tag + slot
+        if (has) {
+            L.base[nbase * kThreads] = sb;
+            tag = (nbase << kRunBits) - sb;
+            nbase += 1;
+        }
+        // branch-free batches: loads at or past the run end stay inside the
+        // padded array (se <= n, kPosPad >= kScanBatch) and are masked out of
+        // the test; every candidate writes list slot cnt, which only a hit
+        // keeps (cnt <= kListCap-1 here: see the evaluation below).  The next
+        // batch is loaded while the current one is tested.
+        float4 q[kScanBatch];
+        {
+            const float4 *src = pos + min(sb, se);
+#pragma unroll
+            for (int k = 0; k < kScanBatch; ++k) q[k] = src[k];
+        }
+        for (int j = sb; __ballot(j < se) != 0; j += kScanBatch) {
+            const int jj = min(j, se);
+            float4 qn[kScanBatch];
+            const float4 *nxt = pos + min(j + kScanBatch, se);
+#pragma unroll
+            for (int k = 0; k < kScanBatch; ++k) qn[k] = nxt[k];
+#pragma unroll
+            for (int k = 0; k < kScanBatch; ++k) {
+                const float d2 = len2(q[k].x - me.x, q[k].y - me.y, q[k].z - me.z);
+                const bool hit = (jj + k < se) & (d2 < P.R2);
+                L.list[cnt * kThreads] = (uint16_t)(tag + jj + k);
+                cnt += hit ? 1 : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kScanBatch; ++k) q[k] = qn[k];
+            // at most kScanBatch entries are added per round
+            if (__ballot(cnt > kListCap - kScanBatch) != 0) {
+                evaluate<KIND>(P, me, pos, vel, np, L, cnt, acc);
+                cnt = 0;
+                // the run in progress keeps going from base slot 0
+                nbase = 0;
+                if (has) {
+                    L.base[0] = sb;
+                    tag = -sb;
+                    nbase = 1;
+                }
+            }
         }
     }
 }
 
-// all partners of one kind: the 9 columns in order, then the list remainder
+// all partners of one kind: the 9 columns in order, then one evaluation
 template <int KIND>
 __device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cell, const int32_t *__restrict__ start,
                                      const float4 *__restrict__ pos, const float4 *__restrict__ vel,
-                                     const float4 *__restrict__ np, int *list, Acc &acc) {
-    int cnt = 0;
+                                     const float4 *__restrict__ np, const Lists &L, Acc &acc) {
+    int cnt = 0, nbase = 0;
     for_each_column(P, cell, [&](int c0, int c1) {
-        scan<KIND>(P, me, start[c0], start[c1], pos, vel, np, list, cnt, acc);
+        scan<KIND>(P, me, start[c0], start[c1], pos, vel, np, L, cnt, nbase, acc);
     });
-    flush<KIND>(P, me, pos, vel, np, list, cnt, acc);
+    evaluate<KIND>(P, me, pos, vel, np, L, cnt, acc);
 }
 
 template <bool RATES>
 __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepArgs &a, int blk, int nblk) {
-    __shared__ int s_list[kListCap * kBlock];
-    const int i = xcd_tile(blk, nblk) * kBlock + threadIdx.x;
+    __shared__ uint16_t s_list[kListCap * kThreads];
+    __shared__ int s_base[kBaseSlots * kThreads];
+    const int i = xcd_tile(blk, nblk) * kThreads + threadIdx.x;
     if (i >= P.nf) return;  // lists are lane-private: no barrier follows
-    int *list = s_list + threadIdx.x;
+    const Lists list{s_list + threadIdx.x, s_base + threadIdx.x};
 
     const float4 pi4 = a.fpos[i];
     const float4 vi4 = a.fvel[i];
@@ -226,22 +280,23 @@ struct OpFluid {
 
 }  // namespace
 
-int blocks_fluid(const FluidJob &j) { return blocks_for(j.P.nf); }
+int blocks_fluid(const FluidJob &j) { return (int)(((long long)j.P.nf + kThreads - 1) / kThreads); }
 
 hipError_t launch_fluid(const DevParams &P, const FluidStepArgs &a, hipStream_t s) {
     if (P.nf == 0) return hipSuccess;
     const FluidJob j{P, a};
     const dim3 grid(blocks_fluid(j));
     if (a.rates)
-        hipLaunchKernelGGL((k_one<OpFluid<true>, FluidJob>), grid, dim3(kBlock), 0, s, j);
+        hipLaunchKernelGGL((k_one<OpFluid<true>, FluidJob, kThreads>), grid, dim3(kThreads), 0, s, j);
     else
-        hipLaunchKernelGGL((k_one<OpFluid<false>, FluidJob>), grid, dim3(kBlock), 0, s, j);
+        hipLaunchKernelGGL((k_one<OpFluid<false>, FluidJob, kThreads>), grid, dim3(kThreads), 0, s, j);
     return hipGetLastError();
 }
 
 hipError_t launch_many_fluid(const FluidJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s) {
     if (nblocks <= 0 || njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_many<OpFluid<false>, FluidJob>), dim3(nblocks), dim3(kBlock), 0, s, jobs, prefix, njobs);
+    hipLaunchKernelGGL((k_many<OpFluid<false>, FluidJob, kThreads>), dim3(nblocks), dim3(kThreads), 0, s, jobs,
+                       prefix, njobs);
     return hipGetLastError();
 }
 

@@ -102,14 +102,15 @@ __host__ __device__ inline int padded_cells(int ncell) { return scan_tiles(ncell
 
 // ---- generic wrappers -------------------------------------------------------
 
-template <class Op, class Job>
-__global__ __launch_bounds__(kBlock) void k_one(Job job) {
+template <class Op, class Job, int THREADS = kBlock>
+__global__ __launch_bounds__(THREADS) void k_one(Job job) {
     Op::run(job, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // prefix[s] is the first block of job s, prefix[n] the grid size
-template <class Op, class Job>
-__global__ __launch_bounds__(kBlock) void k_many(const Job *__restrict__ jobs, const int *__restrict__ prefix, int n) {
+template <class Op, class Job, int THREADS = kBlock>
+__global__ __launch_bounds__(THREADS) void k_many(const Job *__restrict__ jobs, const int *__restrict__ prefix,
+                                                  int n) {
     const int b = (int)blockIdx.x;
     int lo = 0, hi = n;
     while (hi - lo > 1) {
