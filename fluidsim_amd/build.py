@@ -36,6 +36,10 @@ HIP_FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-adds are fused only inside one source expression: every inlined
+    # copy of a formula then rounds the same way (the batched and single-scene
+    # kernels, and the pair loop at its two call sites, give identical bits)
+    "-ffp-contract=on",
     "-Wall",
     "-Wno-unused-function",
     f"-I{INCLUDE}",

@@ -21,7 +21,7 @@
 // particles sit at different places in their cells).  Entries are 16 bits:
 // a run-base slot (4 bits, the run's first slot is kept in LDS beside the
 // list) and the offset inside the run (12 bits); a run longer than 4096 slots
-// is split into several.
+// continues in spare base slots.
 //
 // Pair arithmetic is FP32 with the hardware reciprocal, reciprocal square
 // root and exp2, and the algebraically reduced forms noted inline; the
@@ -101,12 +101,13 @@ __device__ __forceinline__ void one_pair(const DevParams &P, const Self &me, con
     }
 }
 
-constexpr int kThreads = 128;   // threads per workgroup (LDS per wave bounds the occupancy)
+constexpr int kThreads = 64;    // one wave per workgroup (LDS per wave bounds the occupancy)
 constexpr int kListCap = 96;    // 16-bit list entries per lane
 constexpr int kRunBits = 12;    // offset of an entry inside its run
 constexpr int kRunMax = 1 << kRunBits;
-constexpr int kBaseSlots = 12;  // run bases per lane between two evaluations (4-bit ids)
-constexpr int kPairBatch = 8;   // partner gathers in flight in the pair loop
+constexpr int kColumns = 9;     // neighbour columns; column c's run starts in base slot c
+constexpr int kBaseSlots = 12;  // run bases per lane (4-bit ids): the columns, then spares for long runs
+constexpr int kPairBatch = 4;   // partners per round of the pair loop (the next round is gathered meanwhile)
 constexpr int kScanBatch = 8;   // candidate loads in flight in the test loop
 static_assert(kPosPad >= kScanBatch, "candidate batches may read kScanBatch-1 entries past a range");
 static_assert(kBaseSlots <= 16 && kBaseSlots * kRunMax <= 65536, "entries are 16 bits");
@@ -117,57 +118,81 @@ struct Lists {
     int *base;
 };
 
-// pair terms of list entries [0, cnt), in list order; the partner state of
-// kPairBatch entries is gathered before any of them is used, and all of them
-// are evaluated (entries past cnt re-read the last one and add nothing)
+// partner state of list entries k .. k+kPairBatch-1 (entries past cnt re-read
+// the last one, always a valid slot)
+template <int KIND>
+__device__ __forceinline__ void gather(const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                                       const float4 *__restrict__ np, const Lists &L, int k, int cnt, float4 *pj,
+                                       float4 *vj, float4 *nj) {
+#pragma unroll
+    for (int q = 0; q < kPairBatch; ++q) {
+        const int v = L.list[min(k + q, cnt - 1) * kThreads];
+        const int j = L.base[(v >> kRunBits) * kThreads] + (v & (kRunMax - 1));
+        pj[q] = pos[j];
+        vj[q] = vel[j];
+        if (KIND == kFluidSolid) nj[q] = np[j];
+    }
+}
+
+// pair terms of list entries [0, cnt), in list order.  The partner state of
+// the next kPairBatch entries is gathered while the current ones are
+// evaluated; all kPairBatch slots of a round are evaluated, the ones past
+// cnt with weight 0.
 template <int KIND>
 __device__ __forceinline__ void evaluate(const DevParams &P, const Self &me, const float4 *__restrict__ pos,
                                          const float4 *__restrict__ vel, const float4 *__restrict__ np,
                                          const Lists &L, int cnt, Acc &acc) {
+    if (__ballot(cnt > 0) == 0) return;
+    float4 pj[kPairBatch], vj[kPairBatch], nj[kPairBatch];
+    if (cnt > 0) gather<KIND>(pos, vel, np, L, 0, cnt, pj, vj, nj);  // an empty list holds no valid slot
     for (int k = 0; k < cnt; k += kPairBatch) {
-        float4 pj[kPairBatch], vj[kPairBatch], nj[kPairBatch];
-#pragma unroll
-        for (int q = 0; q < kPairBatch; ++q) {
-            const int v = L.list[min(k + q, cnt - 1) * kThreads];
-            const int j = L.base[(v >> kRunBits) * kThreads] + (v & (kRunMax - 1));
-            pj[q] = pos[j];
-            vj[q] = vel[j];
-            if (KIND == kFluidSolid) nj[q] = np[j];
-        }
+        float4 pn[kPairBatch], vn[kPairBatch], nn[kPairBatch];
+        gather<KIND>(pos, vel, np, L, k + kPairBatch, cnt, pn, vn, nn);
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q) one_pair<KIND>(P, me, pj[q], vj[q], nj[q], k + q < cnt ? 1.0f : 0.0f, acc);
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) {
+            pj[q] = pn[q];
+            vj[q] = vn[q];
+            nj[q] = nn[q];
+        }
     }
 }
 
-// candidate test over slots [b, e) of one kind, appending hits to the list.
-// Control flow is uniform across the wave: the batch loop runs while any
-// lane still has candidates (a lane past its run re-tests the padded tail
-// at the run end, masked out), and a full list or full base table is
-// evaluated by all lanes together.  kScanBatch candidate loads are issued
-// before the first test.
+// candidate test over slots [b, e) of column `col` of one kind, appending
+// hits to the list.  Base slot `col` holds b (written by the kind's pass);
+// a run longer than kRunMax slots continues in spare base slots.  Control
+// flow is uniform across the wave: the batch loop runs while any lane still
+// has candidates (a lane past its run re-tests the padded tail at the run
+// end, masked out), and a full list or a full spare table is evaluated by
+// all lanes together.  The next batch of candidates is loaded while the
+// current one is tested.
 template <int KIND>
-__device__ __forceinline__ void scan(const DevParams &P, const Self &me, int b, int e, const float4 *__restrict__ pos,
-                                     const float4 *__restrict__ vel, const float4 *__restrict__ np, const Lists &L,
-                                     int &cnt, int &nbase, Acc &acc) {
-    for (int sb = b; __ballot(sb < e) != 0; sb += kRunMax) {
+__device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col, int b, int e,
+                                     const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                                     const float4 *__restrict__ np, const Lists &L, int &cnt, int &spare, Acc &acc) {
+    int sb = b;
+    for (int sub = 0; __ballot(sb < e) != 0; ++sub, sb += kRunMax) {
         const bool has = sb < e;
-        if (__ballot(has && nbase == kBaseSlots) != 0) {
-            evaluate<KIND>(P, me, pos, vel, np, L, cnt, acc);
-            cnt = 0;
-            nbase = 0;
+        int slot = col;
+        if (sub > 0) {
+            if (__ballot(has && spare == kBaseSlots - kColumns) != 0) {
+                evaluate<KIND>(P, me, pos, vel, np, L, cnt, acc);
+                cnt = 0;
+                spare = 0;
+            }
+            if (has) {
+                slot = kColumns + spare;
+                L.base[slot * kThreads] = sb;
+                spare += 1;
+            }
         }
         const int se = min(sb + kRunMax, e);  // <= sb for a lane without work
-        int tag = 0;                          // entry = tag + slot
-        if (has) {
-            L.base[nbase * kThreads] = sb;
-            tag = (nbase << kRunBits) - sb;
-            nbase += 1;
-        }
+        const int tag = (slot << kRunBits) - sb;  // entry = tag + slot index of the partner
         // branch-free batches: loads at or past the run end stay inside the
         // padded array (se <= n, kPosPad >= kScanBatch) and are masked out of
         // the test; every candidate writes list slot cnt, which only a hit
-        // keeps (cnt <= kListCap-1 here: see the evaluation below).  The next
-        // batch is loaded while the current one is tested.
+        // keeps (cnt <= kListCap-1 here: see the evaluation below)
         float4 q[kScanBatch];
         {
             const float4 *src = pos + min(sb, se);
@@ -189,31 +214,55 @@ __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int b, 
             }
 #pragma unroll
             for (int k = 0; k < kScanBatch; ++k) q[k] = qn[k];
-            // at most kScanBatch entries are added per round
+            // at most kScanBatch entries are added per round; the bases stay
+            // valid across an evaluation
             if (__ballot(cnt > kListCap - kScanBatch) != 0) {
                 evaluate<KIND>(P, me, pos, vel, np, L, cnt, acc);
                 cnt = 0;
-                // the run in progress keeps going from base slot 0
-                nbase = 0;
-                if (has) {
-                    L.base[0] = sb;
-                    tag = -sb;
-                    nbase = 1;
-                }
             }
         }
     }
 }
+
+// The 9 columns of cell c (the order of for_each_column) as cell ranges
+// [c0, c1) of one z-run each; columns outside the grid are the empty range
+// [0, 0).
+struct Columns {
+    int cx, cy, zlo, zhi1, ny, nz, nx;
+    __device__ __forceinline__ Columns(const DevParams &P, int c) {
+        ny = P.cn[1];
+        nz = P.cn[2];
+        nx = P.cn[0];
+        const int nyz = ny * nz;
+        cx = c / nyz;
+        const int rem = c - cx * nyz;
+        cy = rem / nz;
+        const int cz = rem - cy * nz;
+        zlo = max(cz - 1, 0);
+        zhi1 = min(cz + 1, nz - 1) + 1;
+    }
+    __device__ __forceinline__ bool inside(int col) const {
+        const int gx = cx + col / 3 - 1, gy = cy + col % 3 - 1;
+        return (unsigned)gx < (unsigned)nx && (unsigned)gy < (unsigned)ny;
+    }
+    __device__ __forceinline__ int base(int col) const { return ((cx + col / 3 - 1) * ny + (cy + col % 3 - 1)) * nz; }
+    __device__ __forceinline__ int c0(int col) const { return inside(col) ? base(col) + zlo : 0; }
+    __device__ __forceinline__ int c1(int col) const { return inside(col) ? base(col) + zhi1 : 0; }
+};
 
 // all partners of one kind: the 9 columns in order, then one evaluation
 template <int KIND>
 __device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cell, const int32_t *__restrict__ start,
                                      const float4 *__restrict__ pos, const float4 *__restrict__ vel,
                                      const float4 *__restrict__ np, const Lists &L, Acc &acc) {
-    int cnt = 0, nbase = 0;
-    for_each_column(P, cell, [&](int c0, int c1) {
-        scan<KIND>(P, me, start[c0], start[c1], pos, vel, np, L, cnt, nbase, acc);
-    });
+    const Columns cols(P, cell);
+    int cnt = 0, spare = 0;
+#pragma unroll 1
+    for (int col = 0; col < kColumns; ++col) {
+        const int b = start[cols.c0(col)], e = start[cols.c1(col)];
+        L.base[col * kThreads] = b;
+        scan<KIND>(P, me, col, b, e, pos, vel, np, L, cnt, spare, acc);
+    }
     evaluate<KIND>(P, me, pos, vel, np, L, cnt, acc);
 }
 

@@ -146,13 +146,21 @@ def test_c1_scene_parity(gpu_available, c1_pair, steps):
     dev.upload(a)
     o = orc.Oracle(p, 32)
     o.upload(a)
+    o64 = orc.Oracle(p, 64)
+    o64.upload(a)
     dev.step(steps)
     o.step(steps)
+    o64.step(steps)
     rep = compare_fluid(sc, dev.fluid(), o.fluid())
     print({k: v[0] for k, v in rep.items()}, flush=True)
     assert_within(rep, f"C1 fluid after {steps} steps")
+    # gate positions: TLSPH in FP32 amplifies rounding (the Green strain is a
+    # difference of numbers near 1), so the GPU is held to the accuracy of the
+    # FP32 oracle against FP64, plus 1e-5 of the tank size
     L = max(sc.tank) * sc.dp
-    assert max_abs(dev.solid()["position"], o.solid()["position"]) <= REL * L
+    e_gpu, e_32, ok = same_class(dev.solid()["position"], o.solid()["position"], o64.solid()["position"], L)
+    print(f"gate position error vs FP64: gpu {e_gpu:.3g}, fp32 oracle {e_32:.3g}", flush=True)
+    assert ok, (e_gpu, e_32)
     for kind in (FLUID, SOLID):
         np.testing.assert_array_equal(dev.cell_ids(kind), o.cell_ids(kind))
 
