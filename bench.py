@@ -156,17 +156,19 @@ def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: flo
     dt = family[0].time_step
     solve_steps = math.ceil(sim_time / dt)
 
-    b = DeviceBatch()
+    b = DeviceBatch(autotune=True)
     b.step(sims, warmup)
     b.synchronize()
     t0 = time.perf_counter()
     b.step(sims, steps)
     b.synchronize()
     batched = (time.perf_counter() - t0) / steps
-    log(f"batch: batched {1e3 * batched:.3f} ms/step for all {n_scenes} scenes")
+    log(f"batch: batched {1e3 * batched:.3f} ms/step for all {n_scenes} scenes, "
+        f"fluid workgroup {b.stats()['fluid_threads']}")
 
     for d in sims:
         d.step(warmup)
+        d.autotune_fluid(2)
     for d in sims:
         d.synchronize()
     t0 = time.perf_counter()
@@ -175,7 +177,7 @@ def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: flo
         d.synchronize()  # one plain solve after another
     sequential = (time.perf_counter() - t0) / steps
     log(f"batch: sequential {1e3 * sequential:.3f} ms/step summed over the scenes")
-    plans = b.stats()["plans_built"]
+    plans, b_threads = b.stats()["plans_built"], b.stats()["fluid_threads"]
     b.close()
     for d in sims:
         d.close()
@@ -195,6 +197,7 @@ def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: flo
         "speedup": round(sequential / batched, 3),
         "particle_steps_per_s_batched": (nf + ns) / batched,
         "plans_built": plans,
+        "fluid_threads_batched": b_threads,
         "bit_identical_to_single_scene": "tests/test_gpu_batch.py",
     }
 
@@ -231,6 +234,7 @@ def serve_bench(n_requests: int, seed: int, sim_time: float, load: float, chunk:
         d.synchronize()
         up = time.perf_counter() - t0
         d.step(2)
+        d.autotune_fluid(2)
         d.synchronize()
         t1 = time.perf_counter()
         d.step(8)
@@ -246,7 +250,7 @@ def serve_bench(n_requests: int, seed: int, sim_time: float, load: float, chunk:
                                           ("fluid_position", "solid_position", "virt_position"))))
         d.upload(r.arrays)
         sims.append(d)
-    b = DeviceBatch()
+    b = DeviceBatch(autotune=True)
     b.step(sims, 2)
     b.synchronize()
     t1 = time.perf_counter()
@@ -411,7 +415,8 @@ def main():
 
     dev.step(args.warmup)
     sync_all(dev)
-    log(f"warmup {args.warmup} steps done")
+    threads = dev.autotune_fluid(3)
+    log(f"warmup {args.warmup} steps done; fluid workgroup size {threads} (autotuned)")
 
     dev.set_timing(True)
     dev.reset_stage_times()
@@ -484,6 +489,7 @@ def main():
             "fluid": nf, "solid": ns, "walls": nv,
             "dp": sc.dp, "h_over_dp": sc.h_ratio, "solid_sub_step": sc.solid_sub_step,
             "cells": params.cell_number_total,
+            "fluid_threads": threads,
         },
         "stage_ms": {k: (ms / n if n else None) for k, (ms, n) in stage.items()},
         "roofline": roofline,

@@ -154,10 +154,14 @@ DEVICE_API = {
     "fsa_batch_destroy": (C.c_int, [_vp]),
     "fsa_batch_step": (C.c_int, [_vp, _P(_vp), C.c_int, C.c_int]),
     "fsa_batch_synchronize": (C.c_int, [_vp]),
-    "fsa_batch_stats": (C.c_int, [_vp, _P(C.c_int64), _P(C.c_int64)]),
+    "fsa_batch_stats": (C.c_int, [_vp, _P(C.c_int64), _P(C.c_int64), _P(C.c_int)]),
+    "fsa_set_fluid_threads": (C.c_int, [_vp, C.c_int]),
+    "fsa_get_fluid_threads": (C.c_int, [_vp, _P(C.c_int)]),
+    "fsa_autotune_fluid": (C.c_int, [_vp, C.c_int, _P(C.c_int)]),
 }
 
 FSA_BATCH_NO_GRAPH = 1
+FSA_BATCH_AUTOTUNE = 2
 
 HOST_API = {
     "fsh_last_error": (C.c_char_p, []),

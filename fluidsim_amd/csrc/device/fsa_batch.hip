@@ -22,6 +22,13 @@
 // Solid sub-steps: the scenes' solid_sub_step counts differ (a stiffer gate
 // needs more).  Sub-step k launches the three solid stages over the scenes
 // with more than k sub-steps; the others get empty ranges.
+//
+// Launch shape: the fluid pass runs in workgroups of 64, 128 or 256 threads
+// (identical results).  With FSA_BATCH_AUTOTUNE every recording first times
+// the batched fluid pass at each size on the scenes' current state -- the
+// pass reads the current state buffer and writes the other one, which the
+// next step overwrites, so the scenes do not advance -- and keeps the
+// fastest; otherwise it takes the first listed scene's size.
 #include <algorithm>
 #include <cstring>
 #include <new>
@@ -102,6 +109,7 @@ struct fsa_batch {
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     bool planned = false;
+    int fluid_threads = 0;  // workgroup size of the recorded fluid launches
 
     int64_t plans_built = 0;
     int64_t steps_run = 0;
@@ -141,8 +149,8 @@ void add_launch(std::vector<char> &blob, StepTables &t, Launch::Family fam, Batc
 }
 
 // tables of one step for scenes whose buffer indices are `ph`
-void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph, std::vector<char> &blob,
-                StepTables &t) {
+void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph, int fluid_threads,
+                std::vector<char> &blob, StepTables &t) {
     const int n = (int)sims.size();
     std::vector<CellJob> cells;
     std::vector<VirtJob> virt(n);
@@ -198,7 +206,7 @@ void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph
     for (const VirtJob &j : virt) blocks.push_back(blocks_virt(j));
     add_launch(blob, t, Launch::kVirt, kBFillCounts, blocks);
     blocks.clear();
-    for (const FluidJob &j : fluid) blocks.push_back(blocks_fluid(j));
+    for (const FluidJob &j : fluid) blocks.push_back(blocks_fluid(j, fluid_threads));
     add_launch(blob, t, Launch::kFluid, kBFillCounts, blocks);
     for (int sub = 0; sub < smax; ++sub)
         for (BatchOp op : {kBDeform, kBForce, kBSolidUpdate}) {
@@ -228,7 +236,7 @@ hipError_t enqueue_step(fsa_batch *b, const StepTables &t) {
             break;
         case Launch::kFluid:
             e = launch_many_fluid(reinterpret_cast<const FluidJob *>(base + t.fluid_off), prefix, l.njobs,
-                                  l.nblocks, b->stream);
+                                  l.nblocks, b->fluid_threads, b->stream);
             break;
         case Launch::kSolid:
             e = launch_many_solid(l.op, reinterpret_cast<const SolidJob *>(base + t.solid_off), prefix, l.njobs,
@@ -240,18 +248,84 @@ hipError_t enqueue_step(fsa_batch *b, const StepTables &t) {
     return hipSuccess;
 }
 
+// the fastest fluid workgroup size for these scenes, timed on their current
+// state (see the file comment); 0 on failure with the error recorded
+int tune_fluid(fsa_batch *b, const std::vector<fsa_sim *> &sims) {
+    const int n = (int)sims.size();
+    std::vector<FluidJob> jobs(n);
+    for (int i = 0; i < n; ++i) jobs[i] = FluidJob{sims[i]->P, fluid_args(sims[i])};
+    static constexpr int kChoices[3] = {64, 128, 256};
+    std::vector<char> blob;
+    const size_t joff = put(blob, jobs.data(), jobs.size());
+    size_t poff[3];
+    int nblk[3];
+    for (int c = 0; c < 3; ++c) {
+        std::vector<int> prefix(n + 1, 0);
+        for (int i = 0; i < n; ++i) prefix[i + 1] = prefix[i] + round_xcd(blocks_fluid(jobs[i], kChoices[c]));
+        nblk[c] = prefix.back();
+        poff[c] = put(blob, prefix.data(), prefix.size());
+    }
+    if (nblk[0] == 0) return kFluidThreadsDefault;  // no fluid anywhere
+    char *dev = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    auto fail = [&](hipError_t e) {
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+        if (dev) (void)hipFree(dev);
+        set_err(e == hipErrorOutOfMemory ? FSA_ERR_NOMEM : FSA_ERR_HIP,
+                std::string("batch fluid autotune: ") + hipGetErrorString(e));
+        return 0;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&dev, blob.size())) != hipSuccess) return fail(e);
+    if ((e = hipMemcpyAsync(dev, blob.data(), blob.size(), hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
+        (e = hipEventCreate(&t0)) != hipSuccess || (e = hipEventCreate(&t1)) != hipSuccess)
+        return fail(e);
+    float best = 3.0e38f;
+    int best_t = kFluidThreadsDefault;
+    const FluidJob *dj = reinterpret_cast<const FluidJob *>(dev + joff);
+    for (int c = 0; c < 3; ++c) {
+        const int *dp = reinterpret_cast<const int *>(dev + poff[c]);
+        const int t = kChoices[c];
+        if ((e = launch_many_fluid(dj, dp, n, nblk[c], t, b->stream)) != hipSuccess ||  // warm-up
+            (e = hipEventRecord(t0, b->stream)) != hipSuccess)
+            return fail(e);
+        for (int r = 0; r < 2; ++r)
+            if ((e = launch_many_fluid(dj, dp, n, nblk[c], t, b->stream)) != hipSuccess) return fail(e);
+        float ms = 0.0f;
+        if ((e = hipEventRecord(t1, b->stream)) != hipSuccess || (e = hipEventSynchronize(t1)) != hipSuccess ||
+            (e = hipEventElapsedTime(&ms, t0, t1)) != hipSuccess)
+            return fail(e);
+        if (ms < best) {
+            best = ms;
+            best_t = t;
+        }
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipFree(dev);
+    return best_t;
+}
+
 int build_plan(fsa_batch *b, const std::vector<fsa_sim *> &sims) {
     drop_graphs(b);
     b->planned = false;
     const int n = (int)sims.size();
+    if (b->flags & FSA_BATCH_AUTOTUNE) {
+        const int t = tune_fluid(b, sims);
+        if (t == 0) return FSA_ERR_HIP;
+        b->fluid_threads = t;
+    } else {
+        b->fluid_threads = sims[0]->fluid_threads;
+    }
     std::vector<Phase> ph(n), ph1(n);
     for (int i = 0; i < n; ++i) {
         ph[i] = phase_of(sims[i]);
         ph1[i] = next_phase(sims[i], ph[i]);
     }
     std::vector<char> blob;
-    build_step(sims, ph, blob, b->tables[0]);
-    build_step(sims, ph1, blob, b->tables[1]);
+    build_step(sims, ph, b->fluid_threads, blob, b->tables[0]);
+    build_step(sims, ph1, b->fluid_threads, blob, b->tables[1]);
     if (blob.size() > b->blob_cap) {
         HIP_TRY(hipStreamSynchronize(b->stream));  // the old tables may still be read
         if (b->dev_blob) (void)hipFree(b->dev_blob);
@@ -289,6 +363,7 @@ int build_plan(fsa_batch *b, const std::vector<fsa_sim *> &sims) {
 // which recorded step matches the scenes' current buffer indices (-1: none)
 int plan_phase(const fsa_batch *b, const std::vector<fsa_sim *> &sims) {
     if (!b->planned || sims.size() != b->serials.size()) return -1;
+    if (!(b->flags & FSA_BATCH_AUTOTUNE) && sims[0]->fluid_threads != b->fluid_threads) return -1;
     for (size_t i = 0; i < sims.size(); ++i)
         if (sims[i]->serial != b->serials[i]) return -1;
     for (int p = 0; p < 2; ++p) {
@@ -309,7 +384,8 @@ extern "C" {
 int fsa_batch_create(void *stream, unsigned flags, fsa_batch **out) {
     if (!out) return set_err(FSA_ERR_INVALID, "null argument");
     *out = nullptr;
-    if (flags & ~(unsigned)FSA_BATCH_NO_GRAPH) return set_err(FSA_ERR_INVALID, "unknown batch flag");
+    if (flags & ~(unsigned)(FSA_BATCH_NO_GRAPH | FSA_BATCH_AUTOTUNE))
+        return set_err(FSA_ERR_INVALID, "unknown batch flag");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return set_err(FSA_ERR_NODEVICE, "no HIP device visible");
@@ -407,10 +483,11 @@ int fsa_batch_synchronize(fsa_batch *b) {
     return FSA_OK;
 }
 
-int fsa_batch_stats(fsa_batch *b, int64_t *plans_built, int64_t *steps_run) {
+int fsa_batch_stats(fsa_batch *b, int64_t *plans_built, int64_t *steps_run, int *fluid_threads) {
     if (!b) return set_err(FSA_ERR_INVALID, "null batch");
     if (plans_built) *plans_built = b->plans_built;
     if (steps_run) *steps_run = b->steps_run;
+    if (fluid_threads) *fluid_threads = b->fluid_threads;
     return FSA_OK;
 }
 

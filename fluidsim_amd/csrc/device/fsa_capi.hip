@@ -692,7 +692,7 @@ int fsa_fluid_calculation(fsa_sim *s) {
     KindBuffers &f = s->fluid;
     FluidStepArgs a = fluid_args(s);
     a.rates = (s->debug & FSA_DEBUG_PAIR_SUMS) ? s->rates : nullptr;
-    HIP_TRY(launch_fluid(s->P, a, s->stream));
+    HIP_TRY(launch_fluid(s->P, a, s->fluid_threads, s->stream));
     f.st = 1 - f.st;
     stage_end(s, FSA_STAGE_FLUID);
     return FSA_OK;
@@ -916,6 +916,61 @@ int fsa_probe_pressure(fsa_sim *s, const float *points, int n, float *pressure) 
     HIP_TRY(launch_probe(s->P, observe_args(s), s->probe_pts, n, s->probe_out, s->stream));
     HIP_TRY(hipMemcpyAsync(pressure, s->probe_out, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    return FSA_OK;
+}
+
+int fsa_set_fluid_threads(fsa_sim *s, int threads) {
+    CHECK_SIM(s);
+    if (!fluid_threads_valid(threads)) return set_err(FSA_ERR_INVALID, "fluid workgroup size must be 64, 128 or 256");
+    s->fluid_threads = threads;
+    return FSA_OK;
+}
+
+int fsa_get_fluid_threads(fsa_sim *s, int *threads) {
+    CHECK_SIM(s);
+    if (!threads) return set_err(FSA_ERR_INVALID, "null output");
+    *threads = s->fluid_threads;
+    return FSA_OK;
+}
+
+int fsa_autotune_fluid(fsa_sim *s, int repeats, int *threads) {
+    CHECK_SIM(s);
+    if (repeats < 1) return set_err(FSA_ERR_INVALID, "repeats must be >= 1");
+    int rc = require_uploaded(s);
+    if (rc) return rc;
+    if (s->P.nf > 0) {
+        // the pass reads the current state buffer and writes the other one,
+        // which the next step overwrites: timing it leaves the scene as it was
+        const FluidStepArgs a = fluid_args(s);
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        HIP_TRY(hipEventCreate(&t0));
+        if (hipEventCreate(&t1) != hipSuccess) {
+            (void)hipEventDestroy(t0);
+            return set_err(FSA_ERR_HIP, "hipEventCreate failed");
+        }
+        float best = 3.0e38f;
+        int best_t = s->fluid_threads;
+        hipError_t e = hipSuccess;
+        for (int t : {64, 128, 256}) {
+            if ((e = launch_fluid(s->P, a, t, s->stream)) != hipSuccess) break;  // warm-up
+            if ((e = hipEventRecord(t0, s->stream)) != hipSuccess) break;
+            for (int r = 0; r < repeats && e == hipSuccess; ++r) e = launch_fluid(s->P, a, t, s->stream);
+            if (e != hipSuccess || (e = hipEventRecord(t1, s->stream)) != hipSuccess ||
+                (e = hipEventSynchronize(t1)) != hipSuccess)
+                break;
+            float ms = 0.0f;
+            if ((e = hipEventElapsedTime(&ms, t0, t1)) != hipSuccess) break;
+            if (ms < best) {
+                best = ms;
+                best_t = t;
+            }
+        }
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        HIP_TRY(e);
+        s->fluid_threads = best_t;
+    }
+    if (threads) *threads = s->fluid_threads;
     return FSA_OK;
 }
 

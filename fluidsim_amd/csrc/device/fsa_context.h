@@ -47,6 +47,8 @@ struct fsa_sim {
     float *probe_out = nullptr;
     int probe_cap = 0;
 
+    int fluid_threads = fsa::kFluidThreadsDefault;  // workgroup size of the fluid pass
+
     bool uploaded[3] = {false, false, false};
     bool hashed = false;  // cell ids/ranks of the current order are fresh
     unsigned debug = 0;

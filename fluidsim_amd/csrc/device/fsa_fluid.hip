@@ -101,7 +101,9 @@ __device__ __forceinline__ void one_pair(const DevParams &P, const Self &me, con
     }
 }
 
-constexpr int kThreads = 64;    // one wave per workgroup (LDS per wave bounds the occupancy)
+// workgroup sizes of the fluid pass (one to four waves; LDS per wave bounds
+// the occupancy, so the best size depends on the scene: fsa_autotune_fluid)
+constexpr int kThreadChoices[] = {64, 128, 256};
 constexpr int kListCap = 96;    // 16-bit list entries per lane
 constexpr int kRunBits = 12;    // offset of an entry inside its run
 constexpr int kRunMax = 1 << kRunBits;
@@ -112,7 +114,9 @@ constexpr int kScanBatch = 8;   // candidate loads in flight in the test loop
 static_assert(kPosPad >= kScanBatch, "candidate batches may read kScanBatch-1 entries past a range");
 static_assert(kBaseSlots <= 16 && kBaseSlots * kRunMax <= 65536, "entries are 16 bits");
 
-// lane-private LDS views: entry k at list[k * kThreads], base b at base[b * kThreads]
+// lane-private LDS views of a T-thread workgroup: entry k at list[k * T],
+// base b at base[b * T]
+template <int T>
 struct Lists {
     uint16_t *list;
     int *base;
@@ -120,14 +124,14 @@ struct Lists {
 
 // partner state of list entries k .. k+kPairBatch-1 (entries past cnt re-read
 // the last one, always a valid slot)
-template <int KIND>
+template <int KIND, int T>
 __device__ __forceinline__ void gather(const float4 *__restrict__ pos, const float4 *__restrict__ vel,
-                                       const float4 *__restrict__ np, const Lists &L, int k, int cnt, float4 *pj,
+                                       const float4 *__restrict__ np, const Lists<T> &L, int k, int cnt, float4 *pj,
                                        float4 *vj, float4 *nj) {
 #pragma unroll
     for (int q = 0; q < kPairBatch; ++q) {
-        const int v = L.list[min(k + q, cnt - 1) * kThreads];
-        const int j = L.base[(v >> kRunBits) * kThreads] + (v & (kRunMax - 1));
+        const int v = L.list[min(k + q, cnt - 1) * T];
+        const int j = L.base[(v >> kRunBits) * T] + (v & (kRunMax - 1));
         pj[q] = pos[j];
         vj[q] = vel[j];
         if (KIND == kFluidSolid) nj[q] = np[j];
@@ -138,16 +142,16 @@ __device__ __forceinline__ void gather(const float4 *__restrict__ pos, const flo
 // the next kPairBatch entries is gathered while the current ones are
 // evaluated; all kPairBatch slots of a round are evaluated, the ones past
 // cnt with weight 0.
-template <int KIND>
+template <int KIND, int T>
 __device__ __forceinline__ void evaluate(const DevParams &P, const Self &me, const float4 *__restrict__ pos,
                                          const float4 *__restrict__ vel, const float4 *__restrict__ np,
-                                         const Lists &L, int cnt, Acc &acc) {
+                                         const Lists<T> &L, int cnt, Acc &acc) {
     if (__ballot(cnt > 0) == 0) return;
     float4 pj[kPairBatch], vj[kPairBatch], nj[kPairBatch];
-    if (cnt > 0) gather<KIND>(pos, vel, np, L, 0, cnt, pj, vj, nj);  // an empty list holds no valid slot
+    if (cnt > 0) gather<KIND, T>(pos, vel, np, L, 0, cnt, pj, vj, nj);  // an empty list holds no valid slot
     for (int k = 0; k < cnt; k += kPairBatch) {
         float4 pn[kPairBatch], vn[kPairBatch], nn[kPairBatch];
-        gather<KIND>(pos, vel, np, L, k + kPairBatch, cnt, pn, vn, nn);
+        gather<KIND, T>(pos, vel, np, L, k + kPairBatch, cnt, pn, vn, nn);
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q) one_pair<KIND>(P, me, pj[q], vj[q], nj[q], k + q < cnt ? 1.0f : 0.0f, acc);
 #pragma unroll
@@ -167,23 +171,23 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const Self &me, con
 // end, masked out), and a full list or a full spare table is evaluated by
 // all lanes together.  The next batch of candidates is loaded while the
 // current one is tested.
-template <int KIND>
+template <int KIND, int T>
 __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col, int b, int e,
                                      const float4 *__restrict__ pos, const float4 *__restrict__ vel,
-                                     const float4 *__restrict__ np, const Lists &L, int &cnt, int &spare, Acc &acc) {
+                                     const float4 *__restrict__ np, const Lists<T> &L, int &cnt, int &spare, Acc &acc) {
     int sb = b;
     for (int sub = 0; __ballot(sb < e) != 0; ++sub, sb += kRunMax) {
         const bool has = sb < e;
         int slot = col;
         if (sub > 0) {
             if (__ballot(has && spare == kBaseSlots - kColumns) != 0) {
-                evaluate<KIND>(P, me, pos, vel, np, L, cnt, acc);
+                evaluate<KIND, T>(P, me, pos, vel, np, L, cnt, acc);
                 cnt = 0;
                 spare = 0;
             }
             if (has) {
                 slot = kColumns + spare;
-                L.base[slot * kThreads] = sb;
+                L.base[slot * T] = sb;
                 spare += 1;
             }
         }
@@ -209,7 +213,7 @@ __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col
             for (int k = 0; k < kScanBatch; ++k) {
                 const float d2 = len2(q[k].x - me.x, q[k].y - me.y, q[k].z - me.z);
                 const bool hit = (jj + k < se) & (d2 < P.R2);
-                L.list[cnt * kThreads] = (uint16_t)(tag + jj + k);
+                L.list[cnt * T] = (uint16_t)(tag + jj + k);
                 cnt += hit ? 1 : 0;
             }
 #pragma unroll
@@ -217,7 +221,7 @@ __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col
             // at most kScanBatch entries are added per round; the bases stay
             // valid across an evaluation
             if (__ballot(cnt > kListCap - kScanBatch) != 0) {
-                evaluate<KIND>(P, me, pos, vel, np, L, cnt, acc);
+                evaluate<KIND, T>(P, me, pos, vel, np, L, cnt, acc);
                 cnt = 0;
             }
         }
@@ -251,28 +255,28 @@ struct Columns {
 };
 
 // all partners of one kind: the 9 columns in order, then one evaluation
-template <int KIND>
+template <int KIND, int T>
 __device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cell, const int32_t *__restrict__ start,
                                      const float4 *__restrict__ pos, const float4 *__restrict__ vel,
-                                     const float4 *__restrict__ np, const Lists &L, Acc &acc) {
+                                     const float4 *__restrict__ np, const Lists<T> &L, Acc &acc) {
     const Columns cols(P, cell);
     int cnt = 0, spare = 0;
 #pragma unroll 1
     for (int col = 0; col < kColumns; ++col) {
         const int b = start[cols.c0(col)], e = start[cols.c1(col)];
-        L.base[col * kThreads] = b;
-        scan<KIND>(P, me, col, b, e, pos, vel, np, L, cnt, spare, acc);
+        L.base[col * T] = b;
+        scan<KIND, T>(P, me, col, b, e, pos, vel, np, L, cnt, spare, acc);
     }
-    evaluate<KIND>(P, me, pos, vel, np, L, cnt, acc);
+    evaluate<KIND, T>(P, me, pos, vel, np, L, cnt, acc);
 }
 
-template <bool RATES>
+template <bool RATES, int T>
 __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepArgs &a, int blk, int nblk) {
-    __shared__ uint16_t s_list[kListCap * kThreads];
-    __shared__ int s_base[kBaseSlots * kThreads];
-    const int i = xcd_tile(blk, nblk) * kThreads + threadIdx.x;
+    __shared__ uint16_t s_list[kListCap * T];
+    __shared__ int s_base[kBaseSlots * T];
+    const int i = xcd_tile(blk, nblk) * T + threadIdx.x;
     if (i >= P.nf) return;  // lists are lane-private: no barrier follows
-    const Lists list{s_list + threadIdx.x, s_base + threadIdx.x};
+    const Lists<T> list{s_list + threadIdx.x, s_base + threadIdx.x};
 
     const float4 pi4 = a.fpos[i];
     const float4 vi4 = a.fvel[i];
@@ -291,9 +295,9 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
 
     // partners kind by kind: fluid, solid, wall (the canonical order)
     const int cell = a.fcell[i];
-    pass<kFluidFluid>(P, me, cell, a.fstart, a.fpos, a.fvel, nullptr, list, acc);
-    pass<kFluidSolid>(P, me, cell, a.sstart, a.spos, a.svel, a.snp, list, acc);
-    pass<kFluidVirt>(P, me, cell, a.vstart, a.vpos, a.vvel, nullptr, list, acc);
+    pass<kFluidFluid, T>(P, me, cell, a.fstart, a.fpos, a.fvel, nullptr, list, acc);
+    pass<kFluidSolid, T>(P, me, cell, a.sstart, a.spos, a.svel, a.snp, list, acc);
+    pass<kFluidVirt, T>(P, me, cell, a.vstart, a.vpos, a.vvel, nullptr, list, acc);
 
     // fluidUpdate (fluid.cu:234-269): NaN scrub, explicit step with the old
     // velocity for the position, density clamp, EOS, domain clamp, speed cap
@@ -322,31 +326,58 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     a.fvel_out[i] = make_float4(ux, uy, uz, p);
 }
 
-template <bool RATES>
+template <bool RATES, int T>
 struct OpFluid {
-    static __device__ void run(const FluidJob &j, int blk, int nblk) { fluid_body<RATES>(j.P, j.a, blk, nblk); }
+    static __device__ void run(const FluidJob &j, int blk, int nblk) { fluid_body<RATES, T>(j.P, j.a, blk, nblk); }
 };
 
 }  // namespace
 
-int blocks_fluid(const FluidJob &j) { return (int)(((long long)j.P.nf + kThreads - 1) / kThreads); }
-
-hipError_t launch_fluid(const DevParams &P, const FluidStepArgs &a, hipStream_t s) {
-    if (P.nf == 0) return hipSuccess;
-    const FluidJob j{P, a};
-    const dim3 grid(blocks_fluid(j));
-    if (a.rates)
-        hipLaunchKernelGGL((k_one<OpFluid<true>, FluidJob, kThreads>), grid, dim3(kThreads), 0, s, j);
-    else
-        hipLaunchKernelGGL((k_one<OpFluid<false>, FluidJob, kThreads>), grid, dim3(kThreads), 0, s, j);
-    return hipGetLastError();
+bool fluid_threads_valid(int threads) {
+    for (int t : kThreadChoices)
+        if (t == threads) return true;
+    return false;
 }
 
-hipError_t launch_many_fluid(const FluidJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s) {
-    if (nblocks <= 0 || njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_many<OpFluid<false>, FluidJob, kThreads>), dim3(nblocks), dim3(kThreads), 0, s, jobs,
-                       prefix, njobs);
+int blocks_fluid(const FluidJob &j, int threads) { return (int)(((long long)j.P.nf + threads - 1) / threads); }
+
+namespace {
+template <int T>
+hipError_t one_fluid(const FluidJob &j, hipStream_t s) {
+    const dim3 grid(blocks_fluid(j, T));
+    if (j.a.rates)
+        hipLaunchKernelGGL((k_one<OpFluid<true, T>, FluidJob, T>), grid, dim3(T), 0, s, j);
+    else
+        hipLaunchKernelGGL((k_one<OpFluid<false, T>, FluidJob, T>), grid, dim3(T), 0, s, j);
     return hipGetLastError();
+}
+template <int T>
+hipError_t many_fluid(const FluidJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_many<OpFluid<false, T>, FluidJob, T>), dim3(nblocks), dim3(T), 0, s, jobs, prefix, njobs);
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_fluid(const DevParams &P, const FluidStepArgs &a, int threads, hipStream_t s) {
+    if (P.nf == 0) return hipSuccess;
+    const FluidJob j{P, a};
+    switch (threads) {
+    case 64: return one_fluid<64>(j, s);
+    case 128: return one_fluid<128>(j, s);
+    case 256: return one_fluid<256>(j, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_many_fluid(const FluidJob *jobs, const int *prefix, int njobs, int nblocks, int threads,
+                             hipStream_t s) {
+    if (nblocks <= 0 || njobs <= 0) return hipSuccess;
+    switch (threads) {
+    case 64: return many_fluid<64>(jobs, prefix, njobs, nblocks, s);
+    case 128: return many_fluid<128>(jobs, prefix, njobs, nblocks, s);
+    case 256: return many_fluid<256>(jobs, prefix, njobs, nblocks, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace fsa

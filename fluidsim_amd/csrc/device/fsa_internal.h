@@ -159,7 +159,10 @@ struct FluidJob {
     DevParams P;
     FluidStepArgs a;
 };
-hipError_t launch_fluid(const DevParams &P, const FluidStepArgs &a, hipStream_t s);
+// workgroup sizes of the fluid pass: 64, 128 or 256 threads (identical results)
+constexpr int kFluidThreadsDefault = 64;
+bool fluid_threads_valid(int threads);
+hipError_t launch_fluid(const DevParams &P, const FluidStepArgs &a, int threads, hipStream_t s);
 
 // ---- walls (fsa_virt.hip) ------------------------------------------------------
 struct VirtArgs {
@@ -206,8 +209,9 @@ enum BatchOp {
 };
 hipError_t launch_many_cells(BatchOp op, const CellJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s);
 int blocks_cells(BatchOp op, const CellJob &j);
-hipError_t launch_many_fluid(const FluidJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s);
-int blocks_fluid(const FluidJob &j);
+hipError_t launch_many_fluid(const FluidJob *jobs, const int *prefix, int njobs, int nblocks, int threads,
+                             hipStream_t s);
+int blocks_fluid(const FluidJob &j, int threads);
 hipError_t launch_many_virt(const VirtJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s);
 int blocks_virt(const VirtJob &j);
 hipError_t launch_many_solid(BatchOp op, const SolidJob *jobs, const int *prefix, int njobs, int nblocks,

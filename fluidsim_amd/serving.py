@@ -85,7 +85,7 @@ class Server:
     (default WallClock); StepClock makes a run reproducible."""
 
     def __init__(self, memory_budget: int | None = None, max_scenes: int = 64, chunk_steps: int = 32,
-                 graphs: bool = True, keep_state: bool = False, on_event=None):
+                 graphs: bool = True, autotune: bool = True, keep_state: bool = False, on_event=None):
         if memory_budget is None:
             free, _ = device_memory()
             memory_budget = int(0.9 * free)
@@ -93,6 +93,7 @@ class Server:
         self.max_scenes = max_scenes
         self.chunk_steps = chunk_steps
         self.graphs = graphs
+        self.autotune = autotune  # time the fluid workgroup sizes whenever the batch is re-recorded
         self.keep_state = keep_state
         self.on_event = on_event  # called as on_event(kind, request id, time), kind "admit" or "finish"
         self.stats = ServerStats()
@@ -127,7 +128,7 @@ class Server:
         results: list = []
         used = 0
         st = self.stats
-        batch = DeviceBatch(graphs=self.graphs)
+        batch = DeviceBatch(graphs=self.graphs, autotune=self.autotune)
         try:
             while pending or queue or active:
                 now = clock()

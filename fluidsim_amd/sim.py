@@ -186,6 +186,19 @@ class DeviceSim:
     def reset_stage_times(self):
         check(self.lib.fsa_reset_stage_times(self.handle))
 
+    def set_fluid_threads(self, threads: int):
+        check(self.lib.fsa_set_fluid_threads(self.handle, int(threads)))
+
+    def fluid_threads(self) -> int:
+        v = C.c_int()
+        check(self.lib.fsa_get_fluid_threads(self.handle, C.byref(v)))
+        return int(v.value)
+
+    def autotune_fluid(self, repeats: int = 3) -> int:
+        v = C.c_int()
+        check(self.lib.fsa_autotune_fluid(self.handle, int(repeats), C.byref(v)))
+        return int(v.value)
+
     def memory_used(self) -> int:
         v = C.c_int64()
         check(self.lib.fsa_memory_used(self.handle, C.byref(v)))
@@ -197,10 +210,10 @@ class DeviceBatch:
     (fsa_batch_* of include/fluidsim_amd.h).  The list of contexts is given on
     every call and may change between calls."""
 
-    def __init__(self, stream: int | None = None, graphs: bool = True):
+    def __init__(self, stream: int | None = None, graphs: bool = True, autotune: bool = False):
         self.lib = _native.device_lib()
         h = C.c_void_p()
-        flags = 0 if graphs else _native.FSA_BATCH_NO_GRAPH
+        flags = (0 if graphs else _native.FSA_BATCH_NO_GRAPH) | (_native.FSA_BATCH_AUTOTUNE if autotune else 0)
         check(self.lib.fsa_batch_create(C.c_void_p(stream) if stream else None, flags, C.byref(h)))
         self.handle = h
 
@@ -229,9 +242,9 @@ class DeviceBatch:
         check(self.lib.fsa_batch_synchronize(self.handle))
 
     def stats(self) -> dict:
-        plans, steps = C.c_int64(), C.c_int64()
-        check(self.lib.fsa_batch_stats(self.handle, C.byref(plans), C.byref(steps)))
-        return {"plans_built": int(plans.value), "steps_run": int(steps.value)}
+        plans, steps, threads = C.c_int64(), C.c_int64(), C.c_int()
+        check(self.lib.fsa_batch_stats(self.handle, C.byref(plans), C.byref(steps), C.byref(threads)))
+        return {"plans_built": int(plans.value), "steps_run": int(steps.value), "fluid_threads": int(threads.value)}
 
 
 def footprint(params: FsaParams) -> tuple:

@@ -251,6 +251,21 @@ int fsa_reset_stage_times(fsa_sim *sim);
 /* Device bytes held by the context. */
 int fsa_memory_used(fsa_sim *sim, int64_t *bytes);
 
+/* ---- launch shape of the fluid pass --------------------------------------
+ * The reference launches every kernel with `thread_num` threads per block
+ * (src/fluid.cu:276-282).  Here the fluid pass (K8+K9)
+ * runs in workgroups of 64, 128 or 256 threads; all three give identical
+ * results, and the fastest depends on the scene (LDS per wave bounds the
+ * occupancy).  The default is 64. */
+int fsa_set_fluid_threads(fsa_sim *sim, int threads);
+int fsa_get_fluid_threads(fsa_sim *sim, int *threads);
+/* Time the fluid pass at each workgroup size on the current state (one
+ * warm-up and `repeats` timed launches each), keep the fastest and return it
+ * in *threads (may be NULL).  The scene does not advance: the pass reads the
+ * current state buffer and writes the other one, which the next step
+ * overwrites.  Synchronizes the context stream. */
+int fsa_autotune_fluid(fsa_sim *sim, int repeats, int *threads);
+
 /* Device bytes fsa_create would allocate for these parameters (= what
  * fsa_memory_used reports after creation, before optional debug, census and
  * probe buffers), and the peak while fsa_upload_virt runs (it sorts the walls
@@ -285,6 +300,8 @@ int fsa_device_memory(int64_t *free_bytes, int64_t *total_bytes);
 typedef struct fsa_batch fsa_batch;
 
 #define FSA_BATCH_NO_GRAPH 1u /* launch every stage directly instead of replaying a HIP graph */
+#define FSA_BATCH_AUTOTUNE 2u /* time the batched fluid pass at each workgroup size whenever the
+                                 launches are recorded, and keep the fastest */
 
 /* stream: hipStream_t to run on, or NULL for a batch-owned stream. */
 int fsa_batch_create(void *stream, unsigned flags, fsa_batch **out);
@@ -292,8 +309,11 @@ int fsa_batch_destroy(fsa_batch *batch);
 /* n_steps full steps of every listed context (no context listed twice). */
 int fsa_batch_step(fsa_batch *batch, fsa_sim *const *scenes, int n_scenes, int n_steps);
 int fsa_batch_synchronize(fsa_batch *batch);
-/* How often the launches were (re)recorded, and steps run, since creation. */
-int fsa_batch_stats(fsa_batch *batch, int64_t *plans_built, int64_t *steps_run);
+/* How often the launches were (re)recorded and steps run since creation, and
+ * the fluid workgroup size of the current recording (0 before the first).
+ * Without FSA_BATCH_AUTOTUNE a recording uses the first listed scene's size
+ * (fsa_set_fluid_threads). */
+int fsa_batch_stats(fsa_batch *batch, int64_t *plans_built, int64_t *steps_run, int *fluid_threads);
 
 #ifdef __cplusplus
 }

@@ -78,7 +78,7 @@ def test_batched_steps_are_bit_identical_to_single_steps(gpu_available, graphs):
         b.step([mixed[n] for n in names], 3)
         for n in names:
             alone[n].step(3)
-        assert b.stats() == {"plans_built": 1, "steps_run": 3}
+        assert b.stats()["plans_built"] == 1 and b.stats()["steps_run"] == 3
         # 2) same list again: the plan continues at the right recorded step
         b.step([mixed[n] for n in names], 2)
         for n in names:
@@ -109,7 +109,7 @@ def test_batch_of_one_matches_and_reuses_its_plan(gpu_available):
         for _ in range(4):
             b.step([one], 1)
         ref.step(4)
-        assert b.stats() == {"plans_built": 1, "steps_run": 4}
+        assert b.stats()["plans_built"] == 1 and b.stats()["steps_run"] == 4
     _assert_same("single", _state(one), _state(ref))
 
 
@@ -126,3 +126,49 @@ def test_batch_errors_are_status_codes(gpu_available):
             b.step([d, not_uploaded], 1)
         b.step([], 1)  # nothing to do
         assert b.stats()["steps_run"] == 0
+
+
+def test_fluid_workgroup_sizes_give_identical_bits(gpu_available):
+    """The fluid pass at 64, 128 and 256 threads per workgroup: same bits."""
+    sc = scenes.tiny_scene(seed=23, fluid=(12, 12, 8), tank=(22, 16, 8), gate=(2, 10, 8), gate_x0=16)
+    states = []
+    for t in (64, 128, 256):
+        d = _make(sc)
+        d.set_fluid_threads(t)
+        assert d.fluid_threads() == t
+        d.step(5)
+        states.append(_state(d))
+        d.close()
+    _assert_same("128 vs 64", states[1], states[0])
+    _assert_same("256 vs 64", states[2], states[0])
+    d = _make(sc)
+    with pytest.raises(FsaError, match="64, 128 or 256"):
+        d.set_fluid_threads(96)
+
+
+def test_autotune_keeps_the_scene_where_it_was(gpu_available):
+    sc = scenes.tiny_scene(seed=24)
+    tuned, ref = _make(sc), _make(sc)
+    tuned.step(2)
+    ref.step(2)
+    before = _state(tuned)
+    t = tuned.autotune_fluid(repeats=2)
+    assert t in (64, 128, 256) and tuned.fluid_threads() == t
+    _assert_same("autotune", _state(tuned), before)
+    tuned.step(3)
+    ref.step(3)
+    _assert_same("after autotune", _state(tuned), _state(ref))
+
+
+def test_batch_autotune_matches_single_steps(gpu_available):
+    cases = _variants()[:4]
+    alone = [_make(sc) for _, sc in cases]
+    mixed = [_make(sc) for _, sc in cases]
+    with DeviceBatch(autotune=True) as b:
+        b.step(mixed, 3)
+        st = b.stats()
+        assert st["plans_built"] == 1 and st["fluid_threads"] in (64, 128, 256)
+    for d in alone:
+        d.step(3)
+    for (n, _), x, y in zip(cases, mixed, alone):
+        _assert_same(n, _state(x), _state(y))
