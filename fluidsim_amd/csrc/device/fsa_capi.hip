@@ -136,7 +136,9 @@ int alloc_all(fsa_sim *s) {
     if ((rc = s->alloc(&s->sx.np, ns)) || (rc = s->alloc(&s->sx.acc, ns)) || (rc = s->alloc(&s->sx.couple, ns)) ||
         (rc = s->alloc(&s->sx.F, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.P, 9 * (size_t)ns)) ||
         (rc = s->alloc(&s->sx.L, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.PL, 9 * (size_t)ns)) ||
-        (rc = s->alloc(&s->sx.A, 9 * (size_t)ns)))
+        (rc = s->alloc(&s->sx.A, 9 * (size_t)ns)) ||
+        (rc = s->alloc(&s->sx.nbr, (size_t)ns * kSolidListCap * kSolidLanes)) ||
+        (rc = s->alloc(&s->sx.ncnt, (size_t)ns * kSolidLanes)))
         return rc;
     if ((rc = s->alloc(&s->vx.nrm, s->P.nv)) || (rc = s->alloc(&s->virt_x, s->P.nv))) return rc;
     if ((rc = s->alloc(&s->jumps, 1))) return rc;
@@ -706,7 +708,7 @@ int fsa_solid_calculation(fsa_sim *s) {
     if ((rc = stage_open(s, FSA_STAGE_SOLID))) return rc;
     const SolidArgs a = solid_args(s);
     for (int sub = 0; sub < s->P.sub; ++sub) {
-        HIP_TRY(launch_solid_deformation(s->P, a, s->stream));
+        HIP_TRY(launch_solid_deformation(s->P, a, sub == 0, s->stream));
         HIP_TRY(launch_solid_force(s->P, a, s->stream));
         HIP_TRY(launch_solid_update(s->P, a, s->stream));
     }

@@ -41,6 +41,9 @@ constexpr int kScanTile = 1024;  // elements per scan workgroup (256 threads x 4
 // results past the range are masked out of every test)
 constexpr int kPosPad = 16;
 constexpr int kSolidLanes = 16;  // lanes serving one solid particle
+// solid-solid partners recorded per lane in the first sub-step of a step
+// (entries per lane; a lane with more partners walks its candidates instead)
+constexpr int kSolidListCap = 16;
 
 // the kernels' view of fsa_params
 struct DevParams {
@@ -89,6 +92,11 @@ struct SolidExtra {
     float4 *acc = nullptr;  // du_dt (w unused)
     float4 *couple = nullptr;
     float *F = nullptr, *P = nullptr, *L = nullptr, *PL = nullptr, *A = nullptr;  // 9 * ns each
+    // reference-configuration partners of the step: lane l of particle i holds
+    // its hits in walk order at nbr[(i * kSolidListCap + k) * kSolidLanes + l],
+    // their number (uncapped) at ncnt[i * kSolidLanes + l]
+    int32_t *nbr = nullptr;
+    int32_t *ncnt = nullptr;
 };
 
 struct VirtExtra {
@@ -193,7 +201,9 @@ struct SolidJob {
     SolidArgs a;
 };
 hipError_t launch_couple(const DevParams &P, const SolidArgs &a, hipStream_t s);
-hipError_t launch_solid_deformation(const DevParams &P, const SolidArgs &a, hipStream_t s);
+// record = true in the first sub-step of a step: walks the candidates and
+// records every lane's partners; later passes replay the records
+hipError_t launch_solid_deformation(const DevParams &P, const SolidArgs &a, bool record, hipStream_t s);
 hipError_t launch_solid_force(const DevParams &P, const SolidArgs &a, hipStream_t s);
 hipError_t launch_solid_update(const DevParams &P, const SolidArgs &a, hipStream_t s);
 
@@ -205,7 +215,7 @@ hipError_t launch_solid_update(const DevParams &P, const SolidArgs &a, hipStream
 // a block's XCD is the same in the batched grid as in a single-scene grid.
 enum BatchOp {
     kBFillCounts, kBHash, kBScanTiles, kBScanSums, kBScanAdd, kBPlace, kBGather,  // CellJob
-    kBCouple, kBDeform, kBForce, kBSolidUpdate                                     // SolidJob
+    kBCouple, kBDeformRecord, kBDeform, kBForce, kBSolidUpdate                     // SolidJob
 };
 hipError_t launch_many_cells(BatchOp op, const CellJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s);
 int blocks_cells(BatchOp op, const CellJob &j);

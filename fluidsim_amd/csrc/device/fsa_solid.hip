@@ -154,7 +154,53 @@ __device__ __forceinline__ void couple_body(const DevParams &P, const SolidArgs 
     a.x.np[L.i] = make_float4(nx, ny, nz, p);
 }
 
+// ---- solid partners by reference distance ----------------------------------
+// The solid-solid terms (src/solid.cu:53-69, 195-203) take the solids j in
+// the 27 cells of i's cell with |X_j - X_i|^2 < R^2.  Lane `sub` of i's group
+// takes the candidates sub, sub + kSub, ... of every column run.  The cells
+// are fixed for the whole step, so this set and order are the same in every
+// sub-step: the first deformation pass of a step walks the candidates and
+// records each lane's partners (kWalkRecord), later passes replay the record
+// (kReplay).  A lane with more than kSolidListCap partners walks again, so
+// each lane visits the same partners in the same order on either path and
+// the sums have the same bits.
+enum PartnerMode { kWalkRecord = 0, kReplay = 1 };
+
+template <int MODE, class F>
+__device__ __forceinline__ void solid_partners(const DevParams &P, const SolidArgs &a, int i, int sub,
+                                               const float4 &Xi, F &&body) {
+    int32_t *rec = a.x.nbr + (size_t)i * (kSolidListCap * kSub) + sub;
+    int32_t *rec_n = a.x.ncnt + (size_t)i * kSub + sub;
+    if (MODE == kReplay) {
+        const int cnt = *rec_n;
+        if (cnt <= kSolidListCap) {
+            for (int k = 0; k < cnt; ++k) {
+                const int j = rec[k * kSub];
+                const float4 Xj = a.sref[j];
+                const float dX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
+                body(j, dX, len2(dX[0], dX[1], dX[2]));
+            }
+            return;
+        }
+    }
+    int cnt = 0;
+    for_each_column(P, a.scell[i], [&](int c0, int c1) {
+        for (int j = a.sstart[c0] + sub, e = a.sstart[c1]; j < e; j += kSub) {
+            const float4 Xj = a.sref[j];
+            const float dX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
+            const float d2 = len2(dX[0], dX[1], dX[2]);
+            if (d2 < P.R2) {
+                if (MODE == kWalkRecord && cnt < kSolidListCap) rec[cnt * kSub] = j;
+                ++cnt;
+                body(j, dX, d2);
+            }
+        }
+    });
+    if (MODE == kWalkRecord) *rec_n = cnt;
+}
+
 // ---- K10: solidDeformationCalculate (src/solid.cu:10-79) -----------------
+template <int MODE>
 __device__ __forceinline__ void deformation_body(const DevParams &P, const SolidArgs &a, int blk) {
     const Lane L = lane_of(blk);
     if (L.i >= P.ns) return;
@@ -162,24 +208,17 @@ __device__ __forceinline__ void deformation_body(const DevParams &P, const Solid
     const float4 Xi = a.sref[i];
     const float4 xi = a.spos[i];
     M3 F = mzero(), B = mzero();
-    for_each_column(P, a.scell[i], [&](int c0, int c1) {
-        for (int j = a.sstart[c0] + L.sub, e = a.sstart[c1]; j < e; j += kSub) {
-            const float4 Xj = a.sref[j];
-            const float dX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
-            const float d2 = len2(dX[0], dX[1], dX[2]);
-            if (d2 < P.R2) {
-                const float4 xj = a.spos[j];
-                const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
-                const float kc = expf(-(d2 * P.h_inv2)) * P.kd;
+    solid_partners<MODE>(P, a, i, L.sub, Xi, [&](int j, const float (&dX)[3], float d2) {
+        const float4 xj = a.spos[j];
+        const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+        const float kc = expf(-(d2 * P.h_inv2)) * P.kd;
 #pragma unroll
-                for (int r = 0; r < 3; ++r)
+        for (int r = 0; r < 3; ++r)
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        F.m[r][c] += (dx[r] * dX[c]) * kc;
-                        B.m[r][c] += (dX[r] * dX[c]) * kc;
-                    }
+            for (int c = 0; c < 3; ++c) {
+                F.m[r][c] += (dx[r] * dX[c]) * kc;
+                B.m[r][c] += (dX[r] * dX[c]) * kc;
             }
-        }
     });
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -266,47 +305,42 @@ __device__ __forceinline__ void force_body(const DevParams &P, const SolidArgs &
                 cz += (s * dz) * kc;
             }
         }
-        // stress divergence + artificial viscosity (solidSolidForce, solid.cu:119-157)
-        for (int j = a.sstart[c0] + L.sub, e = a.sstart[c1]; j < e; j += kSub) {
-            const float4 Xj = a.sref[j];
-            const float dX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
-            const float d2 = len2(dX[0], dX[1], dX[2]);
-            if (d2 < P.R2) {
-                const float kc = expf(-(d2 * P.h_inv2)) * P.kd;
-                float v[3];
+    });
+    // stress divergence + artificial viscosity (solidSolidForce, solid.cu:119-157)
+    solid_partners<kReplay>(P, a, i, L.sub, Xi, [&](int j, const float (&dX)[3], float d2) {
+        const float kc = expf(-(d2 * P.h_inv2)) * P.kd;
+        float v[3];
 #pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    float t = 0.0f;
+        for (int r = 0; r < 3; ++r) {
+            float t = 0.0f;
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) t += (a.x.PL[(size_t)(3 * r + c) * n + j] + PLi.m[r][c]) * dX[c];
-                    v[r] = t * kc / P.rho_s;
-                }
-                ax += v[0];
-                ay += v[1];
-                az += v[2];
-                const float4 xj = a.spos[j], uj = a.svel[j];
-                const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-                const float dudx = (uj.x - ui.x) * dx + (uj.y - ui.y) * dy + (uj.z - ui.z) * dz;
-                const float nu = P.h * dudx / (len2(dx, dy, dz) + eps);
-                float visc = P.alpha * nu * P.cs_s;
-                visc -= P.beta * nu * nu;
-                // ((((det F F^-T) visc) L_i) dX) k
-                M3 AV;
-#pragma unroll
-                for (int r = 0; r < 3; ++r)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) AV.m[r][c] = Ai.m[r][c] * visc;
-                const M3 AL = mmul(AV, Li);
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    const float t = AL.m[r][0] * dX[0] + AL.m[r][1] * dX[1] + AL.m[r][2] * dX[2];
-                    v[r] = t * kc;
-                }
-                ax += v[0];
-                ay += v[1];
-                az += v[2];
-            }
+            for (int c = 0; c < 3; ++c) t += (a.x.PL[(size_t)(3 * r + c) * n + j] + PLi.m[r][c]) * dX[c];
+            v[r] = t * kc / P.rho_s;
         }
+        ax += v[0];
+        ay += v[1];
+        az += v[2];
+        const float4 xj = a.spos[j], uj = a.svel[j];
+        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const float dudx = (uj.x - ui.x) * dx + (uj.y - ui.y) * dy + (uj.z - ui.z) * dz;
+        const float nu = P.h * dudx / (len2(dx, dy, dz) + eps);
+        float visc = P.alpha * nu * P.cs_s;
+        visc -= P.beta * nu * nu;
+        // ((((det F F^-T) visc) L_i) dX) k
+        M3 AV;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) AV.m[r][c] = Ai.m[r][c] * visc;
+        const M3 AL = mmul(AV, Li);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const float t = AL.m[r][0] * dX[0] + AL.m[r][1] * dX[1] + AL.m[r][2] * dX[2];
+            v[r] = t * kc;
+        }
+        ax += v[0];
+        ay += v[1];
+        az += v[2];
     });
     cx = group_sum(cx);
     cy = group_sum(cy);
@@ -343,8 +377,9 @@ __device__ __forceinline__ void update_body(const DevParams &P, const SolidArgs 
 struct OpCouple {
     static __device__ void run(const SolidJob &j, int blk, int) { couple_body(j.P, j.a, blk); }
 };
+template <int MODE>
 struct OpDeform {
-    static __device__ void run(const SolidJob &j, int blk, int) { deformation_body(j.P, j.a, blk); }
+    static __device__ void run(const SolidJob &j, int blk, int) { deformation_body<MODE>(j.P, j.a, blk); }
 };
 struct OpForce {
     static __device__ void run(const SolidJob &j, int blk, int) { force_body(j.P, j.a, blk); }
@@ -372,8 +407,9 @@ int blocks_solid(BatchOp op, const SolidJob &j) {
 hipError_t launch_couple(const DevParams &P, const SolidArgs &a, hipStream_t s) {
     return one<OpCouple>(P, a, blocks_solid(kBCouple, SolidJob{P, a}), s);
 }
-hipError_t launch_solid_deformation(const DevParams &P, const SolidArgs &a, hipStream_t s) {
-    return one<OpDeform>(P, a, blocks_solid(kBDeform, SolidJob{P, a}), s);
+hipError_t launch_solid_deformation(const DevParams &P, const SolidArgs &a, bool record, hipStream_t s) {
+    const int blocks = blocks_solid(kBDeform, SolidJob{P, a});
+    return record ? one<OpDeform<kWalkRecord>>(P, a, blocks, s) : one<OpDeform<kReplay>>(P, a, blocks, s);
 }
 hipError_t launch_solid_force(const DevParams &P, const SolidArgs &a, hipStream_t s) {
     return one<OpForce>(P, a, blocks_solid(kBForce, SolidJob{P, a}), s);
@@ -388,7 +424,10 @@ hipError_t launch_many_solid(BatchOp op, const SolidJob *jobs, const int *prefix
     const dim3 g(nblocks), b(kBlock);
     switch (op) {
     case kBCouple: hipLaunchKernelGGL((k_many<OpCouple, SolidJob>), g, b, 0, s, jobs, prefix, njobs); break;
-    case kBDeform: hipLaunchKernelGGL((k_many<OpDeform, SolidJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    case kBDeformRecord:
+        hipLaunchKernelGGL((k_many<OpDeform<kWalkRecord>, SolidJob>), g, b, 0, s, jobs, prefix, njobs);
+        break;
+    case kBDeform: hipLaunchKernelGGL((k_many<OpDeform<kReplay>, SolidJob>), g, b, 0, s, jobs, prefix, njobs); break;
     case kBForce: hipLaunchKernelGGL((k_many<OpForce, SolidJob>), g, b, 0, s, jobs, prefix, njobs); break;
     case kBSolidUpdate: hipLaunchKernelGGL((k_many<OpSolidUpdate, SolidJob>), g, b, 0, s, jobs, prefix, njobs); break;
     default: return hipErrorInvalidValue;

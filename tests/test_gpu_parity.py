@@ -94,6 +94,33 @@ def test_one_step_stage_by_stage(gpu_available, tiny):
     assert ok, ("du_dt", e_g, e_32)
 
 
+@pytest.mark.parametrize("h_ratio", [1.3, 1.9, 2.6])
+def test_solid_partner_records(gpu_available, h_ratio):
+    """The first solid sub-step of a step records each lane's reference
+    partners (at most 16 per lane) and the later deformation and force passes
+    replay them; a lane with more partners walks its candidates again.  A
+    thick gate at growing smoothing lengths covers both paths: ~5 partners per
+    lane at h = 1.3 dp (at most 11), a few lanes over the cap at 1.9 dp, a
+    tenth of them at 2.6 dp."""
+    sc = scenes.tiny_scene(seed=5, gate=(6, 8, 6), gate_x0=9, h_ratio=h_ratio, solid_sub_step=3)
+    a, p, dev, o = _setup(sc)
+    o64 = orc.Oracle(p, 64)
+    o64.upload(a)
+    print(f"case thick gate h_ratio={h_ratio}", p.n_fluid, p.n_solid, p.n_virt, flush=True)
+    for st in range(3):
+        for s in (dev, o, o64):
+            s.step(1)
+        gs, os_, os64 = dev.solid(), o.solid(), o64.solid()
+        L = max(sc.tank) * sc.dp
+        assert max_abs(gs["position"], os_["position"]) <= REL * L, st
+        for key in ("deformation", "correction"):
+            e_g, e_32, ok = same_class(gs[key], os_[key], os64[key], 1.0)
+            assert ok, (st, key, e_g, e_32)
+        amax = np.abs(os64["du_dt"]).max()
+        e_g, e_32, ok = same_class(gs["du_dt"], os_["du_dt"], os64["du_dt"], amax)
+        assert ok, (st, "du_dt", e_g, e_32)
+
+
 @pytest.mark.parametrize("steps", [1, 20])
 def test_multi_step_state(gpu_available, tiny, steps):
     a, p, dev, o = _setup(tiny)
