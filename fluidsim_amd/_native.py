@@ -25,6 +25,7 @@ FSA_ERR_NODEVICE = 5
 FLUID, SOLID, VIRT = 0, 1, 2
 UNCONSTRAINED_SOLID, FIXED_SOLID = 0, 1
 DEBUG_PAIR_SUMS = 1
+DEBUG_WALK = 2
 
 STAGES = ("cell", "couple", "virt", "fluid", "solid")
 

@@ -181,6 +181,8 @@ void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph
         // the solid sub-steps see the fluid after its pass (src/calculation.cu:31-32)
         if (f.n) f.st ^= 1;
         solid[i] = SolidJob{s->P, solid_args(s, f, so)};
+        solid[i].a.fluid_rec = (s->P.nf > 0 && s->P.ns > 0) ? 1 : 0;  // the couple launch of this step records
+        solid[i].a.solid_rec = 1;
         smax = std::max(smax, s->P.ns > 0 ? s->P.sub : 0);
     }
     t = StepTables{};
@@ -464,6 +466,7 @@ int fsa_batch_step(fsa_batch *b, fsa_sim *const *scenes, int n_scenes, int n_ste
         for (fsa_sim *s : sims) {
             apply_phase(s, next_phase(s, phase_of(s)));
             s->hashed = false;
+            s->couple_fresh = s->P.nf > 0 && s->P.ns > 0;
         }
         phase ^= 1;
         b->steps_run += 1;

@@ -75,6 +75,11 @@ DevParams make_dev_params(const fsa_params &p) {
     d.eps_vis = 0.01f * p.smoothing_length_square;
     d.visc10 = 10.0f * p.fluid_viscosity;
     d.max_speed = 0.3f * p.fluid_sound_speed;
+    {
+        const double R = std::sqrt((double)p.impact_distance_square), disp = 0.04 * R;
+        d.disp2_max = (float)(disp * disp);
+        d.R2_rec = (float)((R + disp) * (R + disp) * (1.0 + 1.0e-5));
+    }
     return d;
 }
 }  // namespace fsa
@@ -138,7 +143,9 @@ int alloc_all(fsa_sim *s) {
         (rc = s->alloc(&s->sx.L, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.PL, 9 * (size_t)ns)) ||
         (rc = s->alloc(&s->sx.A, 9 * (size_t)ns)) ||
         (rc = s->alloc(&s->sx.nbr, (size_t)ns * kSolidListCap * kSolidLanes)) ||
-        (rc = s->alloc(&s->sx.ncnt, (size_t)ns * kSolidLanes)))
+        (rc = s->alloc(&s->sx.ncnt, (size_t)ns * kSolidLanes)) ||
+        (rc = s->alloc(&s->sx.fnbr, (size_t)ns * kSolidListCap * kSolidLanes)) ||
+        (rc = s->alloc(&s->sx.fcnt, (size_t)ns * kSolidLanes)) || (rc = s->alloc(&s->sx.x0, ns)))
         return rc;
     if ((rc = s->alloc(&s->vx.nrm, s->P.nv)) || (rc = s->alloc(&s->virt_x, s->P.nv))) return rc;
     if ((rc = s->alloc(&s->jumps, 1))) return rc;
@@ -329,10 +336,9 @@ int fsa_create(const fsa_params *params, void *stream, fsa_sim **out) {
     s->P = make_dev_params(*params);
     static std::atomic<uint64_t> next_serial{1};
     s->serial = next_serial.fetch_add(1);
-    hipGetDevice(&s->device);
     {
         hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, s->device) != hipSuccess ||
+        if (hipGetDevice(&s->device) != hipSuccess || hipGetDeviceProperties(&prop, s->device) != hipSuccess ||
             std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
             delete s;
             return set_err(FSA_ERR_NODEVICE, "current device is not gfx950");
@@ -387,14 +393,14 @@ int fsa_device_memory(int64_t *free_bytes, int64_t *total_bytes) {
 
 int fsa_destroy(fsa_sim *s) {
     if (!s) return FSA_OK;
-    if (s->stream) hipStreamSynchronize(s->stream);
-    for (void *p : s->allocations) hipFree(p);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (void *p : s->allocations) (void)hipFree(p);
     for (int st = 0; st < FSA_STAGE_COUNT; ++st)
         for (int r = 0; r < fsa_sim::kRing; ++r) {
-            if (s->ev_begin[st][r]) hipEventDestroy(s->ev_begin[st][r]);
-            if (s->ev_end[st][r]) hipEventDestroy(s->ev_end[st][r]);
+            if (s->ev_begin[st][r]) (void)hipEventDestroy(s->ev_begin[st][r]);
+            if (s->ev_end[st][r]) (void)hipEventDestroy(s->ev_end[st][r]);
         }
-    if (s->own_stream && s->stream) hipStreamDestroy(s->stream);
+    if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return FSA_OK;
 }
@@ -497,7 +503,7 @@ int fsa_upload_virt(fsa_sim *s, const float *x, const int32_t *type) {
         float4 *pos_tmp = nullptr, *vel_tmp = nullptr;
         HIP_TRY(hipMalloc(&pos_tmp, sizeof(float4) * n));
         if (hipMalloc(&vel_tmp, sizeof(float4) * n) != hipSuccess) {
-            hipFree(pos_tmp);
+            (void)hipFree(pos_tmp);
             return set_err(FSA_ERR_NOMEM, "hipMalloc failed for wall scratch");
         }
         k.pos[1] = pos_tmp;
@@ -513,7 +519,7 @@ int fsa_upload_virt(fsa_sim *s, const float *x, const int32_t *type) {
         if (!err && hipMemcpyAsync(s->virt_x, k.pos[0], sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream) !=
                         hipSuccess)
             err = set_err(FSA_ERR_HIP, "wall position copy failed");
-        hipStreamSynchronize(s->stream);
+        if (hipStreamSynchronize(s->stream) != hipSuccess && !err) err = set_err(FSA_ERR_HIP, "wall upload failed");
         (void)hipFree(pos_tmp);
         (void)hipFree(vel_tmp);
         k.pos[1] = nullptr;
@@ -651,6 +657,7 @@ int fsa_fill_cell(fsa_sim *s) {
     // current order only until the next sort
     if (!s->hashed) return set_err(FSA_ERR_STATE, "fill_cell needs particle_cell_update first");
     s->hashed = false;
+    s->couple_fresh = false;  // the coupling records describe the old order
     for (KindBuffers *k : {&s->fluid, &s->solid}) {
         if (!k->n) continue;
         HIP_TRY(launch_sort_one(make_cell_job(s->P, *k, s->jumps, false), s->stream));
@@ -668,6 +675,7 @@ int fsa_couple_calculation(fsa_sim *s) {
     if (s->P.nf == 0 || s->P.ns == 0) return FSA_OK;
     if ((rc = stage_open(s, FSA_STAGE_COUPLE))) return rc;
     HIP_TRY(launch_couple(s->P, solid_args(s), s->stream));
+    s->couple_fresh = true;
     stage_end(s, FSA_STAGE_COUPLE);
     return FSA_OK;
 }
@@ -706,9 +714,12 @@ int fsa_solid_calculation(fsa_sim *s) {
     if (rc) return rc;
     if (s->P.ns == 0) return FSA_OK;
     if ((rc = stage_open(s, FSA_STAGE_SOLID))) return rc;
-    const SolidArgs a = solid_args(s);
+    SolidArgs a = solid_args(s);
+    const bool walk = (s->debug & FSA_DEBUG_WALK) != 0;
+    a.fluid_rec = s->couple_fresh && !walk ? 1 : 0;
+    a.solid_rec = walk ? 0 : 1;
     for (int sub = 0; sub < s->P.sub; ++sub) {
-        HIP_TRY(launch_solid_deformation(s->P, a, sub == 0, s->stream));
+        HIP_TRY(launch_solid_deformation(s->P, a, sub == 0 || walk, s->stream));
         HIP_TRY(launch_solid_force(s->P, a, s->stream));
         HIP_TRY(launch_solid_update(s->P, a, s->stream));
     }

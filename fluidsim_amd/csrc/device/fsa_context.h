@@ -51,6 +51,9 @@ struct fsa_sim {
 
     bool uploaded[3] = {false, false, false};
     bool hashed = false;  // cell ids/ranks of the current order are fresh
+    // the coupling pass ran on the current order, so the solid force pass may
+    // replay its fluid-partner records
+    bool couple_fresh = false;
     unsigned debug = 0;
     // unique per context over the process lifetime: a batch keys its recorded
     // launches on it (a new context may reuse a freed one's address)

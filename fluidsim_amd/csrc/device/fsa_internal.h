@@ -64,6 +64,11 @@ struct DevParams {
     float eps_vis;    // 0.01 h^2
     float visc10;     // 10 * mu
     float max_speed;  // 0.3 c0
+    // fluid partners of the solid force pass are recorded by the coupling
+    // pass within R_rec = R + d of the step-start solid position; they cover
+    // every fluid partner of a solid that has moved at most d since
+    float R2_rec;     // R_rec^2 (with a rounding margin)
+    float disp2_max;  // d^2
 };
 
 DevParams make_dev_params(const fsa_params &p);
@@ -97,6 +102,11 @@ struct SolidExtra {
     // their number (uncapped) at ncnt[i * kSolidLanes + l]
     int32_t *nbr = nullptr;
     int32_t *ncnt = nullptr;
+    // fluid candidates within R_rec of the step-start position x0, recorded
+    // per lane by the coupling pass in the same layout
+    int32_t *fnbr = nullptr;
+    int32_t *fcnt = nullptr;
+    float4 *x0 = nullptr;
 };
 
 struct VirtExtra {
@@ -195,6 +205,8 @@ struct SolidArgs {
     const int32_t *fstart, *sstart;
     const float4 *fpos, *fvel;
     SolidExtra x;  // device pointers, passed by value
+    int fluid_rec;  // the coupling pass of this step (this order) recorded the fluid partners
+    int solid_rec;  // later passes replay the solid partners the first deformation pass recorded
 };
 struct SolidJob {
     DevParams P;

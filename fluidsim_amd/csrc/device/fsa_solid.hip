@@ -110,12 +110,19 @@ __device__ __forceinline__ void couple_body(const DevParams &P, const SolidArgs 
     const float4 xi4 = a.spos[L.i];
     const float xi = xi4.x, yi = xi4.y, zi = xi4.z;
     float nx = 0.0f, ny = 0.0f, nz = 0.0f, pressure = 0.0f, wsum = 0.0f;
+    // fluid candidates within R_rec, recorded for the force passes of the step
+    int32_t *rec = a.x.fnbr + (size_t)L.i * (kSolidListCap * kSub) + L.sub;
+    int nrec = 0;
     for_each_column(P, a.scell[L.i], [&](int c0, int c1) {
         // Shepard pressure from fluid (solidPressureCalculate, solid.cu:275-291)
         for (int j = a.fstart[c0] + L.sub, e = a.fstart[c1]; j < e; j += kSub) {
             const float4 pj = a.fpos[j];
             const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
             const float d2 = len2(dx, dy, dz);
+            if (d2 < P.R2_rec) {
+                if (nrec < kSolidListCap) rec[nrec * kSub] = j;
+                ++nrec;
+            }
             if (d2 < P.R2) {
                 const float w = (expf(-(d2 * P.h_inv2)) - P.c1) * P.c2;
                 wsum += w;
@@ -140,7 +147,9 @@ __device__ __forceinline__ void couple_body(const DevParams &P, const SolidArgs 
     nz = group_sum(nz);
     pressure = group_sum(pressure);
     wsum = group_sum(wsum);
+    a.x.fcnt[(size_t)L.i * kSub + L.sub] = nrec;
     if (L.sub != 0) return;
+    a.x.x0[L.i] = xi4;
     const float n2 = len2(nx, ny, nz);
     if (n2 < kEpsForSum) {
         nx = ny = nz = 0.0f;
@@ -163,7 +172,9 @@ __device__ __forceinline__ void couple_body(const DevParams &P, const SolidArgs 
 // records each lane's partners (kWalkRecord), later passes replay the record
 // (kReplay).  A lane with more than kSolidListCap partners walks again, so
 // each lane visits the same partners in the same order on either path and
-// the sums have the same bits.
+// the sums have the same bits.  (Gathering 2 or 4 recorded partners per
+// round was measured slower: the extra registers cost more occupancy than
+// the overlapped loads gained.)
 enum PartnerMode { kWalkRecord = 0, kReplay = 1 };
 
 template <int MODE, class F>
@@ -171,7 +182,7 @@ __device__ __forceinline__ void solid_partners(const DevParams &P, const SolidAr
                                                const float4 &Xi, F &&body) {
     int32_t *rec = a.x.nbr + (size_t)i * (kSolidListCap * kSub) + sub;
     int32_t *rec_n = a.x.ncnt + (size_t)i * kSub + sub;
-    if (MODE == kReplay) {
+    if (MODE == kReplay && a.solid_rec) {
         const int cnt = *rec_n;
         if (cnt <= kSolidListCap) {
             for (int k = 0; k < cnt; ++k) {
@@ -276,36 +287,59 @@ __device__ __forceinline__ void force_body(const DevParams &P, const SolidArgs &
     float cx = 0.0f, cy = 0.0f, cz = 0.0f;  // couple_dudt
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     const float eps = 0.01f * P.h2;
-    for_each_column(P, a.scell[i], [&](int c0, int c1) {
-        // fluid pressure on the solid (solidFluidCouple, solid.cu:81-117):
-        // the solid side takes the fluid velocity mirrored about its normal
-        for (int j = a.fstart[c0] + L.sub, e = a.fstart[c1]; j < e; j += kSub) {
+    // fluid pressure on the solid (solidFluidCouple, solid.cu:81-117): the
+    // solid side takes the fluid velocity mirrored about its normal.  Lane
+    // `sub` takes the fluid candidates sub, sub + kSub, ... of every column
+    // run.  The coupling pass of the step recorded each lane's candidates
+    // within R_rec of the step-start position x0; while the solid is within d
+    // of x0 they hold every candidate within R, in walk order, so replaying
+    // them gives the walk's sums bit for bit.  Otherwise the lane walks.
+    const auto fluid_term = [&](const float4 &pj, const float4 &uj, float dx, float dy, float dz, float d2) {
+        const float m = 2.0f * ((uj.x - ui.x) * np.x + (uj.y - ui.y) * np.y + (uj.z - ui.z) * np.z);
+        const float uix = uj.x - m * np.x, uiy = uj.y - m * np.y, uiz = uj.z - m * np.z;
+        const float kc = expf(-(d2 * P.h_inv2)) * P.kd;
+        const float rhoj = pj.w, rhoi = rhoj;
+        float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+        if (sqrtf(d2) != 0.0f) {
+            const float r = 1.0f / sqrtf(d2);
+            ex = dx * r;
+            ey = dy * r;
+            ez = dz * r;
+        }
+        float ps = (rhoi * uj.w + rhoj * np.w) / (rhoi + rhoj);
+        const float du = (uix - uj.x) * ex + (uiy - uj.y) * ey + (uiz - uj.z) * ez;
+        if (du >= 0.0f) ps += rhoi * rhoj * du / (rhoi + rhoj) * P.cs;
+        const float s = -2.0f * ps / P.rho_s;
+        cx += (s * dx) * kc;
+        cy += (s * dy) * kc;
+        cz += (s * dz) * kc;
+    };
+    bool replay = false;
+    int nrec = 0;
+    const int32_t *rec = a.x.fnbr + (size_t)i * (kSolidListCap * kSub) + L.sub;
+    if (a.fluid_rec) {
+        const float4 x0 = a.x.x0[i];
+        nrec = a.x.fcnt[(size_t)i * kSub + L.sub];
+        replay = nrec <= kSolidListCap && len2(xi.x - x0.x, xi.y - x0.y, xi.z - x0.z) <= P.disp2_max;
+    }
+    if (replay) {
+        for (int k = 0; k < nrec; ++k) {
+            const int j = rec[k * kSub];
             const float4 pj = a.fpos[j];
             const float dx = pj.x - xi.x, dy = pj.y - xi.y, dz = pj.z - xi.z;
             const float d2 = len2(dx, dy, dz);
-            if (d2 < P.R2) {
-                const float4 uj = a.fvel[j];
-                const float m = 2.0f * ((uj.x - ui.x) * np.x + (uj.y - ui.y) * np.y + (uj.z - ui.z) * np.z);
-                const float uix = uj.x - m * np.x, uiy = uj.y - m * np.y, uiz = uj.z - m * np.z;
-                const float kc = expf(-(d2 * P.h_inv2)) * P.kd;
-                const float rhoj = pj.w, rhoi = rhoj;
-                float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-                if (sqrtf(d2) != 0.0f) {
-                    const float r = 1.0f / sqrtf(d2);
-                    ex = dx * r;
-                    ey = dy * r;
-                    ez = dz * r;
-                }
-                float ps = (rhoi * uj.w + rhoj * np.w) / (rhoi + rhoj);
-                const float du = (uix - uj.x) * ex + (uiy - uj.y) * ey + (uiz - uj.z) * ez;
-                if (du >= 0.0f) ps += rhoi * rhoj * du / (rhoi + rhoj) * P.cs;
-                const float s = -2.0f * ps / P.rho_s;
-                cx += (s * dx) * kc;
-                cy += (s * dy) * kc;
-                cz += (s * dz) * kc;
-            }
+            if (d2 < P.R2) fluid_term(pj, a.fvel[j], dx, dy, dz, d2);
         }
-    });
+    } else {
+        for_each_column(P, a.scell[i], [&](int c0, int c1) {
+            for (int j = a.fstart[c0] + L.sub, e = a.fstart[c1]; j < e; j += kSub) {
+                const float4 pj = a.fpos[j];
+                const float dx = pj.x - xi.x, dy = pj.y - xi.y, dz = pj.z - xi.z;
+                const float d2 = len2(dx, dy, dz);
+                if (d2 < P.R2) fluid_term(pj, a.fvel[j], dx, dy, dz, d2);
+            }
+        });
+    }
     // stress divergence + artificial viscosity (solidSolidForce, solid.cu:119-157)
     solid_partners<kReplay>(P, a, i, L.sub, Xi, [&](int j, const float (&dX)[3], float d2) {
         const float kc = expf(-(d2 * P.h_inv2)) * P.kd;

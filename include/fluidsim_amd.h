@@ -198,6 +198,10 @@ int fsa_download_fluid_rates(fsa_sim *sim, float *drho_dt, float *du_dt);
 int fsa_get_jump_count(fsa_sim *sim, int64_t *count);
 
 #define FSA_DEBUG_PAIR_SUMS 1u
+/* The solid sub-steps walk their candidate cells in every pass instead of
+ * replaying the partners recorded earlier in the step (same sums, bit for
+ * bit; the records only save the walk). */
+#define FSA_DEBUG_WALK 2u
 int fsa_set_debug(fsa_sim *sim, unsigned flags);
 
 /* ---- run-level observables ----------------------------------------------

@@ -121,6 +121,34 @@ def test_solid_partner_records(gpu_available, h_ratio):
         assert ok, (st, "du_dt", e_g, e_32)
 
 
+def test_replayed_partners_give_the_walk_bit_for_bit(gpu_available):
+    """Replaying the recorded solid and fluid partners of a step gives the
+    same bits as walking the candidate cells in every solid pass
+    (FSA_DEBUG_WALK), over steps in which the fluid surrounds the gate."""
+    from fluidsim_amd._native import DEBUG_WALK
+    sc = scenes.tiny_scene(seed=9, fluid=(16, 10, 6), gate=(4, 8, 6), gate_x0=9, solid_sub_step=4)
+    a = sc.arrays()
+    p = sc.params((len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])))
+    print("case walk vs replay", p.n_fluid, p.n_solid, p.n_virt, flush=True)
+    devs = []
+    for flags in (0, DEBUG_WALK):
+        d = DeviceSim(p)
+        d.upload(a)
+        d.set_debug(flags)
+        devs.append(d)
+    for st in range(4):
+        for d in devs:
+            d.step(2)
+        s0, s1 = devs[0].solid(), devs[1].solid()
+        for key in ("position", "velocity", "du_dt", "couple_dudt", "deformation"):
+            np.testing.assert_array_equal(s0[key], s1[key], err_msg=f"{key} after {2 * (st + 1)} steps")
+        f0, f1 = devs[0].fluid(), devs[1].fluid()
+        np.testing.assert_array_equal(f0["velocity"], f1["velocity"])
+    assert np.abs(devs[0].solid()["couple_dudt"]).max() > 0, "the fluid must press on the gate"
+    for d in devs:
+        d.close()
+
+
 @pytest.mark.parametrize("steps", [1, 20])
 def test_multi_step_state(gpu_available, tiny, steps):
     a, p, dev, o = _setup(tiny)
