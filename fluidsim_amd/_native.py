@@ -26,6 +26,8 @@ FLUID, SOLID, VIRT = 0, 1, 2
 UNCONSTRAINED_SOLID, FIXED_SOLID = 0, 1
 DEBUG_PAIR_SUMS = 1
 DEBUG_WALK = 2
+FORM_PACKED = 1
+FORM_COMPENSATED = 2
 
 STAGES = ("cell", "couple", "virt", "fluid", "solid")
 
@@ -77,6 +79,7 @@ class FsaParams(C.Structure):
         ("kernel_coefficient_2", C.c_float),
         ("kernel_differential", C.c_float),
         ("thread_num", C.c_int32),
+        ("precision_form", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -99,6 +102,10 @@ class FsaObservables(C.Structure):
         ("solid_momentum", C.c_double * 3),
         ("gate_tip_displacement", C.c_double * 3),
         ("gate_tip_count", C.c_int64),
+        ("max_fluid_speed", C.c_double),
+        ("max_solid_speed", C.c_double),
+        ("cfl_dt", C.c_double),
+        ("solid_cfl_dt", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -192,10 +199,17 @@ def _bind(path: Path, api: dict):
     return lib
 
 
+ABI_VERSION = 2  # FSA_ABI_VERSION of the header these mirrors follow
+
+
 def device_lib():
     global _device
     if _device is None:
-        _device = _bind(DEVICE_LIB, DEVICE_API)
+        lib = _bind(DEVICE_LIB, DEVICE_API)
+        v = lib.fsa_abi_version()
+        if v != ABI_VERSION:
+            raise RuntimeError(f"{DEVICE_LIB.name} has ABI version {v}, these bindings need {ABI_VERSION}; rebuild it")
+        _device = lib
     return _device
 
 

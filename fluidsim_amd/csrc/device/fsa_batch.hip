@@ -110,6 +110,7 @@ struct fsa_batch {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     bool planned = false;
     int fluid_threads = 0;  // workgroup size of the recorded fluid launches
+    bool packed = false;    // the scenes use the packed precision form (all or none)
 
     int64_t plans_built = 0;
     int64_t steps_run = 0;
@@ -238,7 +239,7 @@ hipError_t enqueue_step(fsa_batch *b, const StepTables &t) {
             break;
         case Launch::kFluid:
             e = launch_many_fluid(reinterpret_cast<const FluidJob *>(base + t.fluid_off), prefix, l.njobs,
-                                  l.nblocks, b->fluid_threads, b->stream);
+                                  l.nblocks, b->fluid_threads, b->packed, b->stream);
             break;
         case Launch::kSolid:
             e = launch_many_solid(l.op, reinterpret_cast<const SolidJob *>(base + t.solid_off), prefix, l.njobs,
@@ -289,11 +290,11 @@ int tune_fluid(fsa_batch *b, const std::vector<fsa_sim *> &sims) {
     for (int c = 0; c < 3; ++c) {
         const int *dp = reinterpret_cast<const int *>(dev + poff[c]);
         const int t = kChoices[c];
-        if ((e = launch_many_fluid(dj, dp, n, nblk[c], t, b->stream)) != hipSuccess ||  // warm-up
+        if ((e = launch_many_fluid(dj, dp, n, nblk[c], t, b->packed, b->stream)) != hipSuccess ||  // warm-up
             (e = hipEventRecord(t0, b->stream)) != hipSuccess)
             return fail(e);
         for (int r = 0; r < 2; ++r)
-            if ((e = launch_many_fluid(dj, dp, n, nblk[c], t, b->stream)) != hipSuccess) return fail(e);
+            if ((e = launch_many_fluid(dj, dp, n, nblk[c], t, b->packed, b->stream)) != hipSuccess) return fail(e);
         float ms = 0.0f;
         if ((e = hipEventRecord(t1, b->stream)) != hipSuccess || (e = hipEventSynchronize(t1)) != hipSuccess ||
             (e = hipEventElapsedTime(&ms, t0, t1)) != hipSuccess)
@@ -313,6 +314,11 @@ int build_plan(fsa_batch *b, const std::vector<fsa_sim *> &sims) {
     drop_graphs(b);
     b->planned = false;
     const int n = (int)sims.size();
+    // one fluid launch serves every scene: the packed form is a kernel variant
+    b->packed = (sims[0]->P.form & FSA_FORM_PACKED) != 0;
+    for (const fsa_sim *s : sims)
+        if (((s->P.form & FSA_FORM_PACKED) != 0) != b->packed)
+            return set_err(FSA_ERR_INVALID, "the scenes of one batch must all use the packed form or none");
     if (b->flags & FSA_BATCH_AUTOTUNE) {
         const int t = tune_fluid(b, sims);
         if (t == 0) return FSA_ERR_HIP;

@@ -80,6 +80,9 @@ DevParams make_dev_params(const fsa_params &p) {
         d.disp2_max = (float)(disp * disp);
         d.R2_rec = (float)((R + disp) * (R + disp) * (1.0 + 1.0e-5));
     }
+    d.form = p.precision_form;
+    for (int a = 0; a < 3; ++a) d.ival[a] = p.interval[a];
+    d.B_inv = p.fluid_coefficient_rho2p != 0.0f ? 1.0f / p.fluid_coefficient_rho2p : 0.0f;
     return d;
 }
 }  // namespace fsa
@@ -95,6 +98,13 @@ int check_params(const fsa_params &p) {
     if (total >= (int64_t)INT32_MAX - 2 * kScanTile) return set_err(FSA_ERR_INVALID, "cell grid too large for int32 ids");
     if (!(p.dt > 0.0f)) return set_err(FSA_ERR_INVALID, "dt must be positive");
     if (p.solid_sub_step < 1) return set_err(FSA_ERR_INVALID, "solid_sub_step must be >= 1");
+    if (p.precision_form < 0 || p.precision_form > (FSA_FORM_PACKED | FSA_FORM_COMPENSATED))
+        return set_err(FSA_ERR_INVALID, "precision_form must be a combination of FSA_FORM_PACKED and FSA_FORM_COMPENSATED");
+    // a packed partner's cell is decoded from the cell-id difference, which is
+    // unambiguous with 3 or more cells per axis
+    if ((p.precision_form & FSA_FORM_PACKED) &&
+        (p.cell_number[0] < 3 || p.cell_number[1] < 3 || p.cell_number[2] < 3))
+        return set_err(FSA_ERR_INVALID, "the packed form needs at least 3 cells along every axis");
     if (!(p.impact_distance_square > 0.0f)) return set_err(FSA_ERR_INVALID, "impact_distance_square must be positive");
     // the 27-cell search needs cells at least one cut-off wide
     for (int a = 0; a < 3; ++a) {
@@ -105,7 +115,8 @@ int check_params(const fsa_params &p) {
     return FSA_OK;
 }
 
-int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_state) {
+int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_state, bool compensated = false,
+               bool packed = false) {
     k.n = n;
     const int ncell = s->P.ncell;
     const size_t padded = (size_t)(ncell + kScanTile - 1) / kScanTile * kScanTile;
@@ -118,11 +129,20 @@ int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_sta
             if ((rc = s->alloc(&k.pos[b], (size_t)n + kPosPad))) return rc;
             if (!s->dry) HIP_TRY(hipMemsetAsync(k.pos[b], 0, sizeof(float4) * ((size_t)n + kPosPad), s->stream));
             if ((rc = s->alloc(&k.vel[b], n))) return rc;
+            if (compensated) {
+                if ((rc = s->alloc(&k.cpos[b], n)) || (rc = s->alloc(&k.cvel[b], n))) return rc;
+                // the upload's state carries no rounding error yet
+                if (!s->dry) {
+                    HIP_TRY(hipMemsetAsync(k.cpos[b], 0, sizeof(float4) * (size_t)n, s->stream));
+                    HIP_TRY(hipMemsetAsync(k.cvel[b], 0, sizeof(float4) * (size_t)n, s->stream));
+                }
+            }
         }
         if ((rc = s->alloc(&k.id[b], n))) return rc;
         if ((rc = s->alloc(&k.cell[b], n))) return rc;
         if (with_ref && (rc = s->alloc(&k.ref[b], n))) return rc;
     }
+    if (packed && (rc = s->alloc(&k.pk, n))) return rc;
     if ((rc = s->alloc(&k.key, n))) return rc;
     if ((rc = s->alloc(&k.rank, n))) return rc;
     if ((rc = s->alloc(&k.tmp_slot, n))) return rc;
@@ -134,8 +154,9 @@ int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_sta
 // every device buffer of a context (dry: only count the bytes)
 int alloc_all(fsa_sim *s) {
     int rc;
-    if ((rc = alloc_kind(s, s->fluid, s->P.nf, false, true))) return rc;
-    if ((rc = alloc_kind(s, s->solid, s->P.ns, true, true))) return rc;
+    const bool comp = (s->P.form & FSA_FORM_COMPENSATED) != 0, packed = (s->P.form & FSA_FORM_PACKED) != 0;
+    if ((rc = alloc_kind(s, s->fluid, s->P.nf, false, true, comp, packed))) return rc;
+    if ((rc = alloc_kind(s, s->solid, s->P.ns, true, true, comp, false))) return rc;
     if ((rc = alloc_kind(s, s->virt, s->P.nv, false, false))) return rc;
     const int ns = s->P.ns;
     if ((rc = s->alloc(&s->sx.np, ns)) || (rc = s->alloc(&s->sx.acc, ns)) || (rc = s->alloc(&s->sx.couple, ns)) ||
@@ -175,6 +196,14 @@ int check_in_grid(const fsa_sim *s, const float *x, int n, const char *what) {
                 return set_err(FSA_ERR_INVALID, std::string(what) + " particle " + std::to_string(i) +
                                                     " lies outside the cell grid");
         }
+    return FSA_OK;
+}
+
+// zero the compensation terms of buffer 0 (compensated form)
+int clear_compensation(fsa_sim *s, KindBuffers &k) {
+    if (!k.cpos[0] || k.n == 0) return FSA_OK;
+    HIP_TRY(hipMemsetAsync(k.cpos[0], 0, sizeof(float4) * (size_t)k.n, s->stream));
+    HIP_TRY(hipMemsetAsync(k.cvel[0], 0, sizeof(float4) * (size_t)k.n, s->stream));
     return FSA_OK;
 }
 
@@ -257,6 +286,8 @@ SolidArgs fsa::solid_args(const fsa_sim *s, const KindBuffers &f, const KindBuff
     SolidArgs a{};
     a.spos = so.pos[so.st];
     a.svel = so.vel[so.st];
+    a.scpos = so.cpos[so.st];
+    a.scvel = so.cvel[so.st];
     a.sref = so.ref[so.ord];
     a.scell = so.cell[so.ord];
     a.fstart = f.start;
@@ -286,6 +317,11 @@ FluidStepArgs fsa::fluid_args(const fsa_sim *s, const KindBuffers &f, const Kind
     FluidStepArgs a{};
     a.fpos = f.pos[f.st];
     a.fvel = f.vel[f.st];
+    a.fcpos = f.cpos[f.st];
+    a.fcvel = f.cvel[f.st];
+    a.fcpos_out = f.cpos[1 - f.st];
+    a.fcvel_out = f.cvel[1 - f.st];
+    a.fpk = f.pk;
     a.fcell = f.cell[f.ord];
     a.fstart = f.start;
     a.sstart = so.start;
@@ -425,6 +461,7 @@ int fsa_upload_fluid(fsa_sim *s, const float *x, const float *u, const float *rh
         HIP_TRY(hipMemcpyAsync(k.pos[0], pos.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
         HIP_TRY(hipMemcpyAsync(k.vel[0], vel.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
         HIP_TRY(hipMemcpyAsync(k.id[0], id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s->stream));
+        if ((rc = clear_compensation(s, k))) return rc;  // an uploaded state carries no rounding error
         if ((rc = build_cells(s, k, false))) return rc;
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -458,6 +495,7 @@ int fsa_upload_solid(fsa_sim *s, const float *x, const float *u, const float *X,
         HIP_TRY(hipMemcpyAsync(k.vel[0], vel.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
         HIP_TRY(hipMemcpyAsync(k.ref[0], ref.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
         HIP_TRY(hipMemcpyAsync(k.id[0], id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s->stream));
+        if ((rc = clear_compensation(s, k))) return rc;  // an uploaded state carries no rounding error
         if ((rc = build_cells(s, k, false))) return rc;
         // Solid_Host_List::addList defaults (include/host_data.cuh:92-99):
         // F = I, P = 0, L = I, n = 0, p = 0, du_dt = 0
@@ -881,6 +919,12 @@ int fsa_observe(fsa_sim *s, fsa_observables *out) {
     out->front_x = s->P.nf > 0 ? v[6] : 0.0;
     out->gate_tip_count = (int64_t)v[13];
     for (int a = 0; a < 3; ++a) out->gate_tip_displacement[a] = v[13] > 0 ? v[10 + a] / v[13] : 0.0;
+    // CFL bounds from the FP64 speed maxima
+    out->max_fluid_speed = std::sqrt(v[14]);
+    out->max_solid_speed = std::sqrt(v[15]);
+    const double h = s->params.smoothing_length;
+    out->cfl_dt = 0.25 * h / ((double)s->params.fluid_sound_speed + out->max_fluid_speed);
+    out->solid_cfl_dt = s->P.ns > 0 ? 0.3 * h / ((double)s->params.solid_sound_speed + out->max_solid_speed) : 0.0;
     return FSA_OK;
 }
 

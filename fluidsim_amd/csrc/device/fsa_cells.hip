@@ -174,9 +174,15 @@ struct OpGatherImpl {
         int r = 0;
         for (int u = b; u < e; ++u) r += j.tmp_id[u] < my ? 1 : 0;
         const int d = b + r;
-        j.pos_out[d] = j.pos_in[s];
-        j.vel_out[d] = j.vel_in[s];
+        const float4 x = j.pos_in[s], u = j.vel_in[s];
+        j.pos_out[d] = x;
+        j.vel_out[d] = u;
         if (j.has_ref) j.ref_out[d] = j.ref_in[s];
+        if (j.cpos_in) {
+            j.cpos_out[d] = j.cpos_in[s];
+            j.cvel_out[d] = j.cvel_in[s];
+        }
+        if (j.pk_out) j.pk_out[d] = pack_partner(j.P, x, u, c);
         j.id_out[d] = my;
         j.cell_out[d] = c;
     }
@@ -231,6 +237,11 @@ CellJob make_cell_job(const DevParams &P, const KindBuffers &k, unsigned long lo
     const int so = k.st, sn = 1 - k.st, oo = k.ord, on = 1 - k.ord;
     j.pos_in = k.pos[so];
     j.vel_in = k.vel[so];
+    j.cpos_in = k.cpos[so];
+    j.cvel_in = k.cvel[so];
+    j.cpos_out = k.cpos[sn];
+    j.cvel_out = k.cvel[sn];
+    j.pk_out = k.pk;
     j.ref_in = k.ref[oo];
     j.id_in = k.id[oo];
     j.prev_cell = k.cell[oo];

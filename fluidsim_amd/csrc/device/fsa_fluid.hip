@@ -44,6 +44,9 @@ enum PairKind { kFluidFluid = 0, kFluidSolid = 1, kFluidVirt = 2 };
 struct Self {
     float x, y, z, rho, ux, uy, uz, p;
     float inv_rho, two_rho;
+    // packed form: own slot and cell, position relative to the cell centre in cells
+    int i, cell;
+    float fx, fy, fz;
 };
 
 struct Acc {
@@ -101,6 +104,30 @@ __device__ __forceinline__ void one_pair(const DevParams &P, const Self &me, con
     }
 }
 
+// A fluid partner from its FP16 record (FSA_FORM_PACKED).  The partner lies
+// in one of the 27 cells around the reader's, so the cell-id difference
+// (dix nyz + diy nz + diz, each d in {-1, 0, 1}) decodes by comparisons once
+// shifted to nonnegative digits (unambiguous with >= 3 cells per axis).  The
+// offset between the two particles is then (d + r_j - f_i) cells.  The
+// reader's own slot gives dx = 0 exactly, as in the FP32 form.
+__device__ __forceinline__ void one_pair_packed(const DevParams &P, const Self &me, const uint4 &rec, int cj, int j,
+                                                float w, Acc &acc) {
+    const int nz = P.cn[2], nyz = P.cn[1] * nz;
+    int t = cj - me.cell + nyz + nz + 1;
+    const int ax = (t >= nyz ? 1 : 0) + (t >= 2 * nyz ? 1 : 0);
+    t -= ax * nyz;
+    const int ay = (t >= nz ? 1 : 0) + (t >= 2 * nz ? 1 : 0);
+    t -= ay * nz;
+    const float2 rxy = unpack_h2(rec.x), rzd = unpack_h2(rec.y), uxy = unpack_h2(rec.z), uzp = unpack_h2(rec.w);
+    float dx = ((float)(ax - 1) + (rxy.x - me.fx)) * P.ival[0];
+    float dy = ((float)(ay - 1) + (rxy.y - me.fy)) * P.ival[1];
+    float dz = ((float)(t - 1) + (rzd.x - me.fz)) * P.ival[2];
+    if (j == me.i) dx = dy = dz = 0.0f;
+    const float d2 = len2(dx, dy, dz);
+    const float rhoj = P.rho0 + P.rho0 * rzd.y;
+    pair_term<kFluidFluid>(P, me, dx, dy, dz, d2, uxy.x, uxy.y, uzp.x, rhoj, uzp.y * P.rho2p, w, acc);
+}
+
 // workgroup sizes of the fluid pass (one to four waves; LDS per wave bounds
 // the occupancy, so the best size depends on the scene: fsa_autotune_fluid)
 constexpr int kThreadChoices[] = {64, 128, 256};
@@ -138,15 +165,55 @@ __device__ __forceinline__ void gather(const float4 *__restrict__ pos, const flo
     }
 }
 
+// FP16 records and cells of list entries k .. k+kPairBatch-1 (packed form)
+template <int T>
+__device__ __forceinline__ void gather_packed(const uint4 *__restrict__ pk, const int32_t *__restrict__ pcell,
+                                              const Lists<T> &L, int k, int cnt, uint4 *rq, int *cq, int *jq) {
+#pragma unroll
+    for (int q = 0; q < kPairBatch; ++q) {
+        const int v = L.list[min(k + q, cnt - 1) * T];
+        const int j = L.base[(v >> kRunBits) * T] + (v & (kRunMax - 1));
+        rq[q] = pk[j];
+        cq[q] = pcell[j];
+        jq[q] = j;
+    }
+}
+
+// Packed partner records of the kind's sources (null in the FP32 form)
+struct PackedSrc {
+    const uint4 *pk;
+    const int32_t *cell;
+};
+
 // pair terms of list entries [0, cnt), in list order.  The partner state of
 // the next kPairBatch entries is gathered while the current ones are
 // evaluated; all kPairBatch slots of a round are evaluated, the ones past
-// cnt with weight 0.
-template <int KIND, int T>
+// cnt with weight 0.  PACK: fluid partners come from their FP16 records.
+template <int KIND, int T, bool PACK>
 __device__ __forceinline__ void evaluate(const DevParams &P, const Self &me, const float4 *__restrict__ pos,
                                          const float4 *__restrict__ vel, const float4 *__restrict__ np,
-                                         const Lists<T> &L, int cnt, Acc &acc) {
+                                         const PackedSrc &ps, const Lists<T> &L, int cnt, Acc &acc) {
     if (__ballot(cnt > 0) == 0) return;
+    if constexpr (PACK && KIND == kFluidFluid) {
+        uint4 rq[kPairBatch];
+        int cq[kPairBatch], jq[kPairBatch];
+        if (cnt > 0) gather_packed<T>(ps.pk, ps.cell, L, 0, cnt, rq, cq, jq);
+        for (int k = 0; k < cnt; k += kPairBatch) {
+            uint4 rn[kPairBatch];
+            int cn[kPairBatch], jn[kPairBatch];
+            gather_packed<T>(ps.pk, ps.cell, L, k + kPairBatch, cnt, rn, cn, jn);
+#pragma unroll
+            for (int q = 0; q < kPairBatch; ++q)
+                one_pair_packed(P, me, rq[q], cq[q], jq[q], k + q < cnt ? 1.0f : 0.0f, acc);
+#pragma unroll
+            for (int q = 0; q < kPairBatch; ++q) {
+                rq[q] = rn[q];
+                cq[q] = cn[q];
+                jq[q] = jn[q];
+            }
+        }
+        return;
+    }
     float4 pj[kPairBatch], vj[kPairBatch], nj[kPairBatch];
     if (cnt > 0) gather<KIND, T>(pos, vel, np, L, 0, cnt, pj, vj, nj);  // an empty list holds no valid slot
     for (int k = 0; k < cnt; k += kPairBatch) {
@@ -171,17 +238,18 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const Self &me, con
 // end, masked out), and a full list or a full spare table is evaluated by
 // all lanes together.  The next batch of candidates is loaded while the
 // current one is tested.
-template <int KIND, int T>
+template <int KIND, int T, bool PACK>
 __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col, int b, int e,
                                      const float4 *__restrict__ pos, const float4 *__restrict__ vel,
-                                     const float4 *__restrict__ np, const Lists<T> &L, int &cnt, int &spare, Acc &acc) {
+                                     const float4 *__restrict__ np, const PackedSrc &ps, const Lists<T> &L, int &cnt,
+                                     int &spare, Acc &acc) {
     int sb = b;
     for (int sub = 0; __ballot(sb < e) != 0; ++sub, sb += kRunMax) {
         const bool has = sb < e;
         int slot = col;
         if (sub > 0) {
             if (__ballot(has && spare == kBaseSlots - kColumns) != 0) {
-                evaluate<KIND, T>(P, me, pos, vel, np, L, cnt, acc);
+                evaluate<KIND, T, PACK>(P, me, pos, vel, np, ps, L, cnt, acc);
                 cnt = 0;
                 spare = 0;
             }
@@ -221,7 +289,7 @@ __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col
             // at most kScanBatch entries are added per round; the bases stay
             // valid across an evaluation
             if (__ballot(cnt > kListCap - kScanBatch) != 0) {
-                evaluate<KIND, T>(P, me, pos, vel, np, L, cnt, acc);
+                evaluate<KIND, T, PACK>(P, me, pos, vel, np, ps, L, cnt, acc);
                 cnt = 0;
             }
         }
@@ -255,22 +323,23 @@ struct Columns {
 };
 
 // all partners of one kind: the 9 columns in order, then one evaluation
-template <int KIND, int T>
+template <int KIND, int T, bool PACK>
 __device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cell, const int32_t *__restrict__ start,
                                      const float4 *__restrict__ pos, const float4 *__restrict__ vel,
-                                     const float4 *__restrict__ np, const Lists<T> &L, Acc &acc) {
+                                     const float4 *__restrict__ np, const PackedSrc &ps, const Lists<T> &L,
+                                     Acc &acc) {
     const Columns cols(P, cell);
     int cnt = 0, spare = 0;
 #pragma unroll 1
     for (int col = 0; col < kColumns; ++col) {
         const int b = start[cols.c0(col)], e = start[cols.c1(col)];
         L.base[col * T] = b;
-        scan<KIND, T>(P, me, col, b, e, pos, vel, np, L, cnt, spare, acc);
+        scan<KIND, T, PACK>(P, me, col, b, e, pos, vel, np, ps, L, cnt, spare, acc);
     }
-    evaluate<KIND, T>(P, me, pos, vel, np, L, cnt, acc);
+    evaluate<KIND, T, PACK>(P, me, pos, vel, np, ps, L, cnt, acc);
 }
 
-template <bool RATES, int T>
+template <bool RATES, int T, bool PACK>
 __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepArgs &a, int blk, int nblk) {
     __shared__ uint16_t s_list[kListCap * T];
     __shared__ int s_base[kBaseSlots * T];
@@ -292,12 +361,22 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     me.inv_rho = 1.0f / me.rho;
     me.two_rho = 2.0f * me.rho;
     Acc acc{0.0f, 0.0f, 0.0f, 0.0f};
+    const int cell = a.fcell[i];
+    me.i = i;
+    me.cell = cell;
+    me.fx = me.fy = me.fz = 0.0f;
+    if (PACK) {
+        const int3 ci = cell_coords(P, cell);
+        me.fx = cell_offset(P, me.x, 0, ci.x);
+        me.fy = cell_offset(P, me.y, 1, ci.y);
+        me.fz = cell_offset(P, me.z, 2, ci.z);
+    }
 
     // partners kind by kind: fluid, solid, wall (the canonical order)
-    const int cell = a.fcell[i];
-    pass<kFluidFluid, T>(P, me, cell, a.fstart, a.fpos, a.fvel, nullptr, list, acc);
-    pass<kFluidSolid, T>(P, me, cell, a.sstart, a.spos, a.svel, a.snp, list, acc);
-    pass<kFluidVirt, T>(P, me, cell, a.vstart, a.vpos, a.vvel, nullptr, list, acc);
+    const PackedSrc fsrc{a.fpk, a.fcell}, none{nullptr, nullptr};
+    pass<kFluidFluid, T, PACK>(P, me, cell, a.fstart, a.fpos, a.fvel, nullptr, fsrc, list, acc);
+    pass<kFluidSolid, T, false>(P, me, cell, a.sstart, a.spos, a.svel, a.snp, none, list, acc);
+    pass<kFluidVirt, T, false>(P, me, cell, a.vstart, a.vpos, a.vvel, nullptr, none, list, acc);
 
     // fluidUpdate (fluid.cu:234-269): NaN scrub, explicit step with the old
     // velocity for the position, density clamp, EOS, domain clamp, speed cap
@@ -306,29 +385,60 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     if (!(ax == ax && ay == ay && az == az)) ax = ay = az = 0.0f;
     if (RATES) a.rates[i] = make_float4(ax, ay, az, drho);
     const float dt = P.dt;
-    float x = me.x + me.ux * dt, y = me.y + me.uy * dt, z = me.z + me.uz * dt;
-    float ux = me.ux + (ax + P.gx) * dt, uy = me.uy + (ay + P.gy) * dt, uz = me.uz + (az + P.gz) * dt;
+    float x, y, z, ux, uy, uz;
+    float4 cp = make_float4(0.0f, 0.0f, 0.0f, 0.0f), cv = cp;
+    if (a.fcpos) {
+        // compensated form: x and u carry their rounding error (cp, cv)
+        cp = a.fcpos[i];
+        cv = a.fcvel[i];
+        const float sx = me.ux * dt, sy = me.uy * dt, sz = me.uz * dt;
+        x = kahan_add(me.x, sx, cp.x);
+        y = kahan_add(me.y, sy, cp.y);
+        z = kahan_add(me.z, sz, cp.z);
+        const float vx = (ax + P.gx) * dt, vy = (ay + P.gy) * dt, vz = (az + P.gz) * dt;
+        ux = kahan_add(me.ux, vx, cv.x);
+        uy = kahan_add(me.uy, vy, cv.y);
+        uz = kahan_add(me.uz, vz, cv.z);
+    } else {
+        x = me.x + me.ux * dt;
+        y = me.y + me.uy * dt;
+        z = me.z + me.uz * dt;
+        ux = me.ux + (ax + P.gx) * dt;
+        uy = me.uy + (ay + P.gy) * dt;
+        uz = me.uz + (az + P.gz) * dt;
+    }
     float rho = me.rho + drho * dt;
     rho = fminf(fmaxf(rho, P.rho_min), P.rho_max);
     // Tait equation of state (densityToPressure, fluid.cu:16-24)
     const float p = fmaxf((powf(rho * P.rho0_inv, P.gamma) - 1.0f) * P.rho2p, P.p_min);
-    x = fminf(fmaxf(x, P.dmin[0]), P.dmax[0]);
-    y = fminf(fmaxf(y, P.dmin[1]), P.dmax[1]);
-    z = fminf(fmaxf(z, P.dmin[2]), P.dmax[2]);
+    // a clamped or rescaled value drops its carried error
+    const float xc = fminf(fmaxf(x, P.dmin[0]), P.dmax[0]);
+    const float yc = fminf(fmaxf(y, P.dmin[1]), P.dmax[1]);
+    const float zc = fminf(fmaxf(z, P.dmin[2]), P.dmax[2]);
+    if (xc != x) cp.x = 0.0f;
+    if (yc != y) cp.y = 0.0f;
+    if (zc != z) cp.z = 0.0f;
     const float speed = sqrtf(len2(ux, uy, uz));
     if (speed > P.max_speed) {
         const float f = P.max_speed / speed;
         ux *= f;
         uy *= f;
         uz *= f;
+        cv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    a.fpos_out[i] = make_float4(x, y, z, rho);
+    a.fpos_out[i] = make_float4(xc, yc, zc, rho);
     a.fvel_out[i] = make_float4(ux, uy, uz, p);
+    if (a.fcpos) {
+        a.fcpos_out[i] = cp;
+        a.fcvel_out[i] = cv;
+    }
 }
 
-template <bool RATES, int T>
+template <bool RATES, int T, bool PACK>
 struct OpFluid {
-    static __device__ void run(const FluidJob &j, int blk, int nblk) { fluid_body<RATES, T>(j.P, j.a, blk, nblk); }
+    static __device__ void run(const FluidJob &j, int blk, int nblk) {
+        fluid_body<RATES, T, PACK>(j.P, j.a, blk, nblk);
+    }
 };
 
 }  // namespace
@@ -342,42 +452,52 @@ bool fluid_threads_valid(int threads) {
 int blocks_fluid(const FluidJob &j, int threads) { return (int)(((long long)j.P.nf + threads - 1) / threads); }
 
 namespace {
-template <int T>
+template <int T, bool PACK>
 hipError_t one_fluid(const FluidJob &j, hipStream_t s) {
     const dim3 grid(blocks_fluid(j, T));
     if (j.a.rates)
-        hipLaunchKernelGGL((k_one<OpFluid<true, T>, FluidJob, T>), grid, dim3(T), 0, s, j);
+        hipLaunchKernelGGL((k_one<OpFluid<true, T, PACK>, FluidJob, T>), grid, dim3(T), 0, s, j);
     else
-        hipLaunchKernelGGL((k_one<OpFluid<false, T>, FluidJob, T>), grid, dim3(T), 0, s, j);
+        hipLaunchKernelGGL((k_one<OpFluid<false, T, PACK>, FluidJob, T>), grid, dim3(T), 0, s, j);
     return hipGetLastError();
 }
-template <int T>
+template <int T, bool PACK>
 hipError_t many_fluid(const FluidJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s) {
-    hipLaunchKernelGGL((k_many<OpFluid<false, T>, FluidJob, T>), dim3(nblocks), dim3(T), 0, s, jobs, prefix, njobs);
+    hipLaunchKernelGGL((k_many<OpFluid<false, T, PACK>, FluidJob, T>), dim3(nblocks), dim3(T), 0, s, jobs, prefix,
+                       njobs);
     return hipGetLastError();
+}
+template <bool PACK>
+hipError_t one_fluid_t(const FluidJob &j, int threads, hipStream_t s) {
+    switch (threads) {
+    case 64: return one_fluid<64, PACK>(j, s);
+    case 128: return one_fluid<128, PACK>(j, s);
+    case 256: return one_fluid<256, PACK>(j, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+template <bool PACK>
+hipError_t many_fluid_t(const FluidJob *jobs, const int *prefix, int njobs, int nblocks, int threads, hipStream_t s) {
+    switch (threads) {
+    case 64: return many_fluid<64, PACK>(jobs, prefix, njobs, nblocks, s);
+    case 128: return many_fluid<128, PACK>(jobs, prefix, njobs, nblocks, s);
+    case 256: return many_fluid<256, PACK>(jobs, prefix, njobs, nblocks, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 }  // namespace
 
 hipError_t launch_fluid(const DevParams &P, const FluidStepArgs &a, int threads, hipStream_t s) {
     if (P.nf == 0) return hipSuccess;
     const FluidJob j{P, a};
-    switch (threads) {
-    case 64: return one_fluid<64>(j, s);
-    case 128: return one_fluid<128>(j, s);
-    case 256: return one_fluid<256>(j, s);
-    default: return hipErrorInvalidValue;
-    }
+    return (P.form & FSA_FORM_PACKED) ? one_fluid_t<true>(j, threads, s) : one_fluid_t<false>(j, threads, s);
 }
 
 hipError_t launch_many_fluid(const FluidJob *jobs, const int *prefix, int njobs, int nblocks, int threads,
-                             hipStream_t s) {
+                             bool packed, hipStream_t s) {
     if (nblocks <= 0 || njobs <= 0) return hipSuccess;
-    switch (threads) {
-    case 64: return many_fluid<64>(jobs, prefix, njobs, nblocks, s);
-    case 128: return many_fluid<128>(jobs, prefix, njobs, nblocks, s);
-    case 256: return many_fluid<256>(jobs, prefix, njobs, nblocks, s);
-    default: return hipErrorInvalidValue;
-    }
+    return packed ? many_fluid_t<true>(jobs, prefix, njobs, nblocks, threads, s)
+                  : many_fluid_t<false>(jobs, prefix, njobs, nblocks, threads, s);
 }
 
 }  // namespace fsa

@@ -69,6 +69,10 @@ struct DevParams {
     // every fluid partner of a solid that has moved at most d since
     float R2_rec;     // R_rec^2 (with a rounding margin)
     float disp2_max;  // d^2
+    // precision form (FSA_FORM_* bits) and its constants
+    int form;
+    float ival[3];  // cell interval (the packed positions are in cells)
+    float B_inv;    // 1 / B, B = rho0 c0^2 / gamma (packed pressure is p / B)
 };
 
 DevParams make_dev_params(const fsa_params &p);
@@ -90,6 +94,12 @@ struct KindBuffers {
     int32_t *tmp_slot = nullptr;
     int32_t *tmp_id = nullptr;
     int32_t *tile_sums = nullptr;  // one per scan tile
+    // compensated form: the rounding error carried by pos.xyz and vel.xyz,
+    // double-buffered and sorted with the state
+    float4 *cpos[2] = {nullptr, nullptr};
+    float4 *cvel[2] = {nullptr, nullptr};
+    // packed form (fluid): the FP16 partner records of the current order
+    uint4 *pk = nullptr;
 };
 
 struct SolidExtra {
@@ -149,8 +159,11 @@ struct CellJob {
     int has_ref;
     int count_jumps;
     const float4 *pos_in, *vel_in, *ref_in;
+    const float4 *cpos_in, *cvel_in;  // compensation terms (null unless compensated)
     const int32_t *id_in, *prev_cell;
     float4 *pos_out, *vel_out, *ref_out;
+    float4 *cpos_out, *cvel_out;
+    uint4 *pk_out;  // FP16 partner records of the sorted state (null unless packed)
     int32_t *id_out, *cell_out;
     int32_t *key, *rank, *count, *start, *tmp_slot, *tmp_id, *tile_sums;
     unsigned long long *jumps;
@@ -166,6 +179,9 @@ hipError_t launch_fill_i32(int32_t *p, int32_t v, int n, hipStream_t s);
 // ---- fluid pass (fsa_fluid.hip) -----------------------------------------------
 struct FluidStepArgs {
     const float4 *fpos, *fvel;
+    const float4 *fcpos, *fcvel;     // compensation terms in (null unless compensated)
+    float4 *fcpos_out, *fcvel_out;   // and out
+    const uint4 *fpk;                // FP16 partner records (null unless packed)
     const int32_t *fcell;
     const int32_t *fstart, *sstart, *vstart;
     const float4 *spos, *svel, *snp;
@@ -200,6 +216,7 @@ hipError_t launch_virt(const DevParams &P, const VirtArgs &a, hipStream_t s);
 // ---- solid (fsa_solid.hip) -------------------------------------------------------
 struct SolidArgs {
     float4 *spos, *svel;
+    float4 *scpos, *scvel;  // compensation terms (null unless compensated)
     const float4 *sref;
     const int32_t *scell;
     const int32_t *fstart, *sstart;
@@ -231,8 +248,9 @@ enum BatchOp {
 };
 hipError_t launch_many_cells(BatchOp op, const CellJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s);
 int blocks_cells(BatchOp op, const CellJob &j);
+// packed: the jobs' scenes use FSA_FORM_PACKED (all of them or none)
 hipError_t launch_many_fluid(const FluidJob *jobs, const int *prefix, int njobs, int nblocks, int threads,
-                             hipStream_t s);
+                             bool packed, hipStream_t s);
 int blocks_fluid(const FluidJob &j, int threads);
 hipError_t launch_many_virt(const VirtJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s);
 int blocks_virt(const VirtJob &j);
@@ -241,7 +259,7 @@ hipError_t launch_many_solid(BatchOp op, const SolidJob *jobs, const int *prefix
 int blocks_solid(BatchOp op, const SolidJob &j);
 
 // ---- observables (fsa_observe.hip) ------------------------------------------------
-constexpr int kObserveSums = 14;
+constexpr int kObserveSums = 16;
 struct ObserveArgs {
     const float4 *fpos, *fvel;
     const int32_t *fstart;

@@ -30,6 +30,54 @@ __device__ __forceinline__ int find_cell(const DevParams &P, float x, float y, f
     return (ix * P.cn[1] + iy) * P.cn[2] + iz;
 }
 
+// cell coordinates (ix, iy, iz) of cell id c = (ix ny + iy) nz + iz
+__device__ __forceinline__ int3 cell_coords(const DevParams &P, int c) {
+    const int nz = P.cn[2], nyz = P.cn[1] * nz;
+    const int ix = c / nyz;
+    const int rem = c - ix * nyz;
+    const int iy = rem / nz;
+    return make_int3(ix, iy, rem - iy * nz);
+}
+
+// ---- precision forms -----------------------------------------------------
+
+// v += d with compensated (Kahan) summation: c carries the rounding error of
+// the previous additions (the increment is rounded in its own statement, so
+// no multiply-add is fused into it)
+__device__ __forceinline__ float kahan_add(float v, float d, float &c) {
+    const float y = d - c;
+    const float t = v + y;
+    c = (t - v) - y;
+    return t;
+}
+
+typedef _Float16 fsa_half2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ unsigned pack_h2(float a, float b) {
+    const fsa_half2 h = {(_Float16)a, (_Float16)b};  // round to nearest even
+    return __builtin_bit_cast(unsigned, h);
+}
+__device__ __forceinline__ float2 unpack_h2(unsigned v) {
+    const fsa_half2 h = __builtin_bit_cast(fsa_half2, v);
+    return make_float2((float)h.x, (float)h.y);
+}
+
+// position on axis a in cells, relative to the centre of cell index i
+// (in [-0.5, 0.5) for the particle's own cell)
+__device__ __forceinline__ float cell_offset(const DevParams &P, float x, int a, int i) {
+    return (x - P.cmin[a]) * P.inv[a] - ((float)i + 0.5f);
+}
+
+// FP16 partner record of a fluid particle in cell c:
+//   (rx, ry) (rz, rho / rho0 - 1) (ux, uy) (uz, p / B)
+// with r the position relative to the cell centre in cells
+__device__ __forceinline__ uint4 pack_partner(const DevParams &P, float4 x, float4 u, int c) {
+    const int3 ci = cell_coords(P, c);
+    return make_uint4(pack_h2(cell_offset(P, x.x, 0, ci.x), cell_offset(P, x.y, 1, ci.y)),
+                      pack_h2(cell_offset(P, x.z, 2, ci.z), x.w * P.rho0_inv - 1.0f),
+                      pack_h2(u.x, u.y), pack_h2(u.z, u.w * P.B_inv));
+}
+
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
 // b % 8), and each XCD has its own L2.  Renumbering so that XCD x walks one
 // contiguous range of tiles keeps neighbouring cell columns in one L2.

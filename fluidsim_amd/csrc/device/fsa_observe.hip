@@ -38,13 +38,16 @@ __device__ __forceinline__ void block_reduce(double (&v)[N], double (*sh)[kBlock
 }
 
 // partial sums: [0] rho, [1..3] rho u, [4] rho |u|^2 / 2, [5] rho (-g.x),
-// [6] max x (as double), [7..9] solid u, [10..12] tip (x - X), [13] tip count
+// [7..9] solid u, [10..12] tip (x - X), [13] tip count; maxima: [6] x,
+// [14] fluid |u|^2, [15] solid |u|^2 (all in FP64)
+__device__ __forceinline__ bool is_max_slot(int k) { return k == 6 || k >= 14; }
+
 __global__ __launch_bounds__(kBlock) void k_sums(DevParams P, ObserveArgs a, double *__restrict__ partial) {
     __shared__ double sh[kObserveSums][kBlock];
     double v[kObserveSums];
 #pragma unroll
     for (int k = 0; k < kObserveSums; ++k) v[k] = 0.0;
-    double front = -1.0e300;
+    double front = -1.0e300, fu2 = 0.0, su2 = 0.0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < P.nf; i += kSumBlocks * kBlock) {
         const float4 x = a.fpos[i], u = a.fvel[i];
         const double rho = x.w;
@@ -55,12 +58,14 @@ __global__ __launch_bounds__(kBlock) void k_sums(DevParams P, ObserveArgs a, dou
         v[4] += 0.5 * rho * ((double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z);
         v[5] -= rho * ((double)P.gx * x.x + (double)P.gy * x.y + (double)P.gz * x.z);
         front = fmax(front, (double)x.x);
+        fu2 = fmax(fu2, (double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z);
     }
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < P.ns; i += kSumBlocks * kBlock) {
         const float4 x = a.spos[i], u = a.svel[i], X = a.sref[i];
         v[7] += u.x;
         v[8] += u.y;
         v[9] += u.z;
+        su2 = fmax(su2, (double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z);
         const int type = __float_as_int(X.w);
         if (type == FSA_UNCONSTRAINED_SOLID && X.y >= a.tip_y) {
             v[10] += (double)x.x - X.x;
@@ -69,21 +74,26 @@ __global__ __launch_bounds__(kBlock) void k_sums(DevParams P, ObserveArgs a, dou
             v[13] += 1.0;
         }
     }
-    v[6] = front;
-    // the max of slot 6 is combined separately from the sums
-    sh[0][threadIdx.x] = front;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[0][threadIdx.x] = fmax(sh[0][threadIdx.x], sh[0][threadIdx.x + s]);
+    // the maxima are combined separately from the sums
+    double mx[3] = {front, fu2, su2};
+    double bmax[3];
+    for (int m = 0; m < 3; ++m) {
+        sh[0][threadIdx.x] = mx[m];
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) sh[0][threadIdx.x] = fmax(sh[0][threadIdx.x], sh[0][threadIdx.x + s]);
+            __syncthreads();
+        }
+        bmax[m] = sh[0][0];
         __syncthreads();
     }
-    const double bmax = sh[0][0];
-    __syncthreads();
     block_reduce(v, sh);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < kObserveSums; ++k) partial[(size_t)blockIdx.x * kObserveSums + k] = v[k];
-        partial[(size_t)blockIdx.x * kObserveSums + 6] = bmax;
+        partial[(size_t)blockIdx.x * kObserveSums + 6] = bmax[0];
+        partial[(size_t)blockIdx.x * kObserveSums + 14] = bmax[1];
+        partial[(size_t)blockIdx.x * kObserveSums + 15] = bmax[2];
     }
 }
 
@@ -93,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_sums_final(const double *__restrict_
     double acc = k == 6 ? -1.0e300 : 0.0;
     for (int b = 0; b < kSumBlocks; ++b) {
         const double p = partial[(size_t)b * kObserveSums + k];
-        acc = k == 6 ? fmax(acc, p) : acc + p;
+        acc = is_max_slot(k) ? fmax(acc, p) : acc + p;
     }
     out[k] = acc;
 }

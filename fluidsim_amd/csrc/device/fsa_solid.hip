@@ -398,11 +398,37 @@ __device__ __forceinline__ void update_body(const DevParams &P, const SolidArgs 
         a.x.acc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     } else if (type == FSA_UNCONSTRAINED_SOLID) {
         const float4 x = a.spos[i], u = a.svel[i], ac = a.x.acc[i];
-        float px = x.x + u.x * dt, py = x.y + u.y * dt, pz = x.z + u.z * dt;
-        const float4 un = make_float4(u.x + (ac.x + P.gx) * dt, u.y + (ac.y + P.gy) * dt, u.z + (ac.z + P.gz) * dt, 0.0f);
-        px = fminf(fmaxf(px, P.dmin[0]), P.dmax[0]);
-        py = fminf(fmaxf(py, P.dmin[1]), P.dmax[1]);
-        pz = fminf(fmaxf(pz, P.dmin[2]), P.dmax[2]);
+        float px, py, pz;
+        float4 un;
+        if (a.scpos) {
+            // compensated form: x and u carry their rounding error
+            float4 cp = a.scpos[i], cv = a.scvel[i];
+            const float sx = u.x * dt, sy = u.y * dt, sz = u.z * dt;
+            px = kahan_add(x.x, sx, cp.x);
+            py = kahan_add(x.y, sy, cp.y);
+            pz = kahan_add(x.z, sz, cp.z);
+            const float vx = (ac.x + P.gx) * dt, vy = (ac.y + P.gy) * dt, vz = (ac.z + P.gz) * dt;
+            un = make_float4(kahan_add(u.x, vx, cv.x), kahan_add(u.y, vy, cv.y), kahan_add(u.z, vz, cv.z), 0.0f);
+            const float qx = fminf(fmaxf(px, P.dmin[0]), P.dmax[0]);
+            const float qy = fminf(fmaxf(py, P.dmin[1]), P.dmax[1]);
+            const float qz = fminf(fmaxf(pz, P.dmin[2]), P.dmax[2]);
+            if (qx != px) cp.x = 0.0f;
+            if (qy != py) cp.y = 0.0f;
+            if (qz != pz) cp.z = 0.0f;
+            px = qx;
+            py = qy;
+            pz = qz;
+            a.scpos[i] = cp;
+            a.scvel[i] = cv;
+        } else {
+            px = x.x + u.x * dt;
+            py = x.y + u.y * dt;
+            pz = x.z + u.z * dt;
+            un = make_float4(u.x + (ac.x + P.gx) * dt, u.y + (ac.y + P.gy) * dt, u.z + (ac.z + P.gz) * dt, 0.0f);
+            px = fminf(fmaxf(px, P.dmin[0]), P.dmax[0]);
+            py = fminf(fmaxf(py, P.dmin[1]), P.dmax[1]);
+            pz = fminf(fmaxf(pz, P.dmin[2]), P.dmax[2]);
+        }
         a.spos[i] = make_float4(px, py, pz, 0.0f);
         a.svel[i] = un;
     }

@@ -57,6 +57,7 @@ struct VirtualProperties {
 struct Kernel {
     float smoothing_length = 0, particle_diameter = 0, particle_volume = 0, diameter_square = 0;
     int thread_num = 32;
+    int precision_form = 0;  // FSA_FORM_* bits (optional key, not in the reference)
     float smoothing_length_square = 0, smoothing_length_inverse = 0, smoothing_length_inverse_square = 0;
     float impact_length_hsml_ratio = 0, impact_distance_square = 0;
     float kernel_coefficient_1 = 0, kernel_coefficient_2 = 0, kernel_differential = 0;

@@ -62,6 +62,7 @@ class DamBreak:
     solid_sub_step: int = 4
     iend: int = 1
     result_interval: int = 1000000
+    precision_form: int = 0           # FSA_FORM_* bits (optional ini key)
 
     # ---- derived --------------------------------------------------------
     @property
@@ -103,6 +104,8 @@ class DamBreak:
             "impact_length_hsml_ratio": self.impact_ratio,
             "thread_num": 256,
         }
+        if self.precision_form:
+            keys["precision_form"] = self.precision_form
         if extra:
             keys.update(extra)
         sections = {

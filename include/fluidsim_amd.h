@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FSA_ABI_VERSION 1
+#define FSA_ABI_VERSION 2
 
 enum fsa_status {
     FSA_OK = 0,
@@ -114,7 +114,22 @@ typedef struct fsa_params {
     float kernel_coefficient_2;
     float kernel_differential;
     int32_t thread_num; /* read for key compatibility; block size is fixed at 256 on gfx950 */
+
+    /* Precision form: FSA_FORM_* bits, from the optional ini key
+     * `precision_form` (no counterpart in the reference; 0 = its plain FP32
+     * arithmetic). */
+    int32_t precision_form;
 } fsa_params;
+
+/* The fluid pass gathers its fluid partners' attributes from 16-byte FP16
+ * records written by the cell build: position relative to the partner's cell
+ * centre (in cells), density as rho/rho0 - 1, velocity, pressure / B.  Pair
+ * sums stay FP32, and the cut-off test still reads FP32 positions, so the
+ * neighbour sets are the plain form's.  Needs at least 3 cells on every axis. */
+#define FSA_FORM_PACKED 1
+/* Compensated (Kahan) FP32 updates of fluid and solid positions and
+ * velocities: each carries its rounding error into the next step. */
+#define FSA_FORM_COMPENSATED 2
 
 typedef struct fsa_sim fsa_sim;
 
@@ -218,6 +233,12 @@ typedef struct fsa_observables {
     double gate_tip_displacement[3];    /* mean x - X over the gate tip row */
     int64_t gate_tip_count;             /* solids in the tip row: unconstrained solids whose
                                            reference y is within dp/2 of the largest one */
+    double max_fluid_speed;             /* max |u| over the fluid (FP64 reduction) */
+    double max_solid_speed;             /* max |u| over the solid */
+    double cfl_dt;                      /* 0.25 h / (c0 + max fluid |u|): the acoustic CFL bound of
+                                           the time step (the dam-break scenes run at 0.25 h / c0) */
+    double solid_cfl_dt;                /* 0.3 h / (c_s + max solid |u|): the bound of one solid sub-step;
+                                           0 without solids */
 } fsa_observables;
 
 int fsa_observe(fsa_sim *sim, fsa_observables *out);

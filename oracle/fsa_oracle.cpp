@@ -145,7 +145,17 @@ template <class T> struct Par {
     T dt;
     int sub;
     T h, h2, h_inv2, R2, c1, c2, kd;
+    bool compensated;  // FSA_FORM_COMPENSATED: Kahan updates of x and u
 };
+
+// v += d with compensated (Kahan) summation, c carrying the rounding error
+// (Kahan, "Further remarks on reducing truncation errors", CACM 8(1), 1965)
+template <class T> inline T kahan_add(T v, T d, T &c) {
+    const T y = d - c;
+    const T t = v + y;
+    c = (t - v) - y;
+    return t;
+}
 
 template <class T> Par<T> convert(const fsa_params &p) {
     Par<T> q;
@@ -183,6 +193,7 @@ template <class T> Par<T> convert(const fsa_params &p) {
     q.h2 = p.smoothing_length_square;
     q.h_inv2 = p.smoothing_length_inverse_square;
     q.R2 = p.impact_distance_square;
+    q.compensated = (p.precision_form & FSA_FORM_COMPENSATED) != 0;
     q.c1 = p.kernel_coefficient_1;
     q.c2 = p.kernel_coefficient_2;
     q.kd = p.kernel_differential;
@@ -232,6 +243,7 @@ template <class T> struct Oracle : OracleBase {
     // fluid (Fluid_List, include/particle.cuh:20-32)
     std::vector<V3<T>> fx, fu, fa;
     std::vector<T> frho, fp, fdrho;
+    std::vector<V3<T>> fcx, fcu, scx, scu;  // carried rounding errors of x and u (compensated form)
     // solid (Solid_List, include/particle.cuh:91-102)
     std::vector<V3<T>> sx, su, sX, sn, sa, sac;
     std::vector<int> stype;
@@ -252,6 +264,8 @@ template <class T> struct Oracle : OracleBase {
 
     Oracle(const fsa_params &p, int mode_) : P(convert<T>(p)), mode(mode_) {
         fx.resize(P.nf); fu.resize(P.nf); fa.assign(P.nf, {0, 0, 0});
+        fcx.assign(P.nf, {0, 0, 0}); fcu.assign(P.nf, {0, 0, 0});
+        scx.assign(P.ns, {0, 0, 0}); scu.assign(P.ns, {0, 0, 0});
         frho.resize(P.nf); fp.resize(P.nf); fdrho.assign(P.nf, T(0));
         sx.resize(P.ns); su.resize(P.ns); sX.resize(P.ns);
         sn.assign(P.ns, {0, 0, 0}); sa.assign(P.ns, {0, 0, 0}); sac.assign(P.ns, {0, 0, 0});
@@ -368,6 +382,8 @@ template <class T> struct Oracle : OracleBase {
             fp[i] = p[i];
             if (!in_grid(fx[i])) { err = "fluid particle outside the cell grid"; return FSA_ERR_INVALID; }
         }
+        fcx.assign(P.nf, {0, 0, 0});
+        fcu.assign(P.nf, {0, 0, 0});
         lists_built = false;
         return FSA_OK;
     }
@@ -379,6 +395,8 @@ template <class T> struct Oracle : OracleBase {
             stype[i] = type[i];
             if (!in_grid(sx[i])) { err = "solid particle outside the cell grid"; return FSA_ERR_INVALID; }
         }
+        scx.assign(P.ns, {0, 0, 0});
+        scu.assign(P.ns, {0, 0, 0});
         lists_built = false;
         return FSA_OK;
     }
@@ -618,16 +636,22 @@ template <class T> struct Oracle : OracleBase {
             if (!(fdrho[i] == fdrho[i])) fdrho[i] = 0;
             const V3<T> a = fa[i];
             if (!(a.x == a.x && a.y == a.y && a.z == a.z)) fa[i] = {0, 0, 0};
-            fx[i] += fu[i] * P.dt;
-            fu[i] += (fa[i] + P.g) * P.dt;
+            if (P.compensated) {
+                step_compensated(fx[i], fu[i] * P.dt, fcx[i]);
+                step_compensated(fu[i], (fa[i] + P.g) * P.dt, fcu[i]);
+            } else {
+                fx[i] += fu[i] * P.dt;
+                fu[i] += (fa[i] + P.g) * P.dt;
+            }
             frho[i] += fdrho[i] * P.dt;
             frho[i] = std::fmin(std::fmax(frho[i], P.rho_min), P.rho_max);
             fp[i] = density_to_pressure(frho[i]);
-            fx[i].x = std::fmin(std::fmax(fx[i].x, P.dmin[0]), P.dmax[0]);
-            fx[i].y = std::fmin(std::fmax(fx[i].y, P.dmin[1]), P.dmax[1]);
-            fx[i].z = std::fmin(std::fmax(fx[i].z, P.dmin[2]), P.dmax[2]);
+            clamp_to_domain(fx[i], fcx[i]);
             const T speed = std::sqrt(length2(fu[i]));
-            if (speed > max_speed) fu[i] = fu[i] * (max_speed / speed);
+            if (speed > max_speed) {
+                fu[i] = fu[i] * (max_speed / speed);
+                fcu[i] = {0, 0, 0};  // a rescaled velocity drops its carried error
+            }
         }
     }
 
@@ -706,6 +730,24 @@ template <class T> struct Oracle : OracleBase {
         }
     }
 
+    // x += d per component, compensated
+    static void step_compensated(V3<T> &x, V3<T> d, V3<T> &c) {
+        x.x = kahan_add(x.x, d.x, c.x);
+        x.y = kahan_add(x.y, d.y, c.y);
+        x.z = kahan_add(x.z, d.z, c.z);
+    }
+    // the domain clamp of src/fluid.cu:258-260 and src/solid.cu:229-231; a
+    // clamped component drops its carried error
+    void clamp_to_domain(V3<T> &x, V3<T> &c) const {
+        T *v[3] = {&x.x, &x.y, &x.z};
+        T *e[3] = {&c.x, &c.y, &c.z};
+        for (int a = 0; a < 3; ++a) {
+            const T q = std::fmin(std::fmax(*v[a], P.dmin[a]), P.dmax[a]);
+            if (q != *v[a]) *e[a] = 0;
+            *v[a] = q;
+        }
+    }
+
     // ---- K12: solidUpdate, src/solid.cu:209-239 --------------------------
     void solid_update() override {
         const T dt = P.dt / T(P.sub);
@@ -716,11 +758,14 @@ template <class T> struct Oracle : OracleBase {
                 sx[i] += su[i] * dt;
                 sa[i] = {0, 0, 0};
             } else if (stype[i] == FSA_UNCONSTRAINED_SOLID) {
-                sx[i] += su[i] * dt;
-                su[i] += (sa[i] + P.g) * dt;
-                sx[i].x = std::fmin(std::fmax(sx[i].x, P.dmin[0]), P.dmax[0]);
-                sx[i].y = std::fmin(std::fmax(sx[i].y, P.dmin[1]), P.dmax[1]);
-                sx[i].z = std::fmin(std::fmax(sx[i].z, P.dmin[2]), P.dmax[2]);
+                if (P.compensated) {
+                    step_compensated(sx[i], su[i] * dt, scx[i]);
+                    step_compensated(su[i], (sa[i] + P.g) * dt, scu[i]);
+                } else {
+                    sx[i] += su[i] * dt;
+                    su[i] += (sa[i] + P.g) * dt;
+                }
+                clamp_to_domain(sx[i], scx[i]);
             }
         }
     }

@@ -16,6 +16,7 @@ LIB_PATH = HERE / "build" / "liboracle.so"
 
 CANONICAL, EXACT = 0, 1
 FLUID, SOLID, VIRT = 0, 1, 2
+FORM_PACKED, FORM_COMPENSATED = 1, 2  # precision_form bits (include/fluidsim_amd.h)
 
 _lib = None
 

@@ -48,7 +48,7 @@ def test_params_struct_layout_matches_the_header():
 
 def test_abi_version_and_error_paths_without_a_device():
     lib = _native.device_lib()
-    assert lib.fsa_abi_version() == 1
+    assert lib.fsa_abi_version() == 2
     n = C.c_int(-1)
     assert lib.fsa_device_count(C.byref(n)) == 0
     if n.value > 0:

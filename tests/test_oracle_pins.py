@@ -302,6 +302,44 @@ def test_isolated_particles_update_eos_and_clamps():
     assert f["position"][6, 2] == np.float32(sc.tank[2] * sc.dp)
 
 
+def test_compensated_updates_track_the_exact_sum():
+    """FSA_FORM_COMPENSATED: Kahan-summed x += u dt and u += (a + g) dt.
+
+    Isolated particles in free fall add the same FP32 increment fl(g dt) to
+    u every step, so the exact velocity after n steps is u0 + n fl(g dt),
+    and the position moves by the old velocity (fluid.cu:250-251).  The
+    plain FP32 update drifts from these sums by the rounding of every
+    addition; the compensated one stays within a few ulp.  A sign or a
+    term dropped from the compensation step loses that."""
+    sc = scenes.DamBreak(dp=0.01, fluid=(1, 1, 1), tank=(40, 40, 40), gate=None, wall_layers=0, gravity=9.81,
+                         sound_speed=20.0)
+    xs = [[0.05, 0.38, 0.05], [0.2, 0.38, 0.2], [0.35, 0.39, 0.3]]
+    us = [[0.3, 0.0, 0.2], [0.0, 0.0, 0.0], [-0.4, 0.1, 0.0]]
+    n = 1000
+    got = {}
+    for form in (0, orc.FORM_COMPENSATED):
+        params, a = custom_case(sc.with_(precision_form=form), fluid={"position": xs, "velocity": us})
+        o = orc.Oracle(params, 32)
+        o.upload(a)
+        o.step(n)
+        got[form] = o.fluid()
+    dt = np.float32(params.dt)
+    du = (np.float32(0.0) + np.array(params.gravity, np.float32)) * dt  # fl(g dt) per component
+    u0 = np.asarray(us, np.float32).astype(np.float64)
+    x0 = np.asarray(xs, np.float32).astype(np.float64)
+    k = np.arange(n, dtype=np.float64)
+    u_exact = u0 + n * du.astype(np.float64)
+    x_exact = x0 + dt.astype(np.float64) * (n * u0 + du.astype(np.float64) * k.sum())
+    e = {f: (np.abs(g["velocity"] - u_exact).max(), np.abs(g["position"] - x_exact).max()) for f, g in got.items()}
+    ulp_u = np.spacing(np.float32(np.abs(u_exact).max()))
+    ulp_x = np.spacing(np.float32(np.abs(x_exact).max()))
+    assert e[orc.FORM_COMPENSATED][0] <= 2 * ulp_u, e
+    assert e[0][0] >= 20 * max(e[orc.FORM_COMPENSATED][0], ulp_u / 2), e
+    # positions: the compensated sum keeps the (rounded) increments; plain loses ~n/4 ulp
+    assert e[orc.FORM_COMPENSATED][1] <= 8 * ulp_x, e
+    assert e[0][1] >= 10 * e[orc.FORM_COMPENSATED][1], e
+
+
 def test_nan_pair_sums_are_scrubbed(open_box):
     """A NaN neighbour pressure makes the pair sums NaN; they are zeroed (fluid.cu:246-249)."""
     xi = np.array([0.1, 0.1, 0.1])
