@@ -312,6 +312,107 @@ def serve_bench(n_requests: int, seed: int, sim_time: float, load: float, chunk:
     }
 
 
+def tip_displacement(solid_pos, ref_pos, types, dp) -> np.ndarray:
+    """Mean x - X over the gate tip row (unconstrained solids whose reference y is
+    within dp/2 of the largest one), in float64."""
+    free = types == 0
+    if not free.any():
+        return np.zeros(3)
+    X = np.asarray(ref_pos, np.float64)
+    row = free & (X[:, 1] >= X[free, 1].max() - 0.5 * dp)
+    return (np.asarray(solid_pos, np.float64)[row] - X[row]).mean(axis=0)
+
+
+def field_errors(sc, got: dict, ref: dict) -> dict:
+    """Max and RMS errors of a fluid state against the FP64 oracle's, each divided
+    by its scale: L (largest tank extent), U = max(max |u|, sqrt(2 g H)), rho0,
+    and rho0 g H (hydrostatic pressure at the column foot)."""
+    H = sc.fluid[1] * sc.dp
+    L = max(sc.tank) * sc.dp
+    uref = np.asarray(ref["velocity"], np.float64)
+    U = max(float(np.sqrt((uref ** 2).sum(axis=1)).max()), math.sqrt(2.0 * sc.gravity * H))
+    scales = {"position": L, "velocity": U, "density": sc.fluid_density,
+              "pressure": sc.fluid_density * sc.gravity * H}
+    out = {}
+    for k, s in scales.items():
+        d = np.abs(np.asarray(got[k], np.float64) - np.asarray(ref[k], np.float64))
+        out[k] = {"max": float(d.max()) / s, "rms": float(np.sqrt((d ** 2).mean())) / s}
+    return out
+
+
+def precision_bench(steps: int, seed: int, threads: int) -> dict:
+    """The precision forms on the 1,000,000 fluid + 8,000 solid dam break:
+    each form advanced on the GPU, the FP64 oracle (the same method in double
+    precision, OpenMP on the host) advanced over the same steps, per-field
+    errors and the gate-tip displacement error of each form against it."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+
+    base = scenes.c3_precision_scene(seed)
+    a = base.arrays()
+    counts = (len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"]))
+    checkpoints = sorted({1, min(100, steps), steps})
+    forms = [(0, "plain"), (2, "compensated"), (1, "packed"), (3, "packed+compensated")]
+    gpu = {}
+    for form, name in forms:
+        d = DeviceSim(base.with_(precision_form=form).params(counts))
+        d.upload(a)
+        d.synchronize()
+        states, done, secs = {}, 0, 0.0
+        for c in checkpoints:
+            t0 = time.perf_counter()
+            d.step(c - done)
+            d.synchronize()
+            secs += time.perf_counter() - t0
+            done = c
+            states[c] = (d.fluid(), d.solid(), d.observe())
+        gpu[name] = (states, 1e3 * secs / steps)
+        d.close()
+        log(f"precision: {name} form on the GPU, {steps} steps, {1e3 * secs / steps:.3f} ms/step")
+    used = oracle.set_threads(threads)
+    o = oracle.Oracle(base.params(counts), 64, oracle.CANONICAL)
+    o.upload(a)
+    ref, done = {}, 0
+    t0 = time.perf_counter()
+    for c in checkpoints:
+        while done < c:
+            k = min(10, c - done)
+            o.step(k)
+            done += k
+            log(f"precision: FP64 oracle step {done}/{steps} ({time.perf_counter() - t0:.0f}s, {used} threads)")
+        ref[c] = (o.fluid(), o.solid())
+    o.close()
+    types = np.asarray(a["solid_type"])
+    X = a["solid_reference_position"]
+    out = {}
+    for name, (states, ms) in gpu.items():
+        per = {}
+        for c in checkpoints:
+            f, s, obs = states[c]
+            rf, rs = ref[c]
+            tip_ref = tip_displacement(rs["position"], X, types, base.dp)
+            tip_gpu = np.asarray(obs["gate_tip_displacement"], np.float64)
+            per[str(c)] = {
+                "fluid": field_errors(base, f, rf),
+                "solid_position_max": float(np.abs(np.asarray(s["position"], np.float64) - rs["position"]).max()) /
+                                      (max(base.tank) * base.dp),
+                "gate_tip_fp64_m": [float(v) for v in tip_ref],
+                "gate_tip_abs_error_m": float(np.linalg.norm(tip_gpu - tip_ref)),
+                "gate_tip_rel_error": float(np.linalg.norm(tip_gpu - tip_ref) / max(np.linalg.norm(tip_ref), 1e-30)),
+            }
+        out[name] = {"ms_per_step": round(ms, 4), "errors": per}
+        last = per[str(checkpoints[-1])]
+        log(f"precision: {name}: step {checkpoints[-1]} position {last['fluid']['position']['max']:.3g} "
+            f"velocity {last['fluid']['velocity']['max']:.3g} density {last['fluid']['density']['max']:.3g} "
+            f"pressure {last['fluid']['pressure']['max']:.3g} gate tip {last['gate_tip_rel_error']:.3g}")
+    return {
+        "workload": f"dam break, {counts[0]} fluid (100^3, dp 2 mm) + {counts[1]} solid (4x50x40) + "
+                    f"{counts[2]} wall particles, gate 2 dp in front of the column, {steps} steps",
+        "steps": steps, "checkpoints": checkpoints, "oracle": f"FP64 CANONICAL oracle, {used} host threads",
+        "forms": out,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -321,7 +422,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--workload", choices=["step", "batch", "serve"], default="step",
+    ap.add_argument("--workload", choices=["step", "batch", "serve", "precision"], default="step",
                     help="step: one scene's step throughput (the headline line, plus a short batched "
                          "measurement); batch: only the many-scene batched measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched measurement of the step workload")
@@ -374,6 +475,29 @@ def main():
             "data": "synthetic (seeded dam-break requests, Poisson arrivals)",
             "config": {"workload": r["workload"]},
             "serving": r,
+        }
+        print(json.dumps(out), flush=True)
+        return
+
+    if args.workload == "precision":
+        steps = args.steps or 100
+        r = precision_bench(steps, args.batch_seed, args.cpu_threads)
+        pc = r["forms"]["packed+compensated"]["errors"][str(steps)]
+        out = {
+            "metric": "gate-tip displacement error vs FP64 oracle (packed+compensated form)",
+            "value": pc["gate_tip_rel_error"],
+            "unit": "relative",
+            "n_gpus": 1,
+            "steps": steps,
+            "warmup": 0,
+            "ms_per_step": r["forms"]["packed+compensated"]["ms_per_step"],
+            "higher_is_better": False,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32 (f16 partner records)",
+            "data": "synthetic (seeded dam-break lattice)",
+            "config": {"workload": r["workload"]},
+            "precision": r,
         }
         print(json.dumps(out), flush=True)
         return

@@ -161,7 +161,7 @@ int alloc_all(fsa_sim *s) {
     const int ns = s->P.ns;
     if ((rc = s->alloc(&s->sx.np, ns)) || (rc = s->alloc(&s->sx.acc, ns)) || (rc = s->alloc(&s->sx.couple, ns)) ||
         (rc = s->alloc(&s->sx.F, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.P, 9 * (size_t)ns)) ||
-        (rc = s->alloc(&s->sx.L, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.PL, 9 * (size_t)ns)) ||
+        (rc = s->alloc(&s->sx.L, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.PL, 3 * (size_t)ns)) ||
         (rc = s->alloc(&s->sx.A, 9 * (size_t)ns)) ||
         (rc = s->alloc(&s->sx.nbr, (size_t)ns * kSolidListCap * kSolidLanes)) ||
         (rc = s->alloc(&s->sx.ncnt, (size_t)ns * kSolidLanes)) ||
@@ -892,6 +892,7 @@ static ObserveArgs observe_args(fsa_sim *s) {
     a.spos = s->solid.pos[s->solid.st];
     a.svel = s->solid.vel[s->solid.st];
     a.sref = s->solid.ref[s->solid.ord];
+    a.scpos = s->solid.cpos[s->solid.st];
     a.tip_y = s->tip_y;
     return a;
 }

@@ -6,8 +6,8 @@
 //   fluid  pos  float4 (x, y, z, rho)   vel float4 (ux, uy, uz, p)
 //   solid  pos  float4 (x, y, z, -)     vel float4 (ux, uy, uz, -)
 //          ref  float4 (X, Y, Z, type)  np  float4 (nx, ny, nz, p)
-//          F, P, L, PL = P*L, A = det(F) F^-T : 9 SoA planes of ns floats
-//          (plane e = row-major element (e/3, e%3))
+//          F, P, L, A = det(F) F^-T : 9 SoA planes of ns floats
+//          (plane e = row-major element (e/3, e%3)); PL = P*L as 3 float4 rows
 //   virt   pos  float4 (x, y, z, rho)   vel float4 (ux, uy, uz, p)
 //          nrm  float4 (nx, ny, nz, -)
 //   cells  start[kind][ncell+1]: slots of cell c are [start[c], start[c+1])
@@ -106,7 +106,10 @@ struct SolidExtra {
     float4 *np = nullptr;   // normal + coupling pressure
     float4 *acc = nullptr;  // du_dt (w unused)
     float4 *couple = nullptr;
-    float *F = nullptr, *P = nullptr, *L = nullptr, *PL = nullptr, *A = nullptr;  // 9 * ns each
+    float *F = nullptr, *P = nullptr, *L = nullptr, *A = nullptr;  // 9 * ns each
+    // P L by rows, (row r of particle i) at PL[3 * i + r] (w unused): the
+    // force pass gathers it per partner, three loads instead of nine
+    float4 *PL = nullptr;
     // reference-configuration partners of the step: lane l of particle i holds
     // its hits in walk order at nbr[(i * kSolidListCap + k) * kSolidLanes + l],
     // their number (uncapped) at ncnt[i * kSolidLanes + l]
@@ -264,6 +267,7 @@ struct ObserveArgs {
     const float4 *fpos, *fvel;
     const int32_t *fstart;
     const float4 *spos, *svel, *sref;
+    const float4 *scpos;  // carried rounding error of the solid positions (compensated form) or null
     float tip_y;  // reference y at and above which an unconstrained solid is in the tip row
 };
 struct CensusArgs {

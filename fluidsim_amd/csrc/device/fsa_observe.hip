@@ -68,9 +68,11 @@ __global__ __launch_bounds__(kBlock) void k_sums(DevParams P, ObserveArgs a, dou
         su2 = fmax(su2, (double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z);
         const int type = __float_as_int(X.w);
         if (type == FSA_UNCONSTRAINED_SOLID && X.y >= a.tip_y) {
-            v[10] += (double)x.x - X.x;
-            v[11] += (double)x.y - X.y;
-            v[12] += (double)x.z - X.z;
+            // compensated form: the position is x - c (c the carried rounding error)
+            const float4 c = a.scpos ? a.scpos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            v[10] += ((double)x.x - c.x) - X.x;
+            v[11] += ((double)x.y - c.y) - X.y;
+            v[12] += ((double)x.z - c.z) - X.z;
             v[13] += 1.0;
         }
     }

@@ -261,7 +261,11 @@ __device__ __forceinline__ void deformation_body(const DevParams &P, const Solid
     store9(a.x.F, n, i, F);
     store9(a.x.P, n, i, Pk);
     store9(a.x.L, n, i, Lc);
-    store9(a.x.PL, n, i, mmul(Pk, Lc));
+    {
+        const M3 PL = mmul(Pk, Lc);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) a.x.PL[3 * (size_t)i + r] = make_float4(PL.m[r][0], PL.m[r][1], PL.m[r][2], 0.0f);
+    }
     // det(F) F^-T, the factor of the artificial-viscosity force
     const M3 Fi = minverse(F);
     const float dF = mdet(F);
@@ -281,7 +285,14 @@ __device__ __forceinline__ void force_body(const DevParams &P, const SolidArgs &
     const int n = P.ns;
     const float4 xi = a.spos[i], ui = a.svel[i], Xi = a.sref[i];
     const float4 np = a.x.np[i];
-    const M3 PLi = load9(a.x.PL, n, i);
+    M3 PLi;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const float4 row = a.x.PL[3 * (size_t)i + r];
+        PLi.m[r][0] = row.x;
+        PLi.m[r][1] = row.y;
+        PLi.m[r][2] = row.z;
+    }
     const M3 Li = load9(a.x.L, n, i);
     const M3 Ai = load9(a.x.A, n, i);
     float cx = 0.0f, cy = 0.0f, cz = 0.0f;  // couple_dudt
@@ -346,9 +357,11 @@ __device__ __forceinline__ void force_body(const DevParams &P, const SolidArgs &
         float v[3];
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
+            const float4 row = a.x.PL[3 * (size_t)j + r];
+            const float pl[3] = {row.x, row.y, row.z};
             float t = 0.0f;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) t += (a.x.PL[(size_t)(3 * r + c) * n + j] + PLi.m[r][c]) * dX[c];
+            for (int c = 0; c < 3; ++c) t += (pl[c] + PLi.m[r][c]) * dX[c];
             v[r] = t * kc / P.rho_s;
         }
         ax += v[0];

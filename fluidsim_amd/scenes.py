@@ -273,6 +273,14 @@ def c2_showcase_scene(seed: int = 0) -> DamBreak:
                     gate_clamp_rows=3, wall_layers=3, seed=seed, iend=1000)
 
 
+def c3_precision_scene(seed: int = 0, precision_form: int = 0) -> DamBreak:
+    """1,000,000 fluid (100^3, dp 2 mm) + 8,000 solid (4x50x40) + 3 wall layers: the
+    precision-form comparison scene.  The gate stands 2 dp in front of the column,
+    so the water loads it within the first few hundred steps."""
+    return DamBreak(dp=0.002, fluid=(100, 100, 100), tank=(200, 160, 100), gate=(4, 50, 40), gate_x0=102,
+                    gate_clamp_rows=3, wall_layers=3, seed=seed, iend=1000, precision_form=precision_form)
+
+
 def tiny_scene(seed: int = 0, **kw) -> DamBreak:
     """A few thousand particles: every stage active, fits an oracle run in well under a second."""
     base = dict(dp=0.01, fluid=(8, 10, 6), tank=(16, 14, 6), gate=(2, 8, 6), gate_x0=11, gate_clamp_rows=2,

@@ -230,7 +230,8 @@ typedef struct fsa_observables {
     double fluid_potential_energy;      /* sum rho V (-g . x) */
     double front_x;                     /* largest fluid x (the surge front along the flow axis) */
     double solid_momentum[3];           /* sum rho_s V u over solids */
-    double gate_tip_displacement[3];    /* mean x - X over the gate tip row */
+    double gate_tip_displacement[3];    /* mean x - X over the gate tip row (in the compensated
+                                           form x includes its carried rounding error) */
     int64_t gate_tip_count;             /* solids in the tip row: unconstrained solids whose
                                            reference y is within dp/2 of the largest one */
     double max_fluid_speed;             /* max |u| over the fluid (FP64 reduction) */
