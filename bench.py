@@ -185,7 +185,7 @@ def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: flo
     return {
         "workload": f"{n_scenes} dam breaks (seed {seed}): fluid log-uniform in [20k, 500k] on columns of "
                     "aspect ratio U[0.5, 2], gate 1-3 % of the fluid count, gate E U[5, 50] MPa, S from the solid CFL, "
-                    "dp 4 mm, c0 30 m/s",
+                    "dp 4 mm, c0 30 m/s, compensated form (batched and sequential alike)",
         "scenes": n_scenes, "fluid": nf, "solid": ns, "walls": nv,
         "solid_sub_steps": [min(subs), max(subs)],
         "solve": f"{sim_time} s simulated = {solve_steps} steps of dt {dt:.4g} s",
@@ -294,7 +294,7 @@ def serve_bench(n_requests: int, seed: int, sim_time: float, load: float, chunk:
     return {
         "workload": f"{n_requests} scene-solve requests (seed {seed}): fluid log-uniform in [10k, 2M], solid "
                     f"1-3 %, gate E U[5, 50] MPa, c0 = 10 sqrt(2 g H), {sim_time} s simulated each, "
-                    f"Poisson arrivals at {load:.0%} of the batched device rate",
+                    f"Poisson arrivals at {load:.0%} of the batched device rate, compensated form",
         "requests": n_requests,
         "done": sum(r.status == "done" for r in res),
         "particle_steps": int(sum(work)),

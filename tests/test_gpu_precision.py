@@ -109,6 +109,48 @@ def test_batched_forms_are_bit_identical_to_single_steps(gpu_available, form):
         d.close()
 
 
+def test_family_scene_gate_tip_within_one_percent_of_fp64(gpu_available):
+    """The smallest scene of the batched family (compensated form, as the batch
+    and serving measurements run it), with its gate moved to 2 dp in front of
+    the column so that the water loads it from the start, keeps its gate-tip
+    displacement within 1 % of the FP64 oracle's over 1500 steps once the
+    tip has moved 1e-3 dp; the same scene in plain FP32 is printed beside it."""
+    fam = scenes.batch_scenes(64, seed=0)
+    sc = min(fam, key=lambda s: s.counts()[0])
+    sc = sc.with_(gate_x0=sc.fluid[0] + 2)
+    a = sc.arrays()
+    counts = (len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"]))
+    print("case family scene", counts, "S", sc.solid_sub_step, "form", sc.precision_form, flush=True)
+    assert sc.precision_form == FORM_COMPENSATED
+    orc.set_threads(16)
+    o64 = orc.Oracle(sc.with_(precision_form=0).params(counts), 64)
+    o64.upload(a)
+    devs = {}
+    for form in (FORM_COMPENSATED, 0):
+        d = DeviceSim(sc.with_(precision_form=form).params(counts))
+        d.upload(a)
+        devs[form] = d
+    X = np.asarray(a["solid_reference_position"], np.float64)
+    free = np.asarray(a["solid_type"]) == 0
+    row = free & (X[:, 1] >= X[free, 1].max() - 0.5 * sc.dp)
+    checked = 0
+    for k in range(6):
+        for d in devs.values():
+            d.step(250)
+        o64.step(250)
+        ref = (o64.solid()["position"][row] - X[row]).mean(axis=0)
+        got = {f: np.asarray(d.observe()["gate_tip_displacement"]) for f, d in devs.items()}
+        err = {f: float(np.linalg.norm(g - ref) / np.linalg.norm(ref)) for f, g in got.items()}
+        print(f"step {250 * (k + 1)}: tip fp64 {np.linalg.norm(ref):.3e} m, rel error compensated "
+              f"{err[FORM_COMPENSATED]:.2e}, plain {err[0]:.2e}", flush=True)
+        if np.linalg.norm(ref) > 1e-3 * sc.dp:
+            assert err[FORM_COMPENSATED] <= 0.01
+            checked += 1
+    assert checked >= 2, "the gate tip must move during the run"
+    for d in devs.values():
+        d.close()
+
+
 def test_batch_refuses_mixed_packing(gpu_available):
     from test_gpu_batch import _make
     ds = [_make(scenes.tiny_scene(seed=26)), _make(scenes.tiny_scene(seed=27, precision_form=FORM_PACKED))]
