@@ -71,6 +71,7 @@ DevParams make_dev_params(const fsa_params &p) {
     d.c2 = p.kernel_coefficient_2;
     d.kd = p.kernel_differential;
     d.kexp2 = (float)((double)p.smoothing_length_inverse_square * 1.4426950408889634);
+    d.log2kd = p.kernel_differential > 0.0f ? (float)std::log2((double)p.kernel_differential) : -1.0e30f;
     d.cs_inv = 1.0f / p.fluid_sound_speed;
     d.eps_vis = 0.01f * p.smoothing_length_square;
     d.visc10 = 10.0f * p.fluid_viscosity;

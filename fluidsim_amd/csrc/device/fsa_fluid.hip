@@ -44,6 +44,9 @@ enum PairKind { kFluidFluid = 0, kFluidSolid = 1, kFluidVirt = 2 };
 struct Self {
     float x, y, z, rho, ux, uy, uz, p;
     float inv_rho, two_rho;
+    float m2_inv_rho;    // -2 / rho_i
+    float nvisc;         // -10 mu / rho_i
+    float half_inv_rho;  // 1 / (rho_i + rho_j) when rho_j = rho_i (solid partners)
     // packed form: own slot and cell, position relative to the cell centre in cells
     int i, cell;
     float fx, fy, fz;
@@ -63,44 +66,53 @@ __device__ __forceinline__ float exp2_hw(float x) { return __builtin_amdgcn_exp2
 //   v*  = (rho_i u_i + rho_j u_j)/(rho_i+rho_j) + (p_i-p_j)/(rho_i+rho_j)/c0 e
 //   a  += -2 p*/rho_i dx k + [viscid] 10 mu ((u_j-u_i).dx)/(|dx|^2+0.01h^2)/rho_i dx k
 //   rho' += 2 rho_i (u_i - v*).dx k
-// with (u_i - v*).dx reduced to rho_j/(rho_i+rho_j) (u_i-u_j).dx - (p_i-p_j)/(rho_i+rho_j)/c0 |dx|.
-// w is 1 for a partner and 0 for a padding slot of the pair loop (whose
-// terms are then exact zeros).
+// evaluated with A = rho_j/(rho_i+rho_j) (1/2 for a solid partner, whose
+// rho_j is rho_i): p* = p_j + A (p_i - p_j) + [du>=0] rho_i A du D, and
+// (u_i - v*).dx = A (u_i-u_j).dx - (p_i-p_j)/(rho_i+rho_j)/c0 |dx|.
+// k = kd exp(-|dx|^2/h^2) = exp2(log2(kd) - |dx|^2 log2(e)/h^2); a padding
+// slot of the pair loop comes with d2 = kPadD2, whose k is exactly 0 (and
+// every other factor finite), so its terms are exact zeros.
 template <int KIND>
 __device__ __forceinline__ void pair_term(const DevParams &P, const Self &me, float dx, float dy, float dz, float d2,
-                                          float ujx, float ujy, float ujz, float rhoj, float pj, float w, Acc &acc) {
-    const float kc = exp2_hw(-d2 * P.kexp2) * P.kd * w;
+                                          float ujx, float ujy, float ujz, float rhoj, float pj, Acc &acc) {
+    const float kc = exp2_hw(fmaf(-d2, P.kexp2, P.log2kd));
     const float r = d2 != 0.0f ? rsq(d2) : 0.0f;  // 1/|dx|, e = dx r
-    const float inv_rsum = KIND == kFluidSolid ? 0.5f * me.inv_rho : rcp(me.rho + rhoj);
+    const float inv_rsum = KIND == kFluidSolid ? me.half_inv_rho : rcp(me.rho + rhoj);
+    const float A = KIND == kFluidSolid ? 0.5f : rhoj * inv_rsum;
+    const float dpij = me.p - pj;
     const float udx = (me.ux - ujx) * dx + (me.uy - ujy) * dy + (me.uz - ujz) * dz;  // (u_i-u_j).dx
     const float du = udx * r;
-    float ps = (me.rho * pj + rhoj * me.p) * inv_rsum;
+    float ps = fmaf(A, dpij, pj);
     const float diss = KIND == kFluidSolid ? P.cs : fminf(3.0f * du, P.cs);
-    const float raise = me.rho * rhoj * inv_rsum * du * diss;
+    const float raise = me.rho * A * du * diss;
     ps += du >= 0.0f ? raise : 0.0f;
     if (KIND == kFluidVirt) ps = fmaxf(ps, 0.0f);
-    const float dp = (me.p - pj) * inv_rsum * P.cs_inv;
-    float coef = -2.0f * ps * me.inv_rho;
-    if (P.viscid) coef -= P.visc10 * udx * rcp(d2 + P.eps_vis) * me.inv_rho;
+    float coef = ps * me.m2_inv_rho;
+    if (P.viscid) coef = fmaf(me.nvisc * udx, rcp(d2 + P.eps_vis), coef);
     coef *= kc;
     acc.ax += dx * coef;
     acc.ay += dy * coef;
     acc.az += dz * coef;
-    acc.drho += me.two_rho * kc * (rhoj * inv_rsum * udx - dp * (d2 * r));
+    const float dpc = dpij * inv_rsum * P.cs_inv;
+    acc.drho += me.two_rho * kc * (A * udx - dpc * (d2 * r));
 }
+
+// squared distance given to padding slots: exp2 of its (negated, scaled)
+// value is 0, its square root and reciprocal square root stay finite
+constexpr float kPadD2 = 1.0e30f;
 
 template <int KIND>
 __device__ __forceinline__ void one_pair(const DevParams &P, const Self &me, const float4 &pj, const float4 &vj,
-                                         const float4 &n, float w, Acc &acc) {
+                                         const float4 &n, bool valid, Acc &acc) {
     const float dx = pj.x - me.x, dy = pj.y - me.y, dz = pj.z - me.z;
-    const float d2 = len2(dx, dy, dz);
+    const float d2 = valid ? len2(dx, dy, dz) : kPadD2;
     if (KIND == kFluidSolid) {
         // mirrored solid velocity (fluid.cu:79-83); rho_j = rho_i, p_j = p_s
         const float m = 2.0f * ((me.ux - vj.x) * n.x + (me.uy - vj.y) * n.y + (me.uz - vj.z) * n.z);
-        pair_term<KIND>(P, me, dx, dy, dz, d2, me.ux - m * n.x, me.uy - m * n.y, me.uz - m * n.z, me.rho, n.w, w,
+        pair_term<KIND>(P, me, dx, dy, dz, d2, me.ux - m * n.x, me.uy - m * n.y, me.uz - m * n.z, me.rho, n.w,
                         acc);
     } else {
-        pair_term<KIND>(P, me, dx, dy, dz, d2, vj.x, vj.y, vj.z, pj.w, vj.w, w, acc);
+        pair_term<KIND>(P, me, dx, dy, dz, d2, vj.x, vj.y, vj.z, pj.w, vj.w, acc);
     }
 }
 
@@ -111,7 +123,7 @@ __device__ __forceinline__ void one_pair(const DevParams &P, const Self &me, con
 // offset between the two particles is then (d + r_j - f_i) cells.  The
 // reader's own slot gives dx = 0 exactly, as in the FP32 form.
 __device__ __forceinline__ void one_pair_packed(const DevParams &P, const Self &me, const uint4 &rec, int cj, int j,
-                                                float w, Acc &acc) {
+                                                bool valid, Acc &acc) {
     const int nz = P.cn[2], nyz = P.cn[1] * nz;
     int t = cj - me.cell + nyz + nz + 1;
     const int ax = (t >= nyz ? 1 : 0) + (t >= 2 * nyz ? 1 : 0);
@@ -123,9 +135,9 @@ __device__ __forceinline__ void one_pair_packed(const DevParams &P, const Self &
     float dy = ((float)(ay - 1) + (rxy.y - me.fy)) * P.ival[1];
     float dz = ((float)(t - 1) + (rzd.x - me.fz)) * P.ival[2];
     if (j == me.i) dx = dy = dz = 0.0f;
-    const float d2 = len2(dx, dy, dz);
+    const float d2 = valid ? len2(dx, dy, dz) : kPadD2;
     const float rhoj = P.rho0 + P.rho0 * rzd.y;
-    pair_term<kFluidFluid>(P, me, dx, dy, dz, d2, uxy.x, uxy.y, uzp.x, rhoj, uzp.y * P.rho2p, w, acc);
+    pair_term<kFluidFluid>(P, me, dx, dy, dz, d2, uxy.x, uxy.y, uzp.x, rhoj, uzp.y * P.rho2p, acc);
 }
 
 // workgroup sizes of the fluid pass (one to four waves; LDS per wave bounds
@@ -204,7 +216,7 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const Self &me, con
             gather_packed<T>(ps.pk, ps.cell, L, k + kPairBatch, cnt, rn, cn, jn);
 #pragma unroll
             for (int q = 0; q < kPairBatch; ++q)
-                one_pair_packed(P, me, rq[q], cq[q], jq[q], k + q < cnt ? 1.0f : 0.0f, acc);
+                one_pair_packed(P, me, rq[q], cq[q], jq[q], k + q < cnt, acc);
 #pragma unroll
             for (int q = 0; q < kPairBatch; ++q) {
                 rq[q] = rn[q];
@@ -220,7 +232,7 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const Self &me, con
         float4 pn[kPairBatch], vn[kPairBatch], nn[kPairBatch];
         gather<KIND, T>(pos, vel, np, L, k + kPairBatch, cnt, pn, vn, nn);
 #pragma unroll
-        for (int q = 0; q < kPairBatch; ++q) one_pair<KIND>(P, me, pj[q], vj[q], nj[q], k + q < cnt ? 1.0f : 0.0f, acc);
+        for (int q = 0; q < kPairBatch; ++q) one_pair<KIND>(P, me, pj[q], vj[q], nj[q], k + q < cnt, acc);
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q) {
             pj[q] = pn[q];
@@ -273,6 +285,8 @@ __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col
         }
         for (int j = sb; __ballot(j < se) != 0; j += kScanBatch) {
             const int jj = min(j, se);
+            const int lim = se - jj;  // candidates k < lim of the batch lie in the run
+            const int e0 = tag + jj;
             float4 qn[kScanBatch];
             const float4 *nxt = pos + min(j + kScanBatch, se);
 #pragma unroll
@@ -280,8 +294,8 @@ __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col
 #pragma unroll
             for (int k = 0; k < kScanBatch; ++k) {
                 const float d2 = len2(q[k].x - me.x, q[k].y - me.y, q[k].z - me.z);
-                const bool hit = (jj + k < se) & (d2 < P.R2);
-                L.list[cnt * T] = (uint16_t)(tag + jj + k);
+                const bool hit = (k < lim) & (d2 < P.R2);
+                L.list[cnt * T] = (uint16_t)(e0 + k);
                 cnt += hit ? 1 : 0;
             }
 #pragma unroll
@@ -360,6 +374,9 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     me.p = vi4.w;
     me.inv_rho = 1.0f / me.rho;
     me.two_rho = 2.0f * me.rho;
+    me.m2_inv_rho = -2.0f * me.inv_rho;
+    me.nvisc = -P.visc10 * me.inv_rho;
+    me.half_inv_rho = 0.5f * me.inv_rho;
     Acc acc{0.0f, 0.0f, 0.0f, 0.0f};
     const int cell = a.fcell[i];
     me.i = i;

@@ -60,6 +60,7 @@ struct DevParams {
     float h, h2, h_inv2, R2, c1, c2, kd;
     // host-folded constants of the hot loops
     float kexp2;      // h^-2 * log2(e): exp(-q^2) = exp2(-d2 * kexp2)
+    float log2kd;     // log2(kd): kd exp(-q^2) = exp2(log2kd - d2 * kexp2)
     float cs_inv;     // 1 / c0
     float eps_vis;    // 0.01 h^2
     float visc10;     // 10 * mu
