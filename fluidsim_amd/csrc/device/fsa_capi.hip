@@ -80,6 +80,7 @@ DevParams make_dev_params(const fsa_params &p) {
         const double R = std::sqrt((double)p.impact_distance_square), disp = 0.04 * R;
         d.disp2_max = (float)(disp * disp);
         d.R2_rec = (float)((R + disp) * (R + disp) * (1.0 + 1.0e-5));
+        d.R2w = (float)(R * R * (1.0 + 1.0e-5));
     }
     d.form = p.precision_form;
     for (int a = 0; a < 3; ++a) d.ival[a] = p.interval[a];
@@ -96,7 +97,8 @@ int check_params(const fsa_params &p) {
         if (p.cell_number[a] <= 0) return set_err(FSA_ERR_INVALID, "cell_number must be positive on every axis");
     const int64_t total = (int64_t)p.cell_number[0] * p.cell_number[1] * p.cell_number[2];
     if (total != p.cell_number_total) return set_err(FSA_ERR_INVALID, "cell_number_total != product of cell_number");
-    if (total >= (int64_t)INT32_MAX - 2 * kScanTile) return set_err(FSA_ERR_INVALID, "cell grid too large for int32 ids");
+    if (total * kSubZ >= (int64_t)INT32_MAX - 2 * kScanTile)
+        return set_err(FSA_ERR_INVALID, "cell grid too large for int32 sort keys");
     if (!(p.dt > 0.0f)) return set_err(FSA_ERR_INVALID, "dt must be positive");
     if (p.solid_sub_step < 1) return set_err(FSA_ERR_INVALID, "solid_sub_step must be >= 1");
     if (p.precision_form < 0 || p.precision_form > (FSA_FORM_PACKED | FSA_FORM_COMPENSATED))
@@ -120,9 +122,11 @@ int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_sta
                bool packed = false) {
     k.n = n;
     const int ncell = s->P.ncell;
-    const size_t padded = (size_t)(ncell + kScanTile - 1) / kScanTile * kScanTile;
+    const int nk = ncell * kSubZ;
+    const size_t padded = (size_t)(nk + kScanTile - 1) / kScanTile * kScanTile;
     int rc;
     if ((rc = s->alloc(&k.start, (size_t)ncell + 1))) return rc;
+    if ((rc = s->alloc(&k.fstart, (size_t)nk + 1))) return rc;
     if ((rc = s->alloc(&k.count, padded))) return rc;
     if (n == 0) return FSA_OK;
     for (int b = 0; b < 2; ++b) {
@@ -148,7 +152,7 @@ int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_sta
     if ((rc = s->alloc(&k.rank, n))) return rc;
     if ((rc = s->alloc(&k.tmp_slot, n))) return rc;
     if ((rc = s->alloc(&k.tmp_id, n))) return rc;
-    if ((rc = s->alloc(&k.tile_sums, (size_t)scan_tiles(ncell)))) return rc;
+    if ((rc = s->alloc(&k.tile_sums, (size_t)scan_tiles(nk)))) return rc;
     return FSA_OK;
 }
 
@@ -179,6 +183,7 @@ int alloc_all(fsa_sim *s) {
 // start arrays of kinds without particles are all zero (every cell empty)
 int zero_starts(fsa_sim *s, KindBuffers &k) {
     HIP_TRY(launch_fill_i32(k.start, 0, s->P.ncell + 1, s->stream));
+    HIP_TRY(launch_fill_i32(k.fstart, 0, s->P.ncell * kSubZ + 1, s->stream));
     return FSA_OK;
 }
 
@@ -325,6 +330,7 @@ FluidStepArgs fsa::fluid_args(const fsa_sim *s, const KindBuffers &f, const Kind
     a.fpk = f.pk;
     a.fcell = f.cell[f.ord];
     a.fstart = f.start;
+    a.ffine = f.fstart;
     a.sstart = so.start;
     a.vstart = s->virt.start;
     a.spos = so.pos[so.st];

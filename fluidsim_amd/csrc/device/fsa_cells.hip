@@ -3,17 +3,20 @@
 // The reference keeps per-cell singly linked lists and rebuilds them each
 // step by walking the previous lists of the 27 neighbour cells with one
 // thread per cell (src/pair.cu:72-207): a latency-bound pointer chase.  Here
-// the cell list is a counting sort of the particles by cell id:
-//   1. fill+hash   zero the per-cell counts; one thread per particle: cell id
-//                  (findPositionCell, src/pair.cu:8-24) and an arrival rank
-//                  in its cell (atomics aggregated per wave)
-//   2. scan        exclusive prefix sum of the per-cell counts -> start[]
-//   3. place       the particle's slot and .dat id at start[c] + rank
-//   4. gather      inside each cell the final rank is the number of cell
-//                  members with a smaller .dat id, which makes the order
-//                  deterministic (ascending .dat id) whatever order the
-//                  atomics came in; the state is gathered into the other
-//                  buffer at start[c] + rank.
+// the cell list is a counting sort of the particles by key, the cell id
+// (findPositionCell, src/pair.cu:8-24) times kSubZ plus the quarter-cell z
+// index inside the cell:
+//   1. fill+hash   zero the per-key counts; one thread per particle: key
+//                  and an arrival rank among its key (one atomic per run of
+//                  equal keys in a wavefront)
+//   2. scan        exclusive prefix sum of the per-key counts -> fstart[];
+//                  the per-cell starts are every kSubZ-th of them
+//   3. place       the particle's slot and .dat id at fstart[k] + rank
+//   4. gather      inside each key the final rank is the number of members
+//                  with a smaller .dat id, which makes the order
+//                  deterministic (ascending .dat id within a key) whatever
+//                  order the atomics came in; the state is gathered into the
+//                  other buffer at fstart[k] + rank.
 // All passes stream over HBM except the small per-cell rank loop, which
 // reads the cell's ids from L1/L2.
 #include "fsa_internal.h"
@@ -31,37 +34,42 @@ __global__ __launch_bounds__(kBlock) void k_fill_i32(int32_t *__restrict__ p, in
 struct OpFillCountsImpl {
     static __device__ void run(const CellJob &j, int blk, int) {
         const int i = blk * kBlock + threadIdx.x;
-        if (i < padded_cells(j.P.ncell)) j.count[i] = 0;
+        if (i < padded_cells(j.nk)) j.count[i] = 0;
     }
 };
 
 struct OpHashImpl {
     static __device__ void run(const CellJob &j, int blk, int) {
         const int s = blk * kBlock + threadIdx.x;
-        if (s >= j.n) return;
+        const bool valid = s < j.n;
         const DevParams &P = j.P;
-        const float4 x = j.pos_in[s];
-        const int c = find_cell(P, x.x, x.y, x.z);
-        j.key[s] = c;
-        // Particles arrive sorted by their previous cell, so the lanes of a
-        // wave share few cells: one atomic per distinct cell and wave, the
-        // lanes of a cell take consecutive ranks (the order is fixed later by
-        // the gather pass).
-        const int lane = threadIdx.x & 63;
-        const unsigned long long below = (1ull << lane) - 1ull;
-        unsigned long long todo = __ballot(1);
-        int r = 0;
-        while (todo) {
-            const int leader = __ffsll((long long)todo) - 1;
-            const int lc = __shfl(c, leader, 64);
-            const unsigned long long same = __ballot(c == lc);
-            int base = 0;
-            if (lane == leader) base = atomicAdd(&j.count[lc], __popcll(same));
-            base = __shfl(base, leader, 64);
-            if (c == lc) r = base + __popcll(same & below);
-            todo &= ~same;
+        // every lane takes part in the wave operations; lanes past the end
+        // (the tail of the last wave) carry key -1 and do no memory access
+        int key = -1;
+        if (valid) {
+            const float4 x = j.pos_in[s];
+            key = cell_key(P, x.x, x.y, x.z);
         }
-        j.rank[s] = r;
+        // Particles arrive sorted by their previous key, so equal keys come
+        // in runs of adjacent lanes.  The first lane of each run (a lane
+        // whose left neighbour has another key) takes the run's ranks with
+        // one atomic; the others add their distance to it.  Equal keys in
+        // separate runs take separate rank blocks, so ranks stay unique and
+        // counts exact whatever the order (the gather pass fixes the order).
+        const int lane = threadIdx.x & 63;
+        const int left = __shfl_up(key, 1, 64);
+        const unsigned long long heads = __ballot(lane == 0 || key != left);
+        const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        const int head = 63 - __clzll((long long)(heads & upto));
+        const unsigned long long later = heads & ~upto;
+        const int next = later ? __ffsll((long long)later) - 1 : 64;
+        int base = 0;
+        if (valid && lane == head) base = atomicAdd(&j.count[key], next - lane);
+        base = __shfl(base, head, 64);
+        if (!valid) return;
+        const int c = key / kSubZ;
+        j.key[s] = key;
+        j.rank[s] = base + (lane - head);
         if (j.count_jumps) {
             // more than one cell in any direction since the last build: the
             // reference's list rebuild would have lost this particle
@@ -103,7 +111,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *wave_tot, int &b
 struct OpScanTilesImpl {
     static __device__ void run(const CellJob &j, int blk, int) {
         __shared__ int wave_tot[kBlock / 64];
-        const int n = j.P.ncell;
+        const int n = j.nk;
         // a batched grid rounds each scene's block range up: whole blocks past
         // the last tile leave before the first barrier
         if (blk >= scan_tiles(n)) return;
@@ -116,7 +124,7 @@ struct OpScanTilesImpl {
         const int vals[4] = {ex + t0, ex + t1, ex + t2, ex + t3};
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            if (base + k < n) j.start[1 + base + k] = vals[k];
+            if (base + k < n) j.fstart[1 + base + k] = vals[k];
         if (threadIdx.x == 0) j.tile_sums[blk] = total;
     }
 };
@@ -127,7 +135,7 @@ struct OpScanSumsImpl {
         __shared__ int wave_tot[kBlock / 64];
         __shared__ int carry;
         if (blk != 0) return;  // padding blocks of a batched grid
-        const int ntiles = scan_tiles(j.P.ncell);
+        const int ntiles = scan_tiles(j.nk);
         if (threadIdx.x == 0) carry = 0;
         __syncthreads();
         for (int b = 0; b < ntiles; b += kBlock) {
@@ -148,8 +156,15 @@ struct OpScanSumsImpl {
 struct OpScanAddImpl {
     static __device__ void run(const CellJob &j, int blk, int) {
         const int i = blk * kBlock + threadIdx.x;
-        if (i == 0) j.start[0] = 0;
-        if (i < j.P.ncell) j.start[1 + i] += j.tile_sums[i / kScanTile];
+        if (i == 0) j.fstart[0] = 0;
+        if (i < j.nk) j.fstart[1 + i] += j.tile_sums[i / kScanTile];
+    }
+};
+// the per-cell starts: cell c begins where its first key does
+struct OpCellStartsImpl {
+    static __device__ void run(const CellJob &j, int blk, int) {
+        const int c = blk * kBlock + threadIdx.x;
+        if (c <= j.P.ncell) j.start[c] = j.fstart[c * kSubZ];
     }
 };
 
@@ -157,7 +172,7 @@ struct OpPlaceImpl {
     static __device__ void run(const CellJob &j, int blk, int) {
         const int s = blk * kBlock + threadIdx.x;
         if (s >= j.n) return;
-        const int t = j.start[j.key[s]] + j.rank[s];
+        const int t = j.fstart[j.key[s]] + j.rank[s];
         j.tmp_slot[t] = s;
         j.tmp_id[t] = j.id_in[s];
     }
@@ -169,8 +184,9 @@ struct OpGatherImpl {
         if (t >= j.n) return;
         const int s = j.tmp_slot[t];
         const int my = j.tmp_id[t];
-        const int c = j.key[s];
-        const int b = j.start[c], e = j.start[c + 1];
+        const int k = j.key[s];
+        const int c = k / kSubZ;
+        const int b = j.fstart[k], e = j.fstart[k + 1];
         int r = 0;
         for (int u = b; u < e; ++u) r += j.tmp_id[u] < my ? 1 : 0;
         const int d = b + r;
@@ -200,11 +216,12 @@ hipError_t one(const CellJob &j, int blocks, hipStream_t s) {
 int blocks_cells(BatchOp op, const CellJob &j) {
     if (j.n == 0) return 0;
     switch (op) {
-    case kBFillCounts: return blocks_for(padded_cells(j.P.ncell));
+    case kBFillCounts: return blocks_for(padded_cells(j.nk));
     case kBHash: return blocks_for(j.n);
-    case kBScanTiles: return scan_tiles(j.P.ncell);
+    case kBScanTiles: return scan_tiles(j.nk);
     case kBScanSums: return 1;
-    case kBScanAdd: return blocks_for(j.P.ncell);
+    case kBScanAdd: return blocks_for(j.nk);
+    case kBCellStarts: return blocks_for((long long)j.P.ncell + 1);
     case kBPlace: return blocks_for(j.n);
     case kBGather: return blocks_for(j.n);
     default: return 0;
@@ -221,6 +238,9 @@ hipError_t launch_many_cells(BatchOp op, const CellJob *jobs, const int *prefix,
     case kBScanTiles: hipLaunchKernelGGL((k_many<OpScanTilesImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
     case kBScanSums: hipLaunchKernelGGL((k_many<OpScanSumsImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
     case kBScanAdd: hipLaunchKernelGGL((k_many<OpScanAddImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    case kBCellStarts:
+        hipLaunchKernelGGL((k_many<OpCellStartsImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs);
+        break;
     case kBPlace: hipLaunchKernelGGL((k_many<OpPlaceImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
     case kBGather: hipLaunchKernelGGL((k_many<OpGatherImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
     default: return hipErrorInvalidValue;
@@ -232,6 +252,7 @@ CellJob make_cell_job(const DevParams &P, const KindBuffers &k, unsigned long lo
     CellJob j{};
     j.P = P;
     j.n = k.n;
+    j.nk = P.ncell * kSubZ;
     j.has_ref = k.ref[0] != nullptr ? 1 : 0;
     j.count_jumps = count_jumps ? 1 : 0;
     const int so = k.st, sn = 1 - k.st, oo = k.ord, on = 1 - k.ord;
@@ -254,6 +275,7 @@ CellJob make_cell_job(const DevParams &P, const KindBuffers &k, unsigned long lo
     j.rank = k.rank;
     j.count = k.count;
     j.start = k.start;
+    j.fstart = k.fstart;
     j.tmp_slot = k.tmp_slot;
     j.tmp_id = k.tmp_id;
     j.tile_sums = k.tile_sums;
@@ -278,6 +300,7 @@ hipError_t launch_sort_one(const CellJob &j, hipStream_t s) {
     if ((e = one<OpScanTilesImpl>(j, blocks_cells(kBScanTiles, j), s)) != hipSuccess) return e;
     if ((e = one<OpScanSumsImpl>(j, blocks_cells(kBScanSums, j), s)) != hipSuccess) return e;
     if ((e = one<OpScanAddImpl>(j, blocks_cells(kBScanAdd, j), s)) != hipSuccess) return e;
+    if ((e = one<OpCellStartsImpl>(j, blocks_cells(kBCellStarts, j), s)) != hipSuccess) return e;
     if ((e = one<OpPlaceImpl>(j, blocks_cells(kBPlace, j), s)) != hipSuccess) return e;
     return one<OpGatherImpl>(j, blocks_cells(kBGather, j), s);
 }

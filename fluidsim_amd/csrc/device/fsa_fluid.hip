@@ -336,17 +336,58 @@ struct Columns {
     __device__ __forceinline__ int c1(int col) const { return inside(col) ? base(col) + zhi1 : 0; }
 };
 
-// all partners of one kind: the 9 columns in order, then one evaluation
+// The z-window of column `col` (fluid partners): inside a cell the slots
+// are sorted by quarter-cell z index, so a column's slots with z in a range
+// are the contiguous key range of that z-range.  The column is at least
+// (gx, gy) away from the particle across x and y (its distance to the cell
+// faces the column lies beyond, less a 1e-4-cell margin), so partners
+// within R lie within rz = sqrt(R^2 (1 + 1e-5) - gx^2 - gy^2) in z; the
+// window is the quarter-cells of [z - rz, z + rz], widened by 1e-3 quarter
+// cell, which covers the rounding of both sides of the comparison.  The
+// exact cut-off test still runs on every slot of the window, so the partner
+// list is the whole column's, in the same order.  fx, fy: the particle's
+// position in its cell in cells; tzs: its z in quarter cells.
+__device__ __forceinline__ void z_window(const DevParams &P, const Columns &cols, const int32_t *__restrict__ fine,
+                                         int col, float fx, float fy, float tzs, int &b, int &e) {
+    const int ox = col / 3 - 1, oy = col % 3 - 1;
+    const float ex = ox < 0 ? fx : (ox > 0 ? 1.0f - fx : 0.0f);
+    const float ey = oy < 0 ? fy : (oy > 0 ? 1.0f - fy : 0.0f);
+    const float gx = fmaxf(ex - 1.0e-4f, 0.0f) * P.ival[0], gy = fmaxf(ey - 1.0e-4f, 0.0f) * P.ival[1];
+    const float r2 = P.R2w - gx * gx - gy * gy;
+    b = e = 0;
+    if (!cols.inside(col) || !(r2 > 0.0f)) return;
+    const float rzs = sqrtf(r2) * P.inv[2] * (float)kSubZ;
+    const int lo = max((int)floorf(tzs - rzs - 1.0e-3f), cols.zlo * kSubZ);
+    const int hi = max(min((int)floorf(tzs + rzs + 1.0e-3f) + 1, cols.zhi1 * kSubZ), lo);
+    const int k0 = cols.base(col) * kSubZ;
+    b = fine[k0 + lo];
+    e = fine[k0 + hi];
+}
+
+// all partners of one kind: the 9 columns in order, then one evaluation;
+// fluid partners are read from each column's z-window only
 template <int KIND, int T, bool PACK>
 __device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cell, const int32_t *__restrict__ start,
-                                     const float4 *__restrict__ pos, const float4 *__restrict__ vel,
-                                     const float4 *__restrict__ np, const PackedSrc &ps, const Lists<T> &L,
-                                     Acc &acc) {
+                                     const int32_t *__restrict__ fine, const float4 *__restrict__ pos,
+                                     const float4 *__restrict__ vel, const float4 *__restrict__ np,
+                                     const PackedSrc &ps, const Lists<T> &L, Acc &acc) {
     const Columns cols(P, cell);
+    float fx = 0.0f, fy = 0.0f, tzs = 0.0f;
+    if (KIND == kFluidFluid) {
+        fx = (me.x - P.cmin[0]) * P.inv[0] - (float)cols.cx;
+        fy = (me.y - P.cmin[1]) * P.inv[1] - (float)cols.cy;
+        tzs = (me.z - P.cmin[2]) * P.inv[2] * (float)kSubZ;
+    }
     int cnt = 0, spare = 0;
 #pragma unroll 1
     for (int col = 0; col < kColumns; ++col) {
-        const int b = start[cols.c0(col)], e = start[cols.c1(col)];
+        int b, e;
+        if (KIND == kFluidFluid) {
+            z_window(P, cols, fine, col, fx, fy, tzs, b, e);
+        } else {
+            b = start[cols.c0(col)];
+            e = start[cols.c1(col)];
+        }
         L.base[col * T] = b;
         scan<KIND, T, PACK>(P, me, col, b, e, pos, vel, np, ps, L, cnt, spare, acc);
     }
@@ -391,9 +432,9 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
 
     // partners kind by kind: fluid, solid, wall (the canonical order)
     const PackedSrc fsrc{a.fpk, a.fcell}, none{nullptr, nullptr};
-    pass<kFluidFluid, T, PACK>(P, me, cell, a.fstart, a.fpos, a.fvel, nullptr, fsrc, list, acc);
-    pass<kFluidSolid, T, false>(P, me, cell, a.sstart, a.spos, a.svel, a.snp, none, list, acc);
-    pass<kFluidVirt, T, false>(P, me, cell, a.vstart, a.vpos, a.vvel, nullptr, none, list, acc);
+    pass<kFluidFluid, T, PACK>(P, me, cell, a.fstart, a.ffine, a.fpos, a.fvel, nullptr, fsrc, list, acc);
+    pass<kFluidSolid, T, false>(P, me, cell, a.sstart, nullptr, a.spos, a.svel, a.snp, none, list, acc);
+    pass<kFluidVirt, T, false>(P, me, cell, a.vstart, nullptr, a.vpos, a.vvel, nullptr, none, list, acc);
 
     // fluidUpdate (fluid.cu:234-269): NaN scrub, explicit step with the old
     // velocity for the position, density clamp, EOS, domain clamp, speed cap

@@ -41,6 +41,11 @@ constexpr int kScanTile = 1024;  // elements per scan workgroup (256 threads x 4
 // results past the range are masked out of every test)
 constexpr int kPosPad = 16;
 constexpr int kSolidLanes = 16;  // lanes serving one solid particle
+// Inside a cell, particles are sorted by a quarter-cell z index (then by
+// .dat index): the sort key is cell * kSubZ + sub.  The per-cell start
+// arrays every pass uses are taken from the per-key ones, which the fluid
+// pass also reads to test only the z-range of a column its cut-off reaches.
+constexpr int kSubZ = 4;
 // solid-solid partners recorded per lane in the first sub-step of a step
 // (entries per lane; a lane with more partners walks its candidates instead)
 constexpr int kSolidListCap = 16;
@@ -68,6 +73,7 @@ struct DevParams {
     // fluid partners of the solid force pass are recorded by the coupling
     // pass within R_rec = R + d of the step-start solid position; they cover
     // every fluid partner of a solid that has moved at most d since
+    float R2w;        // R^2 (1 + 1e-5): the cut-off the fluid pass's z-windows are drawn with
     float R2_rec;     // R_rec^2 (with a rounding margin)
     float disp2_max;  // d^2
     // precision form (FSA_FORM_* bits) and its constants
@@ -88,8 +94,9 @@ struct KindBuffers {
     float4 *ref[2] = {nullptr, nullptr};  // solid only
     int32_t *id[2] = {nullptr, nullptr};
     int32_t *cell[2] = {nullptr, nullptr};
-    int32_t *start = nullptr;  // ncell + 1
-    int32_t *count = nullptr;  // ncell padded to kScanTile
+    int32_t *start = nullptr;   // ncell + 1: slots of cell c are [start[c], start[c+1])
+    int32_t *fstart = nullptr;  // ncell * kSubZ + 1: slots of key k are [fstart[k], fstart[k+1])
+    int32_t *count = nullptr;   // ncell * kSubZ padded to kScanTile
     int32_t *key = nullptr;    // cell of each slot of the current order
     int32_t *rank = nullptr;   // arrival slot inside the cell (atomic)
     int32_t *tmp_slot = nullptr;
@@ -160,6 +167,7 @@ __global__ __launch_bounds__(THREADS) void k_many(const Job *__restrict__ jobs, 
 struct CellJob {
     DevParams P;
     int n;
+    int nk;  // keys: ncell * kSubZ
     int has_ref;
     int count_jumps;
     const float4 *pos_in, *vel_in, *ref_in;
@@ -169,7 +177,7 @@ struct CellJob {
     float4 *cpos_out, *cvel_out;
     uint4 *pk_out;  // FP16 partner records of the sorted state (null unless packed)
     int32_t *id_out, *cell_out;
-    int32_t *key, *rank, *count, *start, *tmp_slot, *tmp_id, *tile_sums;
+    int32_t *key, *rank, *count, *start, *fstart, *tmp_slot, *tmp_id, *tile_sums;
     unsigned long long *jumps;
 };
 // the job of kind k reading the current buffers and writing the other ones
@@ -188,6 +196,7 @@ struct FluidStepArgs {
     const uint4 *fpk;                // FP16 partner records (null unless packed)
     const int32_t *fcell;
     const int32_t *fstart, *sstart, *vstart;
+    const int32_t *ffine;            // per-key starts of the fluid (kSubZ keys per cell)
     const float4 *spos, *svel, *snp;
     const float4 *vpos, *vvel;
     float4 *fpos_out, *fvel_out;
@@ -247,7 +256,7 @@ hipError_t launch_solid_update(const DevParams &P, const SolidArgs &a, hipStream
 // rounds every job's range up to whole XCD rounds (multiples of 8 blocks) so
 // a block's XCD is the same in the batched grid as in a single-scene grid.
 enum BatchOp {
-    kBFillCounts, kBHash, kBScanTiles, kBScanSums, kBScanAdd, kBPlace, kBGather,  // CellJob
+    kBFillCounts, kBHash, kBScanTiles, kBScanSums, kBScanAdd, kBCellStarts, kBPlace, kBGather,  // CellJob
     kBCouple, kBDeformRecord, kBDeform, kBForce, kBSolidUpdate                     // SolidJob
 };
 hipError_t launch_many_cells(BatchOp op, const CellJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s);

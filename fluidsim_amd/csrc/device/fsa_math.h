@@ -30,6 +30,17 @@ __device__ __forceinline__ int find_cell(const DevParams &P, float x, float y, f
     return (ix * P.cn[1] + iy) * P.cn[2] + iz;
 }
 
+// sort key of a position: cell * kSubZ + the quarter-cell z index inside
+// the cell, (int)(t kSubZ) - kSubZ (int)t with t the z coordinate in cells
+// (kSubZ t is exact), clamped like the cell indices
+__device__ __forceinline__ int cell_key(const DevParams &P, float x, float y, float z) {
+    const int c = find_cell(P, x, y, z);
+    const float tz = (z - P.cmin[2]) * P.inv[2];
+    const int iz = min(max((int)tz, 0), P.cn[2] - 1);
+    const int sub = min(max((int)(tz * (float)kSubZ) - kSubZ * iz, 0), kSubZ - 1);
+    return c * kSubZ + sub;
+}
+
 // cell coordinates (ix, iy, iz) of cell id c = (ix ny + iy) nz + iz
 __device__ __forceinline__ int3 cell_coords(const DevParams &P, int c) {
     const int nz = P.cn[2], nyz = P.cn[1] * nz;

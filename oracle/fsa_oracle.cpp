@@ -5,7 +5,10 @@
 //
 // Templated on the arithmetic type T (float = the reference's FP32, double =
 // the FP64 precision oracle) and on the neighbour traversal:
-//   CANONICAL  cells sorted by (cell id, .dat index); for a particle in cell
+//   CANONICAL  cells sorted by (cell id, quarter-cell z index inside the cell,
+//              .dat index) -- the quarter-cell index is (int)(4 t) - 4 (int)t
+//              with t = (z - cell_min_z) / interval_z, the key order of the
+//              GPU's cell build; for a particle in cell
 //              (cx,cy,cz) the partners are walked kind by kind (fluid, solid,
 //              virt), and for each kind the 27 neighbour cells as 9 columns
 //              (ox,oy ascending), each column as the contiguous z-run
@@ -253,8 +256,9 @@ template <class T> struct Oracle : OracleBase {
     std::vector<V3<T>> vx, vn, vu;
     std::vector<T> vrho, vp;
     std::vector<int> vtype;
-    // cell id per particle and kind
-    std::vector<int> cell[3];
+    // cell id and quarter-cell z index per particle and kind
+    std::vector<int> cell[3], sub[3];
+    static constexpr int kSubZ = 4;
     // canonical lists
     std::vector<int> start[3], sorted[3];
     // exact linked lists (include/cell.cuh:11-42)
@@ -276,6 +280,7 @@ template <class T> struct Oracle : OracleBase {
         const int n[3] = {P.nf, P.ns, P.nv};
         for (int k = 0; k < 3; ++k) {
             cell[k].assign(n[k], -1);
+            sub[k].assign(n[k], 0);
             next[k].assign(n[k], -1);
             next_prev[k].assign(n[k], -1);
             head[k].assign(P.ncell, -1);
@@ -447,7 +452,12 @@ template <class T> struct Oracle : OracleBase {
             const auto &x = pos(k);
             const int n = count(k);
 #pragma omp parallel for schedule(static)
-            for (int i = 0; i < n; ++i) cell[k][i] = find_cell(x[i]);
+            for (int i = 0; i < n; ++i) {
+                cell[k][i] = find_cell(x[i]);
+                const T tz = (x[i].z - P.cmin[2]) * P.inv[2];
+                const int iz = std::min(std::max((int)tz, 0), P.cn[2] - 1);
+                sub[k][i] = std::min(std::max((int)(tz * T(kSubZ)) - kSubZ * iz, 0), kSubZ - 1);
+            }
         }
     }
 
@@ -457,15 +467,18 @@ template <class T> struct Oracle : OracleBase {
     // ---- cell lists ------------------------------------------------------
     void fill_cell() override {
         if (mode == CANONICAL) {
-            // stable counting sort by cell over ascending .dat index
+            // stable counting sort by (cell, quarter-cell) over ascending .dat index
             for (int k = 0; k < 3; ++k) {
                 const int n = count(k);
-                start[k].assign(P.ncell + 1, 0);
-                for (int i = 0; i < n; ++i) start[k][cell[k][i] + 1]++;
-                for (int c = 0; c < P.ncell; ++c) start[k][c + 1] += start[k][c];
+                const int nk = P.ncell * kSubZ;
+                std::vector<int> fstart(nk + 1, 0);
+                for (int i = 0; i < n; ++i) fstart[cell[k][i] * kSubZ + sub[k][i] + 1]++;
+                for (int c = 0; c < nk; ++c) fstart[c + 1] += fstart[c];
                 sorted[k].assign(n, -1);
-                std::vector<int> fillp(start[k].begin(), start[k].end() - 1);
-                for (int i = 0; i < n; ++i) sorted[k][fillp[cell[k][i]]++] = i;
+                std::vector<int> fillp(fstart.begin(), fstart.end() - 1);
+                for (int i = 0; i < n; ++i) sorted[k][fillp[cell[k][i] * kSubZ + sub[k][i]]++] = i;
+                start[k].assign(P.ncell + 1, 0);
+                for (int c = 0; c <= P.ncell; ++c) start[k][c] = fstart[c * kSubZ];
             }
             return;
         }

@@ -60,15 +60,24 @@ def test_cell_ids_match_the_float32_formula(tiny):
 
 
 def test_canonical_cell_lists_are_a_stable_sort(tiny):
+    """Slots are sorted by cell, then by the quarter-cell z index inside the
+    cell ((int)(4 t) - 4 (int)t, t the float32 z coordinate in cells), then
+    by .dat index; the per-cell starts count each cell's members."""
     sc, p, a = tiny
     o = orc.Oracle(p, 32)
     o.upload(a)
     o.particle_cell_update()
     o.fill_cell()
-    for kind in (FLUID, SOLID, VIRT):
+    cmin = np.float32(p.cell_min[2])
+    inv = np.float32(p.interval_inv[2])
+    for kind, key in ((FLUID, "fluid_position"), (SOLID, "solid_position"), (VIRT, "virt_position")):
         cell = o.cell_ids(kind)
         ids = np.arange(len(cell))
-        np.testing.assert_array_equal(o.sorted_ids(kind), np.lexsort((ids, cell)))
+        tz = (a[key][:, 2].astype(np.float32) - cmin) * inv
+        iz = tz.astype(np.int32)
+        sub = np.clip((tz * np.float32(4)).astype(np.int32) - 4 * iz, 0, 3)
+        assert sub.min() >= 0 and (sub.max() == 3 or len(sub) < 50)
+        np.testing.assert_array_equal(o.sorted_ids(kind), np.lexsort((ids, sub, cell)))
         start = np.concatenate([[0], np.cumsum(np.bincount(cell, minlength=p.cell_number_total))])
         np.testing.assert_array_equal(o.cell_start(kind), start)
 
