@@ -128,7 +128,7 @@ def cpu_baseline(params, arrays, budget_s: float, threads: int) -> dict:
     }
 
 
-def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: float) -> dict:
+def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: float, streams: int = 2) -> dict:
     """Many scenes of uneven size and stiffness: one batched step of all of them
     (one launch per stage, replayed as a HIP graph) against stepping them one
     after another, each on its own (the plain per-scene solve).  Scenes/hour
@@ -156,14 +156,14 @@ def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: flo
     dt = family[0].time_step
     solve_steps = math.ceil(sim_time / dt)
 
-    b = DeviceBatch(autotune=True)
+    b = DeviceBatch(autotune=True, streams=streams)
     b.step(sims, warmup)
     b.synchronize()
     t0 = time.perf_counter()
     b.step(sims, steps)
     b.synchronize()
     batched = (time.perf_counter() - t0) / steps
-    log(f"batch: batched {1e3 * batched:.3f} ms/step for all {n_scenes} scenes, "
+    log(f"batch: batched {1e3 * batched:.3f} ms/step for all {n_scenes} scenes on {streams} launch chains, "
         f"fluid workgroup {b.stats()['fluid_threads']}")
 
     for d in sims:
@@ -197,6 +197,7 @@ def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: flo
         "speedup": round(sequential / batched, 3),
         "particle_steps_per_s_batched": (nf + ns) / batched,
         "plans_built": plans,
+        "launch_chains": streams,
         "fluid_threads_batched": b_threads,
         "bit_identical_to_single_scene": "tests/test_gpu_batch.py",
     }
@@ -429,6 +430,7 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=30)
     ap.add_argument("--batch-seed", type=int, default=0)
+    ap.add_argument("--batch-streams", type=int, default=2, help="concurrent launch chains of the batch (1..8)")
     ap.add_argument("--sim-time", type=float, default=0.5, help="simulated seconds per solve (scenes/hour)")
     ap.add_argument("--requests", type=int, default=16, help="serve workload: number of requests")
     ap.add_argument("--load", type=float, default=0.8, help="serve workload: offered load")
@@ -504,7 +506,7 @@ def main():
 
     if args.workload == "batch":
         r = batch_bench(args.batch_scenes, args.steps or args.batch_steps,
-                        3 if args.warmup is None else args.warmup, args.batch_seed, args.sim_time)
+                        3 if args.warmup is None else args.warmup, args.batch_seed, args.sim_time, args.batch_streams)
         out = {
             "metric": "batched scenes/h",
             "value": r["scenes_per_hour_batched"],
@@ -620,7 +622,8 @@ def main():
     }
     dev.close()
     if not args.no_batch:
-        result["batched"] = batch_bench(args.batch_scenes, args.batch_steps, 3, args.batch_seed, args.sim_time)
+        result["batched"] = batch_bench(args.batch_scenes, args.batch_steps, 3, args.batch_seed, args.sim_time,
+                                        args.batch_streams)
     if rank == 0 and not args.no_cpu_baseline:
         log(f"cpu baseline: oracle FP32 on {args.cpu_threads} threads, ~{args.cpu_budget:.0f}s budget")
         result["cpu_baseline"] = cpu_baseline(params, arrays, args.cpu_budget, args.cpu_threads)

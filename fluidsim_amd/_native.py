@@ -161,6 +161,7 @@ DEVICE_API = {
     "fsa_batch_create": (C.c_int, [_vp, C.c_uint, _P(_vp)]),
     "fsa_batch_destroy": (C.c_int, [_vp]),
     "fsa_batch_step": (C.c_int, [_vp, _P(_vp), C.c_int, C.c_int]),
+    "fsa_batch_set_streams": (C.c_int, [_vp, C.c_int]),
     "fsa_batch_synchronize": (C.c_int, [_vp]),
     "fsa_batch_stats": (C.c_int, [_vp, _P(C.c_int64), _P(C.c_int64), _P(C.c_int)]),
     "fsa_set_fluid_threads": (C.c_int, [_vp, C.c_int]),

@@ -23,6 +23,15 @@
 // needs more).  Sub-step k launches the three solid stages over the scenes
 // with more than k sub-steps; the others get empty ranges.
 //
+// Concurrency: the scenes are split over a few chains (fsa_batch_set_streams,
+// 2 by default), each with its own stream and its own recorded steps.  A step
+// launches every chain's graph; the chains are independent, so while one
+// runs its latency-bound stages (the solid sub-steps launch a few hundred
+// workgroups each, the cell build a string of short passes) the other's
+// fluid pass fills the CUs.  The split is deterministic in the scene list
+// (heaviest first to the least loaded chain), so a list stepped again finds
+// its chains' plans.
+//
 // Launch shape: the fluid pass runs in workgroups of 64, 128 or 256 threads
 // (identical results).  With FSA_BATCH_AUTOTUNE every recording first times
 // the batched fluid pass at each size on the scenes' current state -- the
@@ -45,6 +54,7 @@ using namespace fsa;
 namespace {
 
 constexpr int kXcdRound = 8;  // blocks per round of the XCD dispatcher
+constexpr int kMaxChains = 8;  // concurrent launch chains of a batch
 constexpr size_t kAlign = 256;
 
 inline int round_xcd(int blocks) { return (blocks + kXcdRound - 1) / kXcdRound * kXcdRound; }
@@ -91,14 +101,10 @@ struct StepTables {
     std::vector<Launch> launches;
 };
 
-}  // namespace
-
-struct fsa_batch {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
+// One chain of launches: the scenes it steps, its stream, its plan.
+struct Chain {
+    hipStream_t stream = nullptr;  // owned; default priority
     unsigned flags = 0;
-    hipEvent_t ev_sync = nullptr;
 
     // the plan
     std::vector<uint64_t> serials;
@@ -111,14 +117,26 @@ struct fsa_batch {
     bool planned = false;
     int fluid_threads = 0;  // workgroup size of the recorded fluid launches
     bool packed = false;    // the scenes use the packed precision form (all or none)
-
     int64_t plans_built = 0;
+};
+
+}  // namespace
+
+struct fsa_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;  // the batch's stream: work is ordered after it and it waits for the chains
+    bool own_stream = false;
+    unsigned flags = 0;
+    hipEvent_t ev_sync = nullptr;
+    int nchains = 2;               // concurrent launch chains the scenes are split over
+    std::vector<Chain *> chains;   // created as needed
+    std::vector<hipEvent_t> ev_chain;
     int64_t steps_run = 0;
 };
 
 namespace {
 
-void drop_graphs(fsa_batch *b) {
+void drop_graphs(Chain *b) {
     for (int p = 0; p < 2; ++p) {
         if (b->exec[p]) (void)hipGraphExecDestroy(b->exec[p]);
         if (b->graph[p]) (void)hipGraphDestroy(b->graph[p]);
@@ -219,7 +237,7 @@ void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph
         }
 }
 
-hipError_t enqueue_step(fsa_batch *b, const StepTables &t) {
+hipError_t enqueue_step(Chain *b, const StepTables &t) {
     const char *base = b->dev_blob;
     for (const Launch &l : t.launches) {
         const int *prefix = reinterpret_cast<const int *>(base + l.prefix_off);
@@ -253,7 +271,7 @@ hipError_t enqueue_step(fsa_batch *b, const StepTables &t) {
 
 // the fastest fluid workgroup size for these scenes, timed on their current
 // state (see the file comment); 0 on failure with the error recorded
-int tune_fluid(fsa_batch *b, const std::vector<fsa_sim *> &sims) {
+int tune_fluid(Chain *b, const std::vector<fsa_sim *> &sims) {
     const int n = (int)sims.size();
     std::vector<FluidJob> jobs(n);
     for (int i = 0; i < n; ++i) jobs[i] = FluidJob{sims[i]->P, fluid_args(sims[i])};
@@ -310,7 +328,7 @@ int tune_fluid(fsa_batch *b, const std::vector<fsa_sim *> &sims) {
     return best_t;
 }
 
-int build_plan(fsa_batch *b, const std::vector<fsa_sim *> &sims) {
+int build_plan(Chain *b, const std::vector<fsa_sim *> &sims) {
     drop_graphs(b);
     b->planned = false;
     const int n = (int)sims.size();
@@ -369,7 +387,7 @@ int build_plan(fsa_batch *b, const std::vector<fsa_sim *> &sims) {
 }
 
 // which recorded step matches the scenes' current buffer indices (-1: none)
-int plan_phase(const fsa_batch *b, const std::vector<fsa_sim *> &sims) {
+int plan_phase(const Chain *b, const std::vector<fsa_sim *> &sims) {
     if (!b->planned || sims.size() != b->serials.size()) return -1;
     if (!(b->flags & FSA_BATCH_AUTOTUNE) && sims[0]->fluid_threads != b->fluid_threads) return -1;
     for (size_t i = 0; i < sims.size(); ++i)
@@ -383,6 +401,54 @@ int plan_phase(const fsa_batch *b, const std::vector<fsa_sim *> &sims) {
         if (all) return p;
     }
     return -1;
+}
+
+// a chain of its own, with its own stream (default priority)
+Chain *new_chain(unsigned flags) {
+    Chain *c = new (std::nothrow) Chain;
+    if (!c) return nullptr;
+    c->flags = flags;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    return c;
+}
+
+void free_chain(Chain *c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
+    if (c->dev_blob) (void)hipFree(c->dev_blob);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// Split the scenes over `nch` chains: heaviest first, each to the chain with
+// the least work so far (work ~ fluid particles + 4 x solid particles x
+// sub-steps).  Deterministic in the scene list, so a list stepped again
+// finds its chains' plans.
+std::vector<std::vector<fsa_sim *>> split(const std::vector<fsa_sim *> &sims, int nch) {
+    std::vector<int> order(sims.size());
+    std::vector<double> work(sims.size());
+    for (size_t i = 0; i < sims.size(); ++i) {
+        order[i] = (int)i;
+        work[i] = (double)sims[i]->P.nf + 4.0 * sims[i]->P.ns * sims[i]->P.sub + 1.0;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return work[a] > work[c]; });
+    std::vector<std::vector<fsa_sim *>> part(nch);
+    std::vector<double> load(nch, 0.0);
+    for (int i : order) {
+        const int c = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        part[c].push_back(sims[i]);
+        load[c] += work[i];
+    }
+    // each chain keeps its scenes in list order
+    for (auto &v : part)
+        std::stable_sort(v.begin(), v.end(), [&](fsa_sim *a, fsa_sim *c) {
+            return std::find(sims.begin(), sims.end(), a) < std::find(sims.begin(), sims.end(), c);
+        });
+    return part;
 }
 
 }  // namespace
@@ -421,11 +487,18 @@ int fsa_batch_create(void *stream, unsigned flags, fsa_batch **out) {
 int fsa_batch_destroy(fsa_batch *b) {
     if (!b) return FSA_OK;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    drop_graphs(b);
-    if (b->dev_blob) (void)hipFree(b->dev_blob);
+    for (Chain *c : b->chains) free_chain(c);
+    for (hipEvent_t e : b->ev_chain) (void)hipEventDestroy(e);
     if (b->ev_sync) (void)hipEventDestroy(b->ev_sync);
     if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
+    return FSA_OK;
+}
+
+int fsa_batch_set_streams(fsa_batch *b, int n) {
+    if (!b) return set_err(FSA_ERR_INVALID, "null batch");
+    if (n < 1 || n > kMaxChains) return set_err(FSA_ERR_INVALID, "stream count must be 1..8");
+    b->nchains = n;
     return FSA_OK;
 }
 
@@ -444,40 +517,72 @@ int fsa_batch_step(fsa_batch *b, fsa_sim *const *scenes, int n_scenes, int n_ste
         for (int j = 0; j < i; ++j)
             if (sims[j] == s) return set_err(FSA_ERR_INVALID, "scene listed twice");
     }
-    // the batch stream waits for work already queued on the scenes' streams
+    // one batch launch serves every scene of a chain: the packed form is a
+    // kernel variant, so the whole list must agree on it
+    const bool packed = (sims[0]->P.form & FSA_FORM_PACKED) != 0;
+    for (const fsa_sim *s : sims)
+        if (((s->P.form & FSA_FORM_PACKED) != 0) != packed)
+            return set_err(FSA_ERR_INVALID, "the scenes of one batch must all use the packed form or none");
+    const int nch = std::min(b->nchains, n_scenes);
+    while ((int)b->chains.size() < nch) {
+        Chain *c = new_chain(b->flags);
+        hipEvent_t e = nullptr;
+        if (!c || hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            free_chain(c);
+            return set_err(FSA_ERR_HIP, "batch chain creation failed");
+        }
+        b->chains.push_back(c);
+        b->ev_chain.push_back(e);
+    }
+    // the chains start after the work already queued on the scenes' streams
+    // and on the batch stream
     for (fsa_sim *s : sims) {
         if (s->stream == b->stream) continue;
         HIP_TRY(hipEventRecord(b->ev_sync, s->stream));
         HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_sync, 0));
     }
-    int phase = plan_phase(b, sims);
-    if (phase < 0) {
-        int rc = build_plan(b, sims);
-        if (rc) return rc;
-        phase = 0;
-    }
-    // wall normals are computed once, on a scene's first update (the walls do
-    // not move); the step's own wall pass then reads them
-    for (fsa_sim *s : sims) {
-        if (s->P.nv == 0 || s->vx.normals_ready) continue;
-        HIP_TRY(launch_virt(s->P, virt_args(s, 1), b->stream));
-        s->vx.normals_ready = true;
+    HIP_TRY(hipEventRecord(b->ev_sync, b->stream));
+    const std::vector<std::vector<fsa_sim *>> part = split(sims, nch);
+    std::vector<int> phase(nch, 0);
+    for (int c = 0; c < nch; ++c) {
+        Chain *ch = b->chains[c];
+        HIP_TRY(hipStreamWaitEvent(ch->stream, b->ev_sync, 0));
+        phase[c] = plan_phase(ch, part[c]);
+        if (phase[c] < 0) {
+            int rc = build_plan(ch, part[c]);
+            if (rc) return rc;
+            phase[c] = 0;
+        }
+        // wall normals are computed once, on a scene's first update (the
+        // walls do not move); the step's own wall pass then reads them
+        for (fsa_sim *s : part[c]) {
+            if (s->P.nv == 0 || s->vx.normals_ready) continue;
+            HIP_TRY(launch_virt(s->P, virt_args(s, 1), ch->stream));
+            s->vx.normals_ready = true;
+        }
     }
     for (int k = 0; k < n_steps; ++k) {
-        if (b->exec[phase]) {
-            HIP_TRY(hipGraphLaunch(b->exec[phase], b->stream));
-        } else {
-            HIP_TRY(enqueue_step(b, b->tables[phase]));
+        for (int c = 0; c < nch; ++c) {
+            Chain *ch = b->chains[c];
+            if (ch->exec[phase[c]]) {
+                HIP_TRY(hipGraphLaunch(ch->exec[phase[c]], ch->stream));
+            } else {
+                HIP_TRY(enqueue_step(ch, ch->tables[phase[c]]));
+            }
+            phase[c] ^= 1;
         }
         for (fsa_sim *s : sims) {
             apply_phase(s, next_phase(s, phase_of(s)));
             s->hashed = false;
             s->couple_fresh = s->P.nf > 0 && s->P.ns > 0;
         }
-        phase ^= 1;
         b->steps_run += 1;
     }
-    // the scenes' streams wait for the batch before their next work
+    // the batch stream, then the scenes' streams, wait for the chains
+    for (int c = 0; c < nch; ++c) {
+        HIP_TRY(hipEventRecord(b->ev_chain[c], b->chains[c]->stream));
+        HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_chain[c], 0));
+    }
     HIP_TRY(hipEventRecord(b->ev_sync, b->stream));
     for (fsa_sim *s : sims) {
         if (s->stream == b->stream) continue;
@@ -494,9 +599,11 @@ int fsa_batch_synchronize(fsa_batch *b) {
 
 int fsa_batch_stats(fsa_batch *b, int64_t *plans_built, int64_t *steps_run, int *fluid_threads) {
     if (!b) return set_err(FSA_ERR_INVALID, "null batch");
-    if (plans_built) *plans_built = b->plans_built;
+    int64_t plans = 0;
+    for (const Chain *c : b->chains) plans += c->plans_built;
+    if (plans_built) *plans_built = plans;
     if (steps_run) *steps_run = b->steps_run;
-    if (fluid_threads) *fluid_threads = b->fluid_threads;
+    if (fluid_threads) *fluid_threads = b->chains.empty() ? 0 : b->chains[0]->fluid_threads;
     return FSA_OK;
 }
 

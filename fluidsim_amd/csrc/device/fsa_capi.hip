@@ -118,15 +118,20 @@ int check_params(const fsa_params &p) {
     return FSA_OK;
 }
 
-int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_state, bool compensated = false,
+int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_state, int nsub, bool compensated = false,
                bool packed = false) {
     k.n = n;
+    k.nsub = nsub;
     const int ncell = s->P.ncell;
-    const int nk = ncell * kSubZ;
+    const int nk = ncell * nsub;
     const size_t padded = (size_t)(nk + kScanTile - 1) / kScanTile * kScanTile;
     int rc;
     if ((rc = s->alloc(&k.start, (size_t)ncell + 1))) return rc;
-    if ((rc = s->alloc(&k.fstart, (size_t)nk + 1))) return rc;
+    if (nsub > 1) {
+        if ((rc = s->alloc(&k.fstart, (size_t)nk + 1))) return rc;
+    } else {
+        k.fstart = k.start;  // one key per cell: the per-key starts are the per-cell ones
+    }
     if ((rc = s->alloc(&k.count, padded))) return rc;
     if (n == 0) return FSA_OK;
     for (int b = 0; b < 2; ++b) {
@@ -160,9 +165,9 @@ int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_sta
 int alloc_all(fsa_sim *s) {
     int rc;
     const bool comp = (s->P.form & FSA_FORM_COMPENSATED) != 0, packed = (s->P.form & FSA_FORM_PACKED) != 0;
-    if ((rc = alloc_kind(s, s->fluid, s->P.nf, false, true, comp, packed))) return rc;
-    if ((rc = alloc_kind(s, s->solid, s->P.ns, true, true, comp, false))) return rc;
-    if ((rc = alloc_kind(s, s->virt, s->P.nv, false, false))) return rc;
+    if ((rc = alloc_kind(s, s->fluid, s->P.nf, false, true, kSubZ, comp, packed))) return rc;
+    if ((rc = alloc_kind(s, s->solid, s->P.ns, true, true, 1, comp, false))) return rc;
+    if ((rc = alloc_kind(s, s->virt, s->P.nv, false, false, 1))) return rc;
     const int ns = s->P.ns;
     if ((rc = s->alloc(&s->sx.np, ns)) || (rc = s->alloc(&s->sx.acc, ns)) || (rc = s->alloc(&s->sx.couple, ns)) ||
         (rc = s->alloc(&s->sx.F, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.P, 9 * (size_t)ns)) ||
@@ -183,7 +188,7 @@ int alloc_all(fsa_sim *s) {
 // start arrays of kinds without particles are all zero (every cell empty)
 int zero_starts(fsa_sim *s, KindBuffers &k) {
     HIP_TRY(launch_fill_i32(k.start, 0, s->P.ncell + 1, s->stream));
-    HIP_TRY(launch_fill_i32(k.fstart, 0, s->P.ncell * kSubZ + 1, s->stream));
+    if (k.fstart != k.start) HIP_TRY(launch_fill_i32(k.fstart, 0, s->P.ncell * k.nsub + 1, s->stream));
     return FSA_OK;
 }
 

@@ -48,7 +48,7 @@ struct OpHashImpl {
         int key = -1;
         if (valid) {
             const float4 x = j.pos_in[s];
-            key = cell_key(P, x.x, x.y, x.z);
+            key = cell_key(P, x.x, x.y, x.z, j.nsub);
         }
         // Particles arrive sorted by their previous key, so equal keys come
         // in runs of adjacent lanes.  The first lane of each run (a lane
@@ -67,7 +67,7 @@ struct OpHashImpl {
         if (valid && lane == head) base = atomicAdd(&j.count[key], next - lane);
         base = __shfl(base, head, 64);
         if (!valid) return;
-        const int c = key / kSubZ;
+        const int c = key / j.nsub;
         j.key[s] = key;
         j.rank[s] = base + (lane - head);
         if (j.count_jumps) {
@@ -164,7 +164,7 @@ struct OpScanAddImpl {
 struct OpCellStartsImpl {
     static __device__ void run(const CellJob &j, int blk, int) {
         const int c = blk * kBlock + threadIdx.x;
-        if (c <= j.P.ncell) j.start[c] = j.fstart[c * kSubZ];
+        if (c <= j.P.ncell) j.start[c] = j.fstart[c * j.nsub];
     }
 };
 
@@ -185,7 +185,7 @@ struct OpGatherImpl {
         const int s = j.tmp_slot[t];
         const int my = j.tmp_id[t];
         const int k = j.key[s];
-        const int c = k / kSubZ;
+        const int c = k / j.nsub;
         const int b = j.fstart[k], e = j.fstart[k + 1];
         int r = 0;
         for (int u = b; u < e; ++u) r += j.tmp_id[u] < my ? 1 : 0;
@@ -221,7 +221,7 @@ int blocks_cells(BatchOp op, const CellJob &j) {
     case kBScanTiles: return scan_tiles(j.nk);
     case kBScanSums: return 1;
     case kBScanAdd: return blocks_for(j.nk);
-    case kBCellStarts: return blocks_for((long long)j.P.ncell + 1);
+    case kBCellStarts: return j.nsub > 1 ? blocks_for((long long)j.P.ncell + 1) : 0;  // nsub 1: the same array
     case kBPlace: return blocks_for(j.n);
     case kBGather: return blocks_for(j.n);
     default: return 0;
@@ -252,7 +252,8 @@ CellJob make_cell_job(const DevParams &P, const KindBuffers &k, unsigned long lo
     CellJob j{};
     j.P = P;
     j.n = k.n;
-    j.nk = P.ncell * kSubZ;
+    j.nsub = k.nsub;
+    j.nk = P.ncell * k.nsub;
     j.has_ref = k.ref[0] != nullptr ? 1 : 0;
     j.count_jumps = count_jumps ? 1 : 0;
     const int so = k.st, sn = 1 - k.st, oo = k.ord, on = 1 - k.ord;
@@ -300,7 +301,7 @@ hipError_t launch_sort_one(const CellJob &j, hipStream_t s) {
     if ((e = one<OpScanTilesImpl>(j, blocks_cells(kBScanTiles, j), s)) != hipSuccess) return e;
     if ((e = one<OpScanSumsImpl>(j, blocks_cells(kBScanSums, j), s)) != hipSuccess) return e;
     if ((e = one<OpScanAddImpl>(j, blocks_cells(kBScanAdd, j), s)) != hipSuccess) return e;
-    if ((e = one<OpCellStartsImpl>(j, blocks_cells(kBCellStarts, j), s)) != hipSuccess) return e;
+    if ((e = one<OpCellStartsImpl>(j, blocks_cells(kBCellStarts, j), s)) != hipSuccess) return e;  // none if nsub 1
     if ((e = one<OpPlaceImpl>(j, blocks_cells(kBPlace, j), s)) != hipSuccess) return e;
     return one<OpGatherImpl>(j, blocks_cells(kBGather, j), s);
 }

@@ -41,10 +41,11 @@ constexpr int kScanTile = 1024;  // elements per scan workgroup (256 threads x 4
 // results past the range are masked out of every test)
 constexpr int kPosPad = 16;
 constexpr int kSolidLanes = 16;  // lanes serving one solid particle
-// Inside a cell, particles are sorted by a quarter-cell z index (then by
-// .dat index): the sort key is cell * kSubZ + sub.  The per-cell start
-// arrays every pass uses are taken from the per-key ones, which the fluid
-// pass also reads to test only the z-range of a column its cut-off reaches.
+// Inside a cell, fluid particles are sorted by a quarter-cell z index (then
+// by .dat index): their sort key is cell * kSubZ + sub.  The per-cell start
+// array every pass uses is taken from the per-key one, which the fluid pass
+// also reads to test only the z-range of a column its cut-off reaches.
+// Solid and wall particles keep one key per cell (cell, then .dat index).
 constexpr int kSubZ = 4;
 // solid-solid partners recorded per lane in the first sub-step of a step
 // (entries per lane; a lane with more partners walks its candidates instead)
@@ -94,9 +95,10 @@ struct KindBuffers {
     float4 *ref[2] = {nullptr, nullptr};  // solid only
     int32_t *id[2] = {nullptr, nullptr};
     int32_t *cell[2] = {nullptr, nullptr};
+    int nsub = 1;               // sort keys per cell: kSubZ (fluid) or 1
     int32_t *start = nullptr;   // ncell + 1: slots of cell c are [start[c], start[c+1])
-    int32_t *fstart = nullptr;  // ncell * kSubZ + 1: slots of key k are [fstart[k], fstart[k+1])
-    int32_t *count = nullptr;   // ncell * kSubZ padded to kScanTile
+    int32_t *fstart = nullptr;  // ncell * nsub + 1: slots of key k are [fstart[k], fstart[k+1]) (start if nsub is 1)
+    int32_t *count = nullptr;   // ncell * nsub padded to kScanTile
     int32_t *key = nullptr;    // cell of each slot of the current order
     int32_t *rank = nullptr;   // arrival slot inside the cell (atomic)
     int32_t *tmp_slot = nullptr;
@@ -167,7 +169,8 @@ __global__ __launch_bounds__(THREADS) void k_many(const Job *__restrict__ jobs, 
 struct CellJob {
     DevParams P;
     int n;
-    int nk;  // keys: ncell * kSubZ
+    int nsub;  // keys per cell: kSubZ or 1
+    int nk;    // keys: ncell * nsub
     int has_ref;
     int count_jumps;
     const float4 *pos_in, *vel_in, *ref_in;

@@ -30,15 +30,16 @@ __device__ __forceinline__ int find_cell(const DevParams &P, float x, float y, f
     return (ix * P.cn[1] + iy) * P.cn[2] + iz;
 }
 
-// sort key of a position: cell * kSubZ + the quarter-cell z index inside
-// the cell, (int)(t kSubZ) - kSubZ (int)t with t the z coordinate in cells
-// (kSubZ t is exact), clamped like the cell indices
-__device__ __forceinline__ int cell_key(const DevParams &P, float x, float y, float z) {
+// sort key of a position with nsub keys per cell: cell * nsub + the
+// 1/nsub-cell z index inside the cell, (int)(t nsub) - nsub (int)t with t
+// the z coordinate in cells (nsub t is exact for nsub = 1, 4), clamped like
+// the cell indices
+__device__ __forceinline__ int cell_key(const DevParams &P, float x, float y, float z, int nsub) {
     const int c = find_cell(P, x, y, z);
     const float tz = (z - P.cmin[2]) * P.inv[2];
     const int iz = min(max((int)tz, 0), P.cn[2] - 1);
-    const int sub = min(max((int)(tz * (float)kSubZ) - kSubZ * iz, 0), kSubZ - 1);
-    return c * kSubZ + sub;
+    const int sub = min(max((int)(tz * (float)nsub) - nsub * iz, 0), nsub - 1);
+    return c * nsub + sub;
 }
 
 // cell coordinates (ix, iy, iz) of cell id c = (ix ny + iy) nz + iz
@@ -122,6 +123,29 @@ __device__ __forceinline__ void for_each_column(const DevParams &P, int c, F &&b
             const int base = (gx * ny + gy) * nz;
             body(base + zlo, base + zhi + 1);
         }
+    }
+}
+
+// The slot ranges [b[k], e[k]) of the 9 columns of cell c (the order of
+// for_each_column; empty outside the grid) in the starts of one kind.  All
+// 18 loads are issued together, where a column-by-column walk waits for
+// each column's pair in turn.
+__device__ __forceinline__ void column_ranges(const DevParams &P, int c, const int32_t *__restrict__ start,
+                                              int (&b)[9], int (&e)[9]) {
+    const int ny = P.cn[1], nz = P.cn[2];
+    const int nyz = ny * nz;
+    const int cx = c / nyz;
+    const int rem = c - cx * nyz;
+    const int cy = rem / nz;
+    const int cz = rem - cy * nz;
+    const int zlo = max(cz - 1, 0), zhi1 = min(cz + 1, nz - 1) + 1;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int gx = cx + k / 3 - 1, gy = cy + k % 3 - 1;
+        const bool in = (unsigned)gx < (unsigned)P.cn[0] && (unsigned)gy < (unsigned)ny;
+        const int base = (gx * ny + gy) * nz;
+        b[k] = start[in ? base + zlo : 0];
+        e[k] = start[in ? base + zhi1 : 0];
     }
 }
 

@@ -26,26 +26,36 @@ __device__ __forceinline__ void virt_body(const DevParams &P, const VirtArgs &a,
     float pressure = 0.0f, density = P.rho0, nearest = kLargeFloat;
     int best = -1;
 
-    for_each_column(P, a.vcell[i], [&](int c0, int c1) {
-        // nearest fluid particle: strict <, so the first in traversal order
-        // wins ties; kScan loads in flight, reads past the run end stay in
-        // the padded array and are masked
+    const int cell = a.vcell[i];
+    // nearest fluid particle over the 9 columns in order: strict <, so the
+    // first in traversal order wins ties; kScan loads in flight, reads past
+    // the run end stay in the padded array and are masked.  Most walls have
+    // no fluid in reach: their column ranges come back empty in one round.
+    {
+        int cb[9], ce[9];
+        column_ranges(P, cell, a.fstart, cb, ce);
         constexpr int kScan = 8;
         static_assert(kPosPad >= kScan, "padded position arrays");
-        for (int j = a.fstart[c0], e = a.fstart[c1]; j < e; j += kScan) {
-            float4 q[kScan];
-            const float4 *src = a.fpos + j;
+#pragma unroll 1
+        for (int col = 0; col < 9; ++col) {
+            for (int j = cb[col], e = ce[col]; j < e; j += kScan) {
+                float4 q[kScan];
+                const float4 *src = a.fpos + j;
 #pragma unroll
-            for (int k = 0; k < kScan; ++k) q[k] = src[k];
+                for (int k = 0; k < kScan; ++k) q[k] = src[k];
 #pragma unroll
-            for (int k = 0; k < kScan; ++k) {
-                const float d2 = len2(q[k].x - xi, q[k].y - yi, q[k].z - zi);
-                const bool better = (j + k < e) & (d2 < nearest);
-                nearest = better ? d2 : nearest;
-                best = better ? j + k : best;
+                for (int k = 0; k < kScan; ++k) {
+                    const float d2 = len2(q[k].x - xi, q[k].y - yi, q[k].z - zi);
+                    const bool better = (j + k < e) & (d2 < nearest);
+                    nearest = better ? d2 : nearest;
+                    best = better ? j + k : best;
+                }
             }
         }
-        if (NORMALS) {
+    }
+    if (NORMALS) {
+        // normal from the other wall particles in reach (first update only)
+        for_each_column(P, cell, [&](int c0, int c1) {
             for (int j = a.vstart[c0], e = a.vstart[c1]; j < e; ++j) {
                 const float4 pj = a.vpos_in[j];
                 const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
@@ -57,8 +67,8 @@ __device__ __forceinline__ void virt_body(const DevParams &P, const VirtArgs &a,
                     nz += (-dz) * kc;
                 }
             }
-        }
-    });
+        });
+    }
 
     if (best >= 0) {
         const float4 pj = a.fpos[best];

@@ -210,12 +210,15 @@ class DeviceBatch:
     (fsa_batch_* of include/fluidsim_amd.h).  The list of contexts is given on
     every call and may change between calls."""
 
-    def __init__(self, stream: int | None = None, graphs: bool = True, autotune: bool = False):
+    def __init__(self, stream: int | None = None, graphs: bool = True, autotune: bool = False,
+                 streams: int | None = None):
         self.lib = _native.device_lib()
         h = C.c_void_p()
         flags = (0 if graphs else _native.FSA_BATCH_NO_GRAPH) | (_native.FSA_BATCH_AUTOTUNE if autotune else 0)
         check(self.lib.fsa_batch_create(C.c_void_p(stream) if stream else None, flags, C.byref(h)))
         self.handle = h
+        if streams is not None:
+            check(self.lib.fsa_batch_set_streams(self.handle, streams))
 
     def close(self):
         if getattr(self, "handle", None):
@@ -237,6 +240,10 @@ class DeviceBatch:
     def step(self, scenes, n: int = 1):
         arr = (C.c_void_p * max(1, len(scenes)))(*[s.handle for s in scenes])
         check(self.lib.fsa_batch_step(self.handle, arr, len(scenes), n))
+
+    def set_streams(self, n: int):
+        """Concurrent launch chains the scenes are split over (1..8)."""
+        check(self.lib.fsa_batch_set_streams(self.handle, n))
 
     def synchronize(self):
         check(self.lib.fsa_batch_synchronize(self.handle))

@@ -322,7 +322,12 @@ int fsa_device_memory(int64_t *free_bytes, int64_t *total_bytes);
  * the batch's device.  The batch stream first waits for work queued on each
  * context's stream, and each context's stream then waits for the batch, so
  * stage calls and downloads on a context stay ordered after the batch step.
- * Per-stage timing and FSA_DEBUG_PAIR_SUMS are not recorded in a batch. */
+ * Per-stage timing and FSA_DEBUG_PAIR_SUMS are not recorded in a batch.
+ *
+ * The contexts are split over a few launch chains (2 by default), each with
+ * its own stream and recording, so that one chain's latency-bound stages
+ * (solid sub-steps, cell build) overlap another chain's fluid pass.  The
+ * split is deterministic in the list; results do not depend on it. */
 typedef struct fsa_batch fsa_batch;
 
 #define FSA_BATCH_NO_GRAPH 1u /* launch every stage directly instead of replaying a HIP graph */
@@ -334,6 +339,10 @@ int fsa_batch_create(void *stream, unsigned flags, fsa_batch **out);
 int fsa_batch_destroy(fsa_batch *batch);
 /* n_steps full steps of every listed context (no context listed twice). */
 int fsa_batch_step(fsa_batch *batch, fsa_sim *const *scenes, int n_scenes, int n_steps);
+/* Number of concurrent launch chains (streams) the contexts are split over,
+ * 1..8 (default 2); takes effect on the next fsa_batch_step.  The batch
+ * scheme has no counterpart in the reference (one scene per process). */
+int fsa_batch_set_streams(fsa_batch *batch, int n);
 int fsa_batch_synchronize(fsa_batch *batch);
 /* How often the launches were (re)recorded and steps run since creation, and
  * the fluid workgroup size of the current recording (0 before the first).

@@ -5,10 +5,11 @@
 //
 // Templated on the arithmetic type T (float = the reference's FP32, double =
 // the FP64 precision oracle) and on the neighbour traversal:
-//   CANONICAL  cells sorted by (cell id, quarter-cell z index inside the cell,
-//              .dat index) -- the quarter-cell index is (int)(4 t) - 4 (int)t
-//              with t = (z - cell_min_z) / interval_z, the key order of the
-//              GPU's cell build; for a particle in cell
+//   CANONICAL  fluid sorted by (cell id, quarter-cell z index inside the
+//              cell, .dat index) -- the quarter-cell index is (int)(4 t) -
+//              4 (int)t with t = (z - cell_min_z) / interval_z -- and solid
+//              and wall particles by (cell id, .dat index), the key order of
+//              the GPU's cell build; for a particle in cell
 //              (cx,cy,cz) the partners are walked kind by kind (fluid, solid,
 //              virt), and for each kind the 27 neighbour cells as 9 columns
 //              (ox,oy ascending), each column as the contiguous z-run
@@ -454,9 +455,10 @@ template <class T> struct Oracle : OracleBase {
 #pragma omp parallel for schedule(static)
             for (int i = 0; i < n; ++i) {
                 cell[k][i] = find_cell(x[i]);
+                const int ns = k == KF ? kSubZ : 1;
                 const T tz = (x[i].z - P.cmin[2]) * P.inv[2];
                 const int iz = std::min(std::max((int)tz, 0), P.cn[2] - 1);
-                sub[k][i] = std::min(std::max((int)(tz * T(kSubZ)) - kSubZ * iz, 0), kSubZ - 1);
+                sub[k][i] = std::min(std::max((int)(tz * T(ns)) - ns * iz, 0), ns - 1);
             }
         }
     }
@@ -470,15 +472,16 @@ template <class T> struct Oracle : OracleBase {
             // stable counting sort by (cell, quarter-cell) over ascending .dat index
             for (int k = 0; k < 3; ++k) {
                 const int n = count(k);
-                const int nk = P.ncell * kSubZ;
+                const int ns = k == KF ? kSubZ : 1;
+                const int nk = P.ncell * ns;
                 std::vector<int> fstart(nk + 1, 0);
-                for (int i = 0; i < n; ++i) fstart[cell[k][i] * kSubZ + sub[k][i] + 1]++;
+                for (int i = 0; i < n; ++i) fstart[cell[k][i] * ns + sub[k][i] + 1]++;
                 for (int c = 0; c < nk; ++c) fstart[c + 1] += fstart[c];
                 sorted[k].assign(n, -1);
                 std::vector<int> fillp(fstart.begin(), fstart.end() - 1);
-                for (int i = 0; i < n; ++i) sorted[k][fillp[cell[k][i] * kSubZ + sub[k][i]]++] = i;
+                for (int i = 0; i < n; ++i) sorted[k][fillp[cell[k][i] * ns + sub[k][i]]++] = i;
                 start[k].assign(P.ncell + 1, 0);
-                for (int c = 0; c <= P.ncell; ++c) start[k][c] = fstart[c * kSubZ];
+                for (int c = 0; c <= P.ncell; ++c) start[k][c] = fstart[c * ns];
             }
             return;
         }

@@ -66,37 +66,40 @@ def _assert_same(name, got, want):
                               w.view(np.uint32) if w.dtype == np.float32 else w), f"{name}: {k} differs"
 
 
-@pytest.mark.parametrize("graphs", [True, False], ids=["graph", "direct"])
-def test_batched_steps_are_bit_identical_to_single_steps(gpu_available, graphs):
+@pytest.mark.parametrize("graphs,streams", [(True, 1), (True, 2), (False, 2), (True, 3)],
+                         ids=["graph-1", "graph-2", "direct-2", "graph-3"])
+def test_batched_steps_are_bit_identical_to_single_steps(gpu_available, graphs, streams):
+    """The scenes are split over `streams` launch chains, each recording its
+    own steps; plans_built counts the chains' recordings."""
     cases = _variants()
     alone = {n: _make(sc) for n, sc in cases}
     mixed = {n: _make(sc) for n, sc in cases}
     names = [n for n, _ in cases]
-    print("case batch", "graphs" if graphs else "direct", names, flush=True)
-    with DeviceBatch(graphs=graphs) as b:
-        # 1) all scenes, odd count: both recorded steps of the plan run
+    print("case batch", "graphs" if graphs else "direct", streams, names, flush=True)
+    with DeviceBatch(graphs=graphs, streams=streams) as b:
+        # 1) all scenes, odd count: both recorded steps of every chain's plan run
         b.step([mixed[n] for n in names], 3)
         for n in names:
             alone[n].step(3)
-        assert b.stats()["plans_built"] == 1 and b.stats()["steps_run"] == 3
-        # 2) same list again: the plan continues at the right recorded step
+        assert b.stats()["plans_built"] == streams and b.stats()["steps_run"] == 3
+        # 2) same list again: the plans continue at the right recorded step
         b.step([mixed[n] for n in names], 2)
         for n in names:
             alone[n].step(2)
-        assert b.stats()["plans_built"] == 1
-        # 3) one scene stepped on its own: its buffers moved, the plan is stale
+        assert b.stats()["plans_built"] == streams
+        # 3) one scene stepped on its own: its buffers moved, its chain's plan is stale
         mixed["base"].step(1)
         alone["base"].step(1)
         b.step([mixed[n] for n in names], 1)
         for n in names:
             alone[n].step(1)
-        assert b.stats()["plans_built"] == 2
+        assert b.stats()["plans_built"] == streams + 1
         # 4) continuous batching: a scene leaves, the order changes
         sub = ["bigger", "no_walls", "stiff_s6", "base", "no_fluid"]
         b.step([mixed[n] for n in sub], 3)
         for n in sub:
             alone[n].step(3)
-        assert b.stats()["plans_built"] == 3
+        assert streams + 2 <= b.stats()["plans_built"] <= 2 * streams + 1
         b.synchronize()
     for n in names:
         _assert_same(n, _state(mixed[n]), _state(alone[n]))
@@ -111,6 +114,13 @@ def test_batch_of_one_matches_and_reuses_its_plan(gpu_available):
         ref.step(4)
         assert b.stats()["plans_built"] == 1 and b.stats()["steps_run"] == 4
     _assert_same("single", _state(one), _state(ref))
+
+
+def test_batch_stream_count_is_checked(gpu_available):
+    with DeviceBatch() as b:
+        for bad in (0, 9):
+            with pytest.raises(FsaError, match="1..8"):
+                b.set_streams(bad)
 
 
 def test_batch_errors_are_status_codes(gpu_available):
@@ -167,7 +177,7 @@ def test_batch_autotune_matches_single_steps(gpu_available):
     with DeviceBatch(autotune=True) as b:
         b.step(mixed, 3)
         st = b.stats()
-        assert st["plans_built"] == 1 and st["fluid_threads"] in (64, 128, 256)
+        assert st["plans_built"] == 2 and st["fluid_threads"] in (64, 128, 256)  # two chains
     for d in alone:
         d.step(3)
     for (n, _), x, y in zip(cases, mixed, alone):
