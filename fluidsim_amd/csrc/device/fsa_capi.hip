@@ -93,6 +93,8 @@ namespace {
 
 int check_params(const fsa_params &p) {
     if (p.n_fluid < 0 || p.n_solid < 0 || p.n_virt < 0) return set_err(FSA_ERR_INVALID, "negative particle count");
+    if (p.n_fluid > kMaxKindParticles || p.n_solid > kMaxKindParticles || p.n_virt > kMaxKindParticles)
+        return set_err(FSA_ERR_INVALID, "more than 268,435,440 particles of one kind");
     for (int a = 0; a < 3; ++a)
         if (p.cell_number[a] <= 0) return set_err(FSA_ERR_INVALID, "cell_number must be positive on every axis");
     const int64_t total = (int64_t)p.cell_number[0] * p.cell_number[1] * p.cell_number[2];
@@ -137,8 +139,14 @@ int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_sta
     for (int b = 0; b < 2; ++b) {
         if (b == 0 || double_state) {
             if ((rc = s->alloc(&k.pos[b], (size_t)n + kPosPad))) return rc;
-            if (!s->dry) HIP_TRY(hipMemsetAsync(k.pos[b], 0, sizeof(float4) * ((size_t)n + kPosPad), s->stream));
-            if ((rc = s->alloc(&k.vel[b], n))) return rc;
+            if ((rc = s->alloc(&k.vel[b], (size_t)n + kPosPad))) return rc;
+            if (!s->dry) {
+                // the padding entries hold the sentinel partner
+                HIP_TRY(hipMemsetAsync(k.pos[b], 0, sizeof(float4) * (size_t)n, s->stream));
+                HIP_TRY(launch_fill_f4(k.pos[b] + n, make_float4(kFarPos, kFarPos, kFarPos, s->P.rho0), kPosPad,
+                                       s->stream));
+                HIP_TRY(hipMemsetAsync(k.vel[b], 0, sizeof(float4) * ((size_t)n + kPosPad), s->stream));
+            }
             if (compensated) {
                 if ((rc = s->alloc(&k.cpos[b], n)) || (rc = s->alloc(&k.cvel[b], n))) return rc;
                 // the upload's state carries no rounding error yet
@@ -169,7 +177,8 @@ int alloc_all(fsa_sim *s) {
     if ((rc = alloc_kind(s, s->solid, s->P.ns, true, true, 1, comp, false))) return rc;
     if ((rc = alloc_kind(s, s->virt, s->P.nv, false, false, 1))) return rc;
     const int ns = s->P.ns;
-    if ((rc = s->alloc(&s->sx.np, ns)) || (rc = s->alloc(&s->sx.acc, ns)) || (rc = s->alloc(&s->sx.couple, ns)) ||
+    if ((rc = s->alloc(&s->sx.np, (size_t)ns + kPosPad)) || (rc = s->alloc(&s->sx.acc, ns)) ||
+        (rc = s->alloc(&s->sx.couple, ns)) ||
         (rc = s->alloc(&s->sx.F, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.P, 9 * (size_t)ns)) ||
         (rc = s->alloc(&s->sx.L, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.PL, 3 * (size_t)ns)) ||
         (rc = s->alloc(&s->sx.A, 9 * (size_t)ns)) ||
@@ -178,6 +187,8 @@ int alloc_all(fsa_sim *s) {
         (rc = s->alloc(&s->sx.fnbr, (size_t)ns * kSolidListCap * kSolidLanes)) ||
         (rc = s->alloc(&s->sx.fcnt, (size_t)ns * kSolidLanes)) || (rc = s->alloc(&s->sx.x0, ns)))
         return rc;
+    // the sentinel solid partner's normal and pressure are zero
+    if (!s->dry && ns > 0) HIP_TRY(hipMemsetAsync(s->sx.np, 0, sizeof(float4) * ((size_t)ns + kPosPad), s->stream));
     if ((rc = s->alloc(&s->vx.nrm, s->P.nv)) || (rc = s->alloc(&s->virt_x, s->P.nv))) return rc;
     if ((rc = s->alloc(&s->jumps, 1))) return rc;
     if ((rc = s->alloc(&s->obs_partial, (size_t)observe_partials())) || (rc = s->alloc(&s->obs_out, kObserveSums)))

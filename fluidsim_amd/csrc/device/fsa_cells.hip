@@ -4,7 +4,7 @@
 // step by walking the previous lists of the 27 neighbour cells with one
 // thread per cell (src/pair.cu:72-207): a latency-bound pointer chase.  Here
 // the cell list is a counting sort of the particles by key, the cell id
-// (findPositionCell, src/pair.cu:8-24) times kSubZ plus the quarter-cell z
+// (findPositionCell, src/pair.cu:8-24) times kSubZ plus the eighth-cell z
 // index inside the cell:
 //   1. fill+hash   zero the per-key counts; one thread per particle: key
 //                  and an arrival rank among its key (one atomic per run of
@@ -27,6 +27,11 @@ namespace fsa {
 namespace {
 
 __global__ __launch_bounds__(kBlock) void k_fill_i32(int32_t *__restrict__ p, int32_t v, int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_f4(float4 *__restrict__ p, float4 v, int n) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) p[i] = v;
 }
@@ -287,6 +292,12 @@ CellJob make_cell_job(const DevParams &P, const KindBuffers &k, unsigned long lo
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n)), dim3(kBlock), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f4(float4 *p, float4 v, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f4, dim3(blocks_for(n)), dim3(kBlock), 0, s, p, v, n);
     return hipGetLastError();
 }
 

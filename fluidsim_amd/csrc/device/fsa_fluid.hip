@@ -19,9 +19,10 @@
 // little across a wave, while its count at any point of the candidate walk
 // varies a lot (the walk visits the columns in a fixed order and the lanes'
 // particles sit at different places in their cells).  Entries are 16 bits:
-// a run-base slot (4 bits, the run's first slot is kept in LDS beside the
-// list) and the offset inside the run (12 bits); a run longer than 4096 slots
-// continues in spare base slots.
+// the offset inside the run (12 bits, high) and a run-base slot (4 bits,
+// low; the run's first slot is kept in LDS beside the list), so that the
+// high part is the offset in bytes of 16-byte records; a run longer than
+// 4096 slots continues in spare base slots.
 //
 // Pair arithmetic is FP32 with the hardware reciprocal, reciprocal square
 // root and exp2, and the algebraically reduced forms noted inline; the
@@ -45,13 +46,15 @@ struct Self {
     float x, y, z, rho, ux, uy, uz, p;
     float inv_rho, two_rho;
     float m2_inv_rho;    // -2 / rho_i
-    float nvisc;         // -10 mu / rho_i
+    float vis5;          // 5 mu (0 for an inviscid fluid)
     float half_inv_rho;  // 1 / (rho_i + rho_j) when rho_j = rho_i (solid partners)
     // packed form: own slot and cell, position relative to the cell centre in cells
     int i, cell;
     float fx, fy, fz;
 };
 
+// The sums leave out the particle's own factors, applied once at the end:
+// drho/dt = 2 rho_i drho, a = -2/rho_i (ax, ay, az).
 struct Acc {
     float drho, ax, ay, az;
 };
@@ -67,45 +70,46 @@ __device__ __forceinline__ float exp2_hw(float x) { return __builtin_amdgcn_exp2
 //   a  += -2 p*/rho_i dx k + [viscid] 10 mu ((u_j-u_i).dx)/(|dx|^2+0.01h^2)/rho_i dx k
 //   rho' += 2 rho_i (u_i - v*).dx k
 // evaluated with A = rho_j/(rho_i+rho_j) (1/2 for a solid partner, whose
-// rho_j is rho_i): p* = p_j + A (p_i - p_j) + [du>=0] rho_i A du D, and
-// (u_i - v*).dx = A (u_i-u_j).dx - (p_i-p_j)/(rho_i+rho_j)/c0 |dx|.
-// k = kd exp(-|dx|^2/h^2) = exp2(log2(kd) - |dx|^2 log2(e)/h^2); a padding
-// slot of the pair loop comes with d2 = kPadD2, whose k is exactly 0 (and
-// every other factor finite), so its terms are exact zeros.
+// rho_j is rho_i): p* = p_j + A (p_i - p_j) + rho_i A max(du, 0) D, and
+// (u_i - v*).dx = A (u_i-u_j).dx - (p_i-p_j)/(rho_i+rho_j)/c0 |dx|; the
+// acceleration term is -2/rho_i [p* + 5 mu (u_i-u_j).dx/(|dx|^2+0.01h^2)] dx k.
+// k = kd exp(-|dx|^2/h^2) = exp2(log2(kd) - |dx|^2 log2(e)/h^2).  A padding
+// slot of the pair loop holds the sentinel partner (or comes with d2 =
+// kPadD2 in the packed form): its k is exactly 0 and every other factor
+// finite, so its terms are exact zeros.  The self pair (d2 = 0) takes
+// 1/|dx| = rsq(FLT_MIN), finite, so its du and |dx| are 0 as well.
 template <int KIND>
 __device__ __forceinline__ void pair_term(const DevParams &P, const Self &me, float dx, float dy, float dz, float d2,
                                           float ujx, float ujy, float ujz, float rhoj, float pj, Acc &acc) {
     const float kc = exp2_hw(fmaf(-d2, P.kexp2, P.log2kd));
-    const float r = d2 != 0.0f ? rsq(d2) : 0.0f;  // 1/|dx|, e = dx r
+    const float r = rsq(fmaxf(d2, 1.17549435e-38f));  // 1/|dx|, e = dx r
     const float inv_rsum = KIND == kFluidSolid ? me.half_inv_rho : rcp(me.rho + rhoj);
     const float A = KIND == kFluidSolid ? 0.5f : rhoj * inv_rsum;
     const float dpij = me.p - pj;
     const float udx = (me.ux - ujx) * dx + (me.uy - ujy) * dy + (me.uz - ujz) * dz;  // (u_i-u_j).dx
-    const float du = udx * r;
-    float ps = fmaf(A, dpij, pj);
+    const float du = fmaxf(udx * r, 0.0f);  // the dissipation acts on approaching pairs only
     const float diss = KIND == kFluidSolid ? P.cs : fminf(3.0f * du, P.cs);
-    const float raise = me.rho * A * du * diss;
-    ps += du >= 0.0f ? raise : 0.0f;
+    float ps = fmaf(me.rho * A * du, diss, fmaf(A, dpij, pj));
     if (KIND == kFluidVirt) ps = fmaxf(ps, 0.0f);
-    float coef = ps * me.m2_inv_rho;
-    if (P.viscid) coef = fmaf(me.nvisc * udx, rcp(d2 + P.eps_vis), coef);
+    float coef = fmaf(me.vis5 * udx, rcp(d2 + P.eps_vis), ps);
     coef *= kc;
     acc.ax += dx * coef;
     acc.ay += dy * coef;
     acc.az += dz * coef;
     const float dpc = dpij * inv_rsum * P.cs_inv;
-    acc.drho += me.two_rho * kc * (A * udx - dpc * (d2 * r));
+    acc.drho += kc * (A * udx - dpc * (d2 * r));
 }
 
-// squared distance given to padding slots: exp2 of its (negated, scaled)
-// value is 0, its square root and reciprocal square root stay finite
+// squared distance given to padding slots of the packed form: exp2 of its
+// (negated, scaled) value is 0, its square root and reciprocal square root
+// stay finite
 constexpr float kPadD2 = 1.0e30f;
 
 template <int KIND>
 __device__ __forceinline__ void one_pair(const DevParams &P, const Self &me, const float4 &pj, const float4 &vj,
-                                         const float4 &n, bool valid, Acc &acc) {
+                                         const float4 &n, Acc &acc) {
     const float dx = pj.x - me.x, dy = pj.y - me.y, dz = pj.z - me.z;
-    const float d2 = valid ? len2(dx, dy, dz) : kPadD2;
+    const float d2 = len2(dx, dy, dz);
     if (KIND == kFluidSolid) {
         // mirrored solid velocity (fluid.cu:79-83); rho_j = rho_i, p_j = p_s
         const float m = 2.0f * ((me.ux - vj.x) * n.x + (me.uy - vj.y) * n.y + (me.uz - vj.z) * n.z);
@@ -143,37 +147,58 @@ __device__ __forceinline__ void one_pair_packed(const DevParams &P, const Self &
 // workgroup sizes of the fluid pass (one to four waves; LDS per wave bounds
 // the occupancy, so the best size depends on the scene: fsa_autotune_fluid)
 constexpr int kThreadChoices[] = {64, 128, 256};
-constexpr int kListCap = 96;    // 16-bit list entries per lane
+// LDS per lane: (kListCap + kListPad) 16-bit entries + kBaseSlots 32-bit bases = 240 B, so that ten wave64s fit a
+// CU's 160 KiB (a lane more of either, and LDS allocation granules leave room for nine)
+constexpr int kListCap = 89;    // 16-bit list entries per lane
+constexpr int kListPad = 7;     // sentinel entries past the last one (the rounds and their prefetch read them)
 constexpr int kRunBits = 12;    // offset of an entry inside its run
 constexpr int kRunMax = 1 << kRunBits;
 constexpr int kColumns = 9;     // neighbour columns; column c's run starts in base slot c
-constexpr int kBaseSlots = 12;  // run bases per lane (4-bit ids): the columns, then spares for long runs
+constexpr int kBaseSlots = 12;  // run bases per lane (4-bit ids): the columns, spares for long runs, the sentinel
+constexpr int kSentinelSlot = kBaseSlots - 1;  // holds the kind's particle count: the first padding slot
+constexpr int kSpareSlots = kSentinelSlot - kColumns;
 constexpr int kPairBatch = 4;   // partners per round of the pair loop (the next round is gathered meanwhile)
 constexpr int kScanBatch = 8;   // candidate loads in flight in the test loop
+constexpr uint16_t kSentinelEntry = kSentinelSlot;  // offset 0 of the sentinel's run
 static_assert(kPosPad >= kScanBatch, "candidate batches may read kScanBatch-1 entries past a range");
 static_assert(kBaseSlots <= 16 && kBaseSlots * kRunMax <= 65536, "entries are 16 bits");
+static_assert(kListPad >= 2 * kPairBatch - 1, "a round and the prefetch of the next read up to 2 kPairBatch - 1 "
+                                              "entries past the last one");
+
+// The run bases in LDS are byte offsets of the run's first slot in the
+// kind's float4 arrays (slot counts stay below 2^28, so they fit 32 bits);
+// a partner's arrays are read at base + 32-bit byte offset.
+__device__ __forceinline__ const float4 &at(const float4 *__restrict__ base, uint32_t byte_off) {
+    return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+__device__ __forceinline__ uint32_t slot_bytes(int slot) { return (uint32_t)slot << 4; }
+
+// byte offset of the slot named by list entry v
+template <int T>
+__device__ __forceinline__ uint32_t entry_bytes(const uint32_t *base, uint32_t v) {
+    return base[(v & 15u) * T] + (v & 0xfff0u);
+}
 
 // lane-private LDS views of a T-thread workgroup: entry k at list[k * T],
 // base b at base[b * T]
 template <int T>
 struct Lists {
     uint16_t *list;
-    int *base;
+    uint32_t *base;
 };
 
-// partner state of list entries k .. k+kPairBatch-1 (entries past cnt re-read
-// the last one, always a valid slot)
+// partner state of list entries k .. k+kPairBatch-1 (the entries past the
+// last one name the sentinel partner)
 template <int KIND, int T>
 __device__ __forceinline__ void gather(const float4 *__restrict__ pos, const float4 *__restrict__ vel,
-                                       const float4 *__restrict__ np, const Lists<T> &L, int k, int cnt, float4 *pj,
+                                       const float4 *__restrict__ np, const Lists<T> &L, int k, float4 *pj,
                                        float4 *vj, float4 *nj) {
 #pragma unroll
     for (int q = 0; q < kPairBatch; ++q) {
-        const int v = L.list[min(k + q, cnt - 1) * T];
-        const int j = L.base[(v >> kRunBits) * T] + (v & (kRunMax - 1));
-        pj[q] = pos[j];
-        vj[q] = vel[j];
-        if (KIND == kFluidSolid) nj[q] = np[j];
+        const uint32_t off = entry_bytes<T>(L.base, L.list[(k + q) * T]);
+        pj[q] = at(pos, off);
+        vj[q] = at(vel, off);
+        if (KIND == kFluidSolid) nj[q] = at(np, off);
     }
 }
 
@@ -183,8 +208,7 @@ __device__ __forceinline__ void gather_packed(const uint4 *__restrict__ pk, cons
                                               const Lists<T> &L, int k, int cnt, uint4 *rq, int *cq, int *jq) {
 #pragma unroll
     for (int q = 0; q < kPairBatch; ++q) {
-        const int v = L.list[min(k + q, cnt - 1) * T];
-        const int j = L.base[(v >> kRunBits) * T] + (v & (kRunMax - 1));
+        const int j = (int)(entry_bytes<T>(L.base, L.list[min(k + q, cnt - 1) * T]) >> 4);
         rq[q] = pk[j];
         cq[q] = pcell[j];
         jq[q] = j;
@@ -200,7 +224,9 @@ struct PackedSrc {
 // pair terms of list entries [0, cnt), in list order.  The partner state of
 // the next kPairBatch entries is gathered while the current ones are
 // evaluated; all kPairBatch slots of a round are evaluated, the ones past
-// cnt with weight 0.  PACK: fluid partners come from their FP16 records.
+// cnt with weight 0 (the sentinel partner; in the packed form the last entry
+// again, with d2 = kPadD2).  PACK: fluid partners come from their FP16
+// records.
 template <int KIND, int T, bool PACK>
 __device__ __forceinline__ void evaluate(const DevParams &P, const Self &me, const float4 *__restrict__ pos,
                                          const float4 *__restrict__ vel, const float4 *__restrict__ np,
@@ -226,13 +252,15 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const Self &me, con
         }
         return;
     }
+#pragma unroll
+    for (int q = 0; q < kListPad; ++q) L.list[(cnt + q) * T] = kSentinelEntry;
     float4 pj[kPairBatch], vj[kPairBatch], nj[kPairBatch];
-    if (cnt > 0) gather<KIND, T>(pos, vel, np, L, 0, cnt, pj, vj, nj);  // an empty list holds no valid slot
+    gather<KIND, T>(pos, vel, np, L, 0, pj, vj, nj);
     for (int k = 0; k < cnt; k += kPairBatch) {
         float4 pn[kPairBatch], vn[kPairBatch], nn[kPairBatch];
-        gather<KIND, T>(pos, vel, np, L, k + kPairBatch, cnt, pn, vn, nn);
+        gather<KIND, T>(pos, vel, np, L, k + kPairBatch, pn, vn, nn);
 #pragma unroll
-        for (int q = 0; q < kPairBatch; ++q) one_pair<KIND>(P, me, pj[q], vj[q], nj[q], k + q < cnt, acc);
+        for (int q = 0; q < kPairBatch; ++q) one_pair<KIND>(P, me, pj[q], vj[q], nj[q], acc);
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q) {
             pj[q] = pn[q];
@@ -260,19 +288,20 @@ __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col
         const bool has = sb < e;
         int slot = col;
         if (sub > 0) {
-            if (__ballot(has && spare == kBaseSlots - kColumns) != 0) {
+            if (__ballot(has && spare == kSpareSlots) != 0) {
                 evaluate<KIND, T, PACK>(P, me, pos, vel, np, ps, L, cnt, acc);
                 cnt = 0;
                 spare = 0;
             }
             if (has) {
-                slot = kColumns + spare;
-                L.base[slot * T] = sb;
-                spare += 1;
+                // a spare slot, or without spares the column's own one, free after the evaluation
+                if (kSpareSlots > 0) slot = kColumns + spare++;
+                L.base[slot * T] = slot_bytes(sb);
             }
         }
         const int se = min(sb + kRunMax, e);  // <= sb for a lane without work
-        const int tag = (slot << kRunBits) - sb;  // entry = tag + slot index of the partner
+        // entry = tag + 16 * (slot index of the partner), in 16 bits
+        const uint32_t tag = (uint32_t)slot - ((uint32_t)sb << 4);
         // branch-free batches: loads at or past the run end stay inside the
         // padded array (se <= n, kPosPad >= kScanBatch) and are masked out of
         // the test; every candidate writes list slot cnt, which only a hit
@@ -286,7 +315,7 @@ __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col
         for (int j = sb; __ballot(j < se) != 0; j += kScanBatch) {
             const int jj = min(j, se);
             const int lim = se - jj;  // candidates k < lim of the batch lie in the run
-            const int e0 = tag + jj;
+            const uint32_t e0 = tag + ((uint32_t)jj << 4);
             float4 qn[kScanBatch];
             const float4 *nxt = pos + min(j + kScanBatch, se);
 #pragma unroll
@@ -295,7 +324,7 @@ __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col
             for (int k = 0; k < kScanBatch; ++k) {
                 const float d2 = len2(q[k].x - me.x, q[k].y - me.y, q[k].z - me.z);
                 const bool hit = (k < lim) & (d2 < P.R2);
-                L.list[cnt * T] = (uint16_t)(e0 + k);
+                L.list[cnt * T] = (uint16_t)(e0 + 16u * k);
                 cnt += hit ? 1 : 0;
             }
 #pragma unroll
@@ -336,59 +365,73 @@ struct Columns {
     __device__ __forceinline__ int c1(int col) const { return inside(col) ? base(col) + zhi1 : 0; }
 };
 
+// Where a fluid particle sits, for its z-windows: fx, fy its position in its
+// cell in cells, tzs its z in sub-cells (1/kSubZ cell), and the margins that
+// cover the rounding of cell coordinates.  A coordinate t = (x - cell_min) /
+// interval carries a relative error below 2^-23 (two roundings), on the
+// particle's side and on the partner's, so the margins grow with t: 1e-4
+// cell + 3e-7 t across x and y, 1e-3 sub-cell + 3e-7 t in z.
+struct Window {
+    float fx, fy, tzs, mx, my, mz;
+};
+
 // The z-window of column `col` (fluid partners): inside a cell the slots
-// are sorted by quarter-cell z index, so a column's slots with z in a range
-// are the contiguous key range of that z-range.  The column is at least
-// (gx, gy) away from the particle across x and y (its distance to the cell
-// faces the column lies beyond, less a 1e-4-cell margin), so partners
-// within R lie within rz = sqrt(R^2 (1 + 1e-5) - gx^2 - gy^2) in z; the
-// window is the quarter-cells of [z - rz, z + rz], widened by 1e-3 quarter
-// cell, which covers the rounding of both sides of the comparison.  The
-// exact cut-off test still runs on every slot of the window, so the partner
-// list is the whole column's, in the same order.  fx, fy: the particle's
-// position in its cell in cells; tzs: its z in quarter cells.
+// are sorted by their sub-cell z index, so a column's slots with z in a
+// range are the contiguous key range of that z-range.  The column is at
+// least (gx, gy) away from the particle across x and y (its distance to the
+// cell faces the column lies beyond, less the margin), so partners within R
+// lie within rz = sqrt(R^2 (1 + 1e-5) - gx^2 - gy^2) in z; the window is the
+// sub-cells of [z - rz, z + rz], widened by the z margin.  The exact cut-off
+// test still runs on every slot of the window, so the partner list is the
+// whole column's, in the same order.
 __device__ __forceinline__ void z_window(const DevParams &P, const Columns &cols, const int32_t *__restrict__ fine,
-                                         int col, float fx, float fy, float tzs, int &b, int &e) {
+                                         int col, const Window &w, int &b, int &e) {
     const int ox = col / 3 - 1, oy = col % 3 - 1;
-    const float ex = ox < 0 ? fx : (ox > 0 ? 1.0f - fx : 0.0f);
-    const float ey = oy < 0 ? fy : (oy > 0 ? 1.0f - fy : 0.0f);
-    const float gx = fmaxf(ex - 1.0e-4f, 0.0f) * P.ival[0], gy = fmaxf(ey - 1.0e-4f, 0.0f) * P.ival[1];
+    const float ex = ox < 0 ? w.fx : (ox > 0 ? 1.0f - w.fx : 0.0f);
+    const float ey = oy < 0 ? w.fy : (oy > 0 ? 1.0f - w.fy : 0.0f);
+    const float gx = fmaxf(ex - w.mx, 0.0f) * P.ival[0], gy = fmaxf(ey - w.my, 0.0f) * P.ival[1];
     const float r2 = P.R2w - gx * gx - gy * gy;
     b = e = 0;
     if (!cols.inside(col) || !(r2 > 0.0f)) return;
-    const float rzs = sqrtf(r2) * P.inv[2] * (float)kSubZ;
-    const int lo = max((int)floorf(tzs - rzs - 1.0e-3f), cols.zlo * kSubZ);
-    const int hi = max(min((int)floorf(tzs + rzs + 1.0e-3f) + 1, cols.zhi1 * kSubZ), lo);
+    const float rzs = sqrtf(r2) * P.inv[2] * (float)kSubZ + w.mz;
+    const int lo = max((int)floorf(w.tzs - rzs), cols.zlo * kSubZ);
+    const int hi = max(min((int)floorf(w.tzs + rzs) + 1, cols.zhi1 * kSubZ), lo);
     const int k0 = cols.base(col) * kSubZ;
     b = fine[k0 + lo];
     e = fine[k0 + hi];
 }
 
-// all partners of one kind: the 9 columns in order, then one evaluation;
-// fluid partners are read from each column's z-window only
+// all partners of one kind (n particles): the 9 columns in order, then one
+// evaluation; fluid partners are read from each column's z-window only
 template <int KIND, int T, bool PACK>
-__device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cell, const int32_t *__restrict__ start,
+__device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cell, int n,
+                                     const int32_t *__restrict__ start,
                                      const int32_t *__restrict__ fine, const float4 *__restrict__ pos,
                                      const float4 *__restrict__ vel, const float4 *__restrict__ np,
                                      const PackedSrc &ps, const Lists<T> &L, Acc &acc) {
     const Columns cols(P, cell);
-    float fx = 0.0f, fy = 0.0f, tzs = 0.0f;
+    Window w{};
     if (KIND == kFluidFluid) {
-        fx = (me.x - P.cmin[0]) * P.inv[0] - (float)cols.cx;
-        fy = (me.y - P.cmin[1]) * P.inv[1] - (float)cols.cy;
-        tzs = (me.z - P.cmin[2]) * P.inv[2] * (float)kSubZ;
+        const float tx = (me.x - P.cmin[0]) * P.inv[0], ty = (me.y - P.cmin[1]) * P.inv[1];
+        w.fx = tx - (float)cols.cx;
+        w.fy = ty - (float)cols.cy;
+        w.tzs = (me.z - P.cmin[2]) * P.inv[2] * (float)kSubZ;
+        w.mx = fmaf(tx, 3.0e-7f, 1.0e-4f);
+        w.my = fmaf(ty, 3.0e-7f, 1.0e-4f);
+        w.mz = fmaf(w.tzs, 3.0e-7f, 1.0e-3f);
     }
     int cnt = 0, spare = 0;
+    L.base[kSentinelSlot * T] = slot_bytes(n);
 #pragma unroll 1
     for (int col = 0; col < kColumns; ++col) {
         int b, e;
         if (KIND == kFluidFluid) {
-            z_window(P, cols, fine, col, fx, fy, tzs, b, e);
+            z_window(P, cols, fine, col, w, b, e);
         } else {
             b = start[cols.c0(col)];
             e = start[cols.c1(col)];
         }
-        L.base[col * T] = b;
+        L.base[col * T] = slot_bytes(b);
         scan<KIND, T, PACK>(P, me, col, b, e, pos, vel, np, ps, L, cnt, spare, acc);
     }
     evaluate<KIND, T, PACK>(P, me, pos, vel, np, ps, L, cnt, acc);
@@ -396,8 +439,8 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cel
 
 template <bool RATES, int T, bool PACK>
 __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepArgs &a, int blk, int nblk) {
-    __shared__ uint16_t s_list[kListCap * T];
-    __shared__ int s_base[kBaseSlots * T];
+    __shared__ uint16_t s_list[(kListCap + kListPad) * T];
+    __shared__ uint32_t s_base[kBaseSlots * T];
     const int i = xcd_tile(blk, nblk) * T + threadIdx.x;
     if (i >= P.nf) return;  // lists are lane-private: no barrier follows
     const Lists<T> list{s_list + threadIdx.x, s_base + threadIdx.x};
@@ -416,7 +459,7 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     me.inv_rho = 1.0f / me.rho;
     me.two_rho = 2.0f * me.rho;
     me.m2_inv_rho = -2.0f * me.inv_rho;
-    me.nvisc = -P.visc10 * me.inv_rho;
+    me.vis5 = P.viscid ? 0.5f * P.visc10 : 0.0f;
     me.half_inv_rho = 0.5f * me.inv_rho;
     Acc acc{0.0f, 0.0f, 0.0f, 0.0f};
     const int cell = a.fcell[i];
@@ -432,13 +475,14 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
 
     // partners kind by kind: fluid, solid, wall (the canonical order)
     const PackedSrc fsrc{a.fpk, a.fcell}, none{nullptr, nullptr};
-    pass<kFluidFluid, T, PACK>(P, me, cell, a.fstart, a.ffine, a.fpos, a.fvel, nullptr, fsrc, list, acc);
-    pass<kFluidSolid, T, false>(P, me, cell, a.sstart, nullptr, a.spos, a.svel, a.snp, none, list, acc);
-    pass<kFluidVirt, T, false>(P, me, cell, a.vstart, nullptr, a.vpos, a.vvel, nullptr, none, list, acc);
+    pass<kFluidFluid, T, PACK>(P, me, cell, P.nf, a.fstart, a.ffine, a.fpos, a.fvel, nullptr, fsrc, list, acc);
+    pass<kFluidSolid, T, false>(P, me, cell, P.ns, a.sstart, nullptr, a.spos, a.svel, a.snp, none, list, acc);
+    pass<kFluidVirt, T, false>(P, me, cell, P.nv, a.vstart, nullptr, a.vpos, a.vvel, nullptr, none, list, acc);
 
     // fluidUpdate (fluid.cu:234-269): NaN scrub, explicit step with the old
     // velocity for the position, density clamp, EOS, domain clamp, speed cap
-    float drho = acc.drho, ax = acc.ax, ay = acc.ay, az = acc.az;
+    float drho = me.two_rho * acc.drho, ax = me.m2_inv_rho * acc.ax, ay = me.m2_inv_rho * acc.ay,
+          az = me.m2_inv_rho * acc.az;
     if (!(drho == drho)) drho = 0.0f;
     if (!(ax == ax && ay == ay && az == az)) ax = ay = az = 0.0f;
     if (RATES) a.rates[i] = make_float4(ax, ay, az, drho);

@@ -36,17 +36,24 @@ namespace fsa {
 
 constexpr int kBlock = 256;      // 4 wave64 per workgroup
 constexpr int kScanTile = 1024;  // elements per scan workgroup (256 threads x 4)
-// position arrays carry kPosPad zeroed entries past the last particle, so a
-// batch of candidate loads may run past the end of a slot range (the
-// results past the range are masked out of every test)
+// Position, velocity and solid-normal arrays carry kPosPad entries past the
+// last particle, so a batch of candidate loads may run past the end of a slot
+// range (the results past the range are masked out of every test).  Each of
+// those entries holds a sentinel partner: position (kFarPos, kFarPos,
+// kFarPos), density w = rho0, zero velocity, pressure and normal.  A pair
+// term with it is finite and its kernel weight is exactly 0, so the fluid
+// pass pads its partner rounds with the sentinel instead of masking.
 constexpr int kPosPad = 16;
+constexpr float kFarPos = 1.0e15f;
+// slot byte offsets are 32-bit (slot * 16 B)
+constexpr int kMaxKindParticles = (1 << 28) - kPosPad;
 constexpr int kSolidLanes = 16;  // lanes serving one solid particle
-// Inside a cell, fluid particles are sorted by a quarter-cell z index (then
+// Inside a cell, fluid particles are sorted by an eighth-cell z index (then
 // by .dat index): their sort key is cell * kSubZ + sub.  The per-cell start
 // array every pass uses is taken from the per-key one, which the fluid pass
 // also reads to test only the z-range of a column its cut-off reaches.
 // Solid and wall particles keep one key per cell (cell, then .dat index).
-constexpr int kSubZ = 4;
+constexpr int kSubZ = 8;
 // solid-solid partners recorded per lane in the first sub-step of a step
 // (entries per lane; a lane with more partners walks its candidates instead)
 constexpr int kSolidListCap = 16;
@@ -190,6 +197,7 @@ hipError_t launch_cells_one(const CellJob &j, hipStream_t s);
 hipError_t launch_hash_one(const CellJob &j, hipStream_t s);     // fill + hash
 hipError_t launch_sort_one(const CellJob &j, hipStream_t s);     // scan + place + gather
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int n, hipStream_t s);
+hipError_t launch_fill_f4(float4 *p, float4 v, int n, hipStream_t s);
 
 // ---- fluid pass (fsa_fluid.hip) -----------------------------------------------
 struct FluidStepArgs {

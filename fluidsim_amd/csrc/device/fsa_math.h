@@ -32,7 +32,7 @@ __device__ __forceinline__ int find_cell(const DevParams &P, float x, float y, f
 
 // sort key of a position with nsub keys per cell: cell * nsub + the
 // 1/nsub-cell z index inside the cell, (int)(t nsub) - nsub (int)t with t
-// the z coordinate in cells (nsub t is exact for nsub = 1, 4), clamped like
+// the z coordinate in cells (nsub t is exact for nsub a power of 2), clamped like
 // the cell indices
 __device__ __forceinline__ int cell_key(const DevParams &P, float x, float y, float z, int nsub) {
     const int c = find_cell(P, x, y, z);

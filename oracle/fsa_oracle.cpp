@@ -5,9 +5,9 @@
 //
 // Templated on the arithmetic type T (float = the reference's FP32, double =
 // the FP64 precision oracle) and on the neighbour traversal:
-//   CANONICAL  fluid sorted by (cell id, quarter-cell z index inside the
-//              cell, .dat index) -- the quarter-cell index is (int)(4 t) -
-//              4 (int)t with t = (z - cell_min_z) / interval_z -- and solid
+//   CANONICAL  fluid sorted by (cell id, eighth-cell z index inside the
+//              cell, .dat index) -- the eighth-cell index is (int)(8 t) -
+//              8 (int)t with t = (z - cell_min_z) / interval_z -- and solid
 //              and wall particles by (cell id, .dat index), the key order of
 //              the GPU's cell build; for a particle in cell
 //              (cx,cy,cz) the partners are walked kind by kind (fluid, solid,
@@ -257,9 +257,9 @@ template <class T> struct Oracle : OracleBase {
     std::vector<V3<T>> vx, vn, vu;
     std::vector<T> vrho, vp;
     std::vector<int> vtype;
-    // cell id and quarter-cell z index per particle and kind
+    // cell id and eighth-cell z index per particle and kind
     std::vector<int> cell[3], sub[3];
-    static constexpr int kSubZ = 4;
+    static constexpr int kSubZ = 8;
     // canonical lists
     std::vector<int> start[3], sorted[3];
     // exact linked lists (include/cell.cuh:11-42)
@@ -469,7 +469,7 @@ template <class T> struct Oracle : OracleBase {
     // ---- cell lists ------------------------------------------------------
     void fill_cell() override {
         if (mode == CANONICAL) {
-            // stable counting sort by (cell, quarter-cell) over ascending .dat index
+            // stable counting sort by (cell, eighth-cell) over ascending .dat index
             for (int k = 0; k < 3; ++k) {
                 const int n = count(k);
                 const int ns = k == KF ? kSubZ : 1;

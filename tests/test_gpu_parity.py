@@ -241,9 +241,9 @@ def test_showcase_scale_integer_invariants(gpu_available):
         expect = (idx[:, 0] * n[1] + idx[:, 1]) * n[2] + idx[:, 2]
         cells = dev.cell_ids(kind)
         np.testing.assert_array_equal(cells, expect)
-        # slots: by cell, then (fluid) quarter-cell z index inside the cell, then .dat index
+        # slots: by cell, then (fluid) eighth-cell z index inside the cell, then .dat index
         tz = (x[:, 2].astype(np.float32) - cmin[2]) * inv[2]
-        sub = np.clip((tz * np.float32(4)).astype(np.int64) - 4 * idx[:, 2], 0, 3) if kind == FLUID else 0 * idx[:, 2]
+        sub = np.clip((tz * np.float32(8)).astype(np.int64) - 8 * idx[:, 2], 0, 7) if kind == FLUID else 0 * idx[:, 2]
         order = dev.sorted_ids(kind)
         np.testing.assert_array_equal(order, np.lexsort((np.arange(len(cells)), sub, cells)))
         start = dev.cell_start(kind)
@@ -286,6 +286,32 @@ def test_missing_kinds(gpu_available, variant):
     if p.n_solid:
         L = max(sc.tank) * sc.dp
         assert max_abs(dev.solid()["position"], o.solid()["position"]) <= REL * L
+
+
+def test_dense_cells_long_runs(gpu_available):
+    """h = 7 dp: cells 18 dp wide (2 per axis), ~6,500 fluid slots in a
+    column's z-window and ~11,000 partners per particle.  The fluid pass then
+    splits a run at 4096 slots (the reach of a 16-bit list entry) and
+    evaluates a full list hundreds of times per particle; solid particles
+    overflow their recorded-partner lists.  Fluid: the usual tolerances; the
+    gate: as accurate as the FP32 oracle against FP64 (this coarse, stiff
+    gate is badly conditioned in FP32)."""
+    sc = scenes.DamBreak(dp=0.01, fluid=(20, 20, 20), tank=(20, 20, 20), gate=(2, 8, 6), gate_x0=12,
+                         gate_clamp_rows=2, wall_layers=1, h_ratio=7.0, sound_speed=20.0, youngs_modulus=2.0e6,
+                         seed=11)
+    a, p, dev, o = _setup(sc)
+    assert tuple(p.cell_number) == (2, 2, 2)
+    print(f"dense cells: {p.n_fluid} fluid, {p.n_solid} solid, {p.n_virt} walls", flush=True)
+    o64 = orc.Oracle(p, 64)
+    o64.upload(a)
+    dev.step(2)
+    o.step(2)
+    o64.step(2)
+    assert_within(compare_fluid(sc, dev.fluid(), o.fluid()), "dense cells")
+    L = max(sc.tank) * sc.dp
+    e_gpu, e_32, ok = same_class(dev.solid()["position"], o.solid()["position"], o64.solid()["position"], L)
+    print(f"gate position error vs FP64: gpu {e_gpu:.3g}, fp32 oracle {e_32:.3g}", flush=True)
+    assert ok, (e_gpu, e_32)
 
 
 def test_teleported_particle_is_kept_and_counted(gpu_available, tiny):

@@ -60,8 +60,8 @@ def test_cell_ids_match_the_float32_formula(tiny):
 
 
 def test_canonical_cell_lists_are_a_stable_sort(tiny):
-    """Slots are sorted by cell, then (fluid only) by the quarter-cell z index
-    inside the cell ((int)(4 t) - 4 (int)t, t the float32 z coordinate in
+    """Slots are sorted by cell, then (fluid only) by the eighth-cell z index
+    inside the cell ((int)(8 t) - 8 (int)t, t the float32 z coordinate in
     cells), then by .dat index; the per-cell starts count each cell's members."""
     sc, p, a = tiny
     o = orc.Oracle(p, 32)
@@ -75,9 +75,9 @@ def test_canonical_cell_lists_are_a_stable_sort(tiny):
         ids = np.arange(len(cell))
         tz = (a[key][:, 2].astype(np.float32) - cmin) * inv
         iz = tz.astype(np.int32)
-        sub = np.clip((tz * np.float32(4)).astype(np.int32) - 4 * iz, 0, 3)
+        sub = np.clip((tz * np.float32(8)).astype(np.int32) - 8 * iz, 0, 7)
         if kind == FLUID:
-            assert sub.max() == 3
+            assert sub.max() == 7
         else:
             sub = np.zeros_like(sub)
         np.testing.assert_array_equal(o.sorted_ids(kind), np.lexsort((ids, sub, cell)))
