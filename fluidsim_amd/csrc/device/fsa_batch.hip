@@ -214,7 +214,7 @@ void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph
     t.solid_off = put(blob, solid.data(), solid.size());
 
     std::vector<int> blocks;
-    for (BatchOp op : {kBFillCounts, kBHash, kBScanTiles, kBScanSums, kBScanAdd, kBCellStarts, kBPlace, kBGather}) {
+    for (BatchOp op : {kBFillCounts, kBHash, kBScanTiles, kBScanSums, kBScanAdd, kBPlace, kBGather}) {
         blocks.clear();
         for (const CellJob &j : cells) blocks.push_back(blocks_cells(op, j));
         add_launch(blob, t, Launch::kCells, op, blocks);

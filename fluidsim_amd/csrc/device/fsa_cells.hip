@@ -157,19 +157,21 @@ struct OpScanSumsImpl {
     }
 };
 
-// pass 3: add tile offsets; start[0] = 0
+// pass 3: add tile offsets; start[0] = 0.  With several keys per cell the
+// per-cell starts are every nsub-th key start (cell c begins where its first
+// key does), written by the thread that finishes that key start.
 struct OpScanAddImpl {
     static __device__ void run(const CellJob &j, int blk, int) {
         const int i = blk * kBlock + threadIdx.x;
-        if (i == 0) j.fstart[0] = 0;
-        if (i < j.nk) j.fstart[1 + i] += j.tile_sums[i / kScanTile];
-    }
-};
-// the per-cell starts: cell c begins where its first key does
-struct OpCellStartsImpl {
-    static __device__ void run(const CellJob &j, int blk, int) {
-        const int c = blk * kBlock + threadIdx.x;
-        if (c <= j.P.ncell) j.start[c] = j.fstart[c * j.nsub];
+        if (i == 0) {
+            j.fstart[0] = 0;
+            if (j.nsub > 1) j.start[0] = 0;
+        }
+        if (i < j.nk) {
+            const int v = j.fstart[1 + i] + j.tile_sums[i / kScanTile];
+            j.fstart[1 + i] = v;
+            if (j.nsub > 1 && (i + 1) % j.nsub == 0) j.start[(i + 1) / j.nsub] = v;
+        }
     }
 };
 
@@ -226,7 +228,6 @@ int blocks_cells(BatchOp op, const CellJob &j) {
     case kBScanTiles: return scan_tiles(j.nk);
     case kBScanSums: return 1;
     case kBScanAdd: return blocks_for(j.nk);
-    case kBCellStarts: return j.nsub > 1 ? blocks_for((long long)j.P.ncell + 1) : 0;  // nsub 1: the same array
     case kBPlace: return blocks_for(j.n);
     case kBGather: return blocks_for(j.n);
     default: return 0;
@@ -243,9 +244,6 @@ hipError_t launch_many_cells(BatchOp op, const CellJob *jobs, const int *prefix,
     case kBScanTiles: hipLaunchKernelGGL((k_many<OpScanTilesImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
     case kBScanSums: hipLaunchKernelGGL((k_many<OpScanSumsImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
     case kBScanAdd: hipLaunchKernelGGL((k_many<OpScanAddImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
-    case kBCellStarts:
-        hipLaunchKernelGGL((k_many<OpCellStartsImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs);
-        break;
     case kBPlace: hipLaunchKernelGGL((k_many<OpPlaceImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
     case kBGather: hipLaunchKernelGGL((k_many<OpGatherImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
     default: return hipErrorInvalidValue;
@@ -312,7 +310,6 @@ hipError_t launch_sort_one(const CellJob &j, hipStream_t s) {
     if ((e = one<OpScanTilesImpl>(j, blocks_cells(kBScanTiles, j), s)) != hipSuccess) return e;
     if ((e = one<OpScanSumsImpl>(j, blocks_cells(kBScanSums, j), s)) != hipSuccess) return e;
     if ((e = one<OpScanAddImpl>(j, blocks_cells(kBScanAdd, j), s)) != hipSuccess) return e;
-    if ((e = one<OpCellStartsImpl>(j, blocks_cells(kBCellStarts, j), s)) != hipSuccess) return e;  // none if nsub 1
     if ((e = one<OpPlaceImpl>(j, blocks_cells(kBPlace, j), s)) != hipSuccess) return e;
     return one<OpGatherImpl>(j, blocks_cells(kBGather, j), s);
 }

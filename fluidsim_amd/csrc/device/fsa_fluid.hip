@@ -147,6 +147,9 @@ __device__ __forceinline__ void one_pair_packed(const DevParams &P, const Self &
 // workgroup sizes of the fluid pass (one to four waves; LDS per wave bounds
 // the occupancy, so the best size depends on the scene: fsa_autotune_fluid)
 constexpr int kThreadChoices[] = {64, 128, 256};
+// waves per SIMD the fluid kernels keep registers for: three (168 VGPRs
+// each), as many as LDS admits with workgroups of up to 128 threads
+constexpr int fluid_waves(int threads) { return threads > 128 ? 2 : 3; }
 // LDS per lane: (kListCap + kListPad) 16-bit entries + kBaseSlots 32-bit bases = 240 B, so that ten wave64s fit a
 // CU's 160 KiB (a lane more of either, and LDS allocation granules leave room for nine)
 constexpr int kListCap = 89;    // 16-bit list entries per lane
@@ -402,16 +405,20 @@ __device__ __forceinline__ void z_window(const DevParams &P, const Columns &cols
 }
 
 // all partners of one kind (n particles): the 9 columns in order, then one
-// evaluation; fluid partners are read from each column's z-window only
+// evaluation; fluid partners are read from each column's z-window only.  The
+// slot ranges of all 9 columns are taken first (their loads in flight
+// together); a kind with no candidate for any lane of the wave is skipped,
+// and so is a column without one.
 template <int KIND, int T, bool PACK>
 __device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cell, int n,
                                      const int32_t *__restrict__ start,
                                      const int32_t *__restrict__ fine, const float4 *__restrict__ pos,
                                      const float4 *__restrict__ vel, const float4 *__restrict__ np,
                                      const PackedSrc &ps, const Lists<T> &L, Acc &acc) {
-    const Columns cols(P, cell);
-    Window w{};
+    int cb[kColumns], ce[kColumns];
     if (KIND == kFluidFluid) {
+        const Columns cols(P, cell);
+        Window w;
         const float tx = (me.x - P.cmin[0]) * P.inv[0], ty = (me.y - P.cmin[1]) * P.inv[1];
         w.fx = tx - (float)cols.cx;
         w.fy = ty - (float)cols.cy;
@@ -419,18 +426,21 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cel
         w.mx = fmaf(tx, 3.0e-7f, 1.0e-4f);
         w.my = fmaf(ty, 3.0e-7f, 1.0e-4f);
         w.mz = fmaf(w.tzs, 3.0e-7f, 1.0e-3f);
+#pragma unroll
+        for (int col = 0; col < kColumns; ++col) z_window(P, cols, fine, col, w, cb[col], ce[col]);
+    } else {
+        column_ranges(P, cell, start, cb, ce);
     }
+    bool any = false;
+#pragma unroll
+    for (int col = 0; col < kColumns; ++col) any |= cb[col] < ce[col];
+    if (__ballot(any) == 0) return;
     int cnt = 0, spare = 0;
     L.base[kSentinelSlot * T] = slot_bytes(n);
 #pragma unroll 1
     for (int col = 0; col < kColumns; ++col) {
-        int b, e;
-        if (KIND == kFluidFluid) {
-            z_window(P, cols, fine, col, w, b, e);
-        } else {
-            b = start[cols.c0(col)];
-            e = start[cols.c1(col)];
-        }
+        const int b = cb[col], e = ce[col];
+        if (__ballot(b < e) == 0) continue;
         L.base[col * T] = slot_bytes(b);
         scan<KIND, T, PACK>(P, me, col, b, e, pos, vel, np, ps, L, cnt, spare, acc);
     }
@@ -558,14 +568,14 @@ template <int T, bool PACK>
 hipError_t one_fluid(const FluidJob &j, hipStream_t s) {
     const dim3 grid(blocks_fluid(j, T));
     if (j.a.rates)
-        hipLaunchKernelGGL((k_one<OpFluid<true, T, PACK>, FluidJob, T>), grid, dim3(T), 0, s, j);
+        hipLaunchKernelGGL((k_one<OpFluid<true, T, PACK>, FluidJob, T, fluid_waves(T)>), grid, dim3(T), 0, s, j);
     else
-        hipLaunchKernelGGL((k_one<OpFluid<false, T, PACK>, FluidJob, T>), grid, dim3(T), 0, s, j);
+        hipLaunchKernelGGL((k_one<OpFluid<false, T, PACK>, FluidJob, T, fluid_waves(T)>), grid, dim3(T), 0, s, j);
     return hipGetLastError();
 }
 template <int T, bool PACK>
 hipError_t many_fluid(const FluidJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s) {
-    hipLaunchKernelGGL((k_many<OpFluid<false, T, PACK>, FluidJob, T>), dim3(nblocks), dim3(T), 0, s, jobs, prefix,
+    hipLaunchKernelGGL((k_many<OpFluid<false, T, PACK>, FluidJob, T, fluid_waves(T)>), dim3(nblocks), dim3(T), 0, s, jobs, prefix,
                        njobs);
     return hipGetLastError();
 }

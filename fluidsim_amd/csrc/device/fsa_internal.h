@@ -150,15 +150,17 @@ __host__ __device__ inline int padded_cells(int ncell) { return scan_tiles(ncell
 
 // ---- generic wrappers -------------------------------------------------------
 
-template <class Op, class Job, int THREADS = kBlock>
-__global__ __launch_bounds__(THREADS) void k_one(Job job) {
+// WAVES: the register allocation leaves room for at least this many waves
+// per SIMD
+template <class Op, class Job, int THREADS = kBlock, int WAVES = 1>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_one(Job job) {
     Op::run(job, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // prefix[s] is the first block of job s, prefix[n] the grid size
-template <class Op, class Job, int THREADS = kBlock>
-__global__ __launch_bounds__(THREADS) void k_many(const Job *__restrict__ jobs, const int *__restrict__ prefix,
-                                                  int n) {
+template <class Op, class Job, int THREADS = kBlock, int WAVES = 1>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_many(
+    const Job *__restrict__ jobs, const int *__restrict__ prefix, int n) {
     const int b = (int)blockIdx.x;
     int lo = 0, hi = n;
     while (hi - lo > 1) {
@@ -267,7 +269,7 @@ hipError_t launch_solid_update(const DevParams &P, const SolidArgs &a, hipStream
 // rounds every job's range up to whole XCD rounds (multiples of 8 blocks) so
 // a block's XCD is the same in the batched grid as in a single-scene grid.
 enum BatchOp {
-    kBFillCounts, kBHash, kBScanTiles, kBScanSums, kBScanAdd, kBCellStarts, kBPlace, kBGather,  // CellJob
+    kBFillCounts, kBHash, kBScanTiles, kBScanSums, kBScanAdd, kBPlace, kBGather,  // CellJob
     kBCouple, kBDeformRecord, kBDeform, kBForce, kBSolidUpdate                     // SolidJob
 };
 hipError_t launch_many_cells(BatchOp op, const CellJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s);
