@@ -121,18 +121,24 @@ int check_params(const fsa_params &p) {
 }
 
 int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_state, int nsub, bool compensated = false,
-               bool packed = false) {
+               bool packed = false, bool double_starts = false) {
     k.n = n;
     k.nsub = nsub;
     const int ncell = s->P.ncell;
     const int nk = ncell * nsub;
     const size_t padded = (size_t)(nk + kScanTile - 1) / kScanTile * kScanTile;
     int rc;
-    if ((rc = s->alloc(&k.start, (size_t)ncell + 1))) return rc;
-    if (nsub > 1) {
-        if ((rc = s->alloc(&k.fstart, (size_t)nk + 1))) return rc;
-    } else {
-        k.fstart = k.start;  // one key per cell: the per-key starts are the per-cell ones
+    for (int b = 0; b < (double_starts ? 2 : 1); ++b) {
+        if ((rc = s->alloc(&k.start_buf[b], (size_t)ncell + 1))) return rc;
+        if (nsub > 1) {
+            if ((rc = s->alloc(&k.fstart_buf[b], (size_t)nk + 1))) return rc;
+        } else {
+            k.fstart_buf[b] = k.start_buf[b];  // one key per cell: the per-key starts are the per-cell ones
+        }
+    }
+    if (!double_starts) {
+        k.start_buf[1] = k.start_buf[0];
+        k.fstart_buf[1] = k.fstart_buf[0];
     }
     if ((rc = s->alloc(&k.count, padded))) return rc;
     if (n == 0) return FSA_OK;
@@ -173,7 +179,9 @@ int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_sta
 int alloc_all(fsa_sim *s) {
     int rc;
     const bool comp = (s->P.form & FSA_FORM_COMPENSATED) != 0, packed = (s->P.form & FSA_FORM_PACKED) != 0;
-    if ((rc = alloc_kind(s, s->fluid, s->P.nf, false, true, kSubZ, comp, packed))) return rc;
+    // the next step's fluid cell build may run beside the solid sub-steps,
+    // which read the fluid cells of the current order
+    if ((rc = alloc_kind(s, s->fluid, s->P.nf, false, true, kSubZ, comp, packed, true))) return rc;
     if ((rc = alloc_kind(s, s->solid, s->P.ns, true, true, 1, comp, false))) return rc;
     if ((rc = alloc_kind(s, s->virt, s->P.nv, false, false, 1))) return rc;
     const int ns = s->P.ns;
@@ -198,8 +206,12 @@ int alloc_all(fsa_sim *s) {
 
 // start arrays of kinds without particles are all zero (every cell empty)
 int zero_starts(fsa_sim *s, KindBuffers &k) {
-    HIP_TRY(launch_fill_i32(k.start, 0, s->P.ncell + 1, s->stream));
-    if (k.fstart != k.start) HIP_TRY(launch_fill_i32(k.fstart, 0, s->P.ncell * k.nsub + 1, s->stream));
+    for (int b = 0; b < 2; ++b) {
+        if (b == 1 && k.start_buf[1] == k.start_buf[0]) break;
+        HIP_TRY(launch_fill_i32(k.start_buf[b], 0, s->P.ncell + 1, s->stream));
+        if (k.fstart_buf[b] != k.start_buf[b])
+            HIP_TRY(launch_fill_i32(k.fstart_buf[b], 0, s->P.ncell * k.nsub + 1, s->stream));
+    }
     return FSA_OK;
 }
 
@@ -271,19 +283,20 @@ int stage_collect(fsa_sim *s, int st) {
     }
     return FSA_OK;
 }
-int stage_open(fsa_sim *s, int st) {
+// a stage's bracket: both events on the stream its work runs on
+int stage_open(fsa_sim *s, int st, hipStream_t on = nullptr) {
     if (!s->timing) return FSA_OK;
     if (s->ev_pending[st] >= fsa_sim::kRing) {
         int rc = stage_collect(s, st);
         if (rc) return rc;
     }
-    HIP_TRY(hipEventRecord(s->ev_begin[st][s->ev_head[st]], s->stream));
+    HIP_TRY(hipEventRecord(s->ev_begin[st][s->ev_head[st]], on ? on : s->stream));
     s->ev_open[st] = true;
     return FSA_OK;
 }
-void stage_end(fsa_sim *s, int st) {
+void stage_end(fsa_sim *s, int st, hipStream_t on = nullptr) {
     if (!s->timing || !s->ev_open[st]) return;
-    if (hipEventRecord(s->ev_end[st][s->ev_head[st]], s->stream) != hipSuccess) return;
+    if (hipEventRecord(s->ev_end[st][s->ev_head[st]], on ? on : s->stream) != hipSuccess) return;
     s->ev_head[st] = (s->ev_head[st] + 1) % fsa_sim::kRing;
     s->ev_pending[st] += 1;
     s->ev_open[st] = false;
@@ -312,8 +325,8 @@ SolidArgs fsa::solid_args(const fsa_sim *s, const KindBuffers &f, const KindBuff
     a.scvel = so.cvel[so.st];
     a.sref = so.ref[so.ord];
     a.scell = so.cell[so.ord];
-    a.fstart = f.start;
-    a.sstart = so.start;
+    a.fstart = f.cur_start();
+    a.sstart = so.cur_start();
     a.fpos = f.pos[f.st];
     a.fvel = f.vel[f.st];
     a.x = s->sx;
@@ -327,8 +340,8 @@ VirtArgs fsa::virt_args(const fsa_sim *s, const KindBuffers &f, int compute_norm
     a.vvel = s->virt.vel[0];
     a.vnrm = s->vx.nrm;
     a.vcell = s->virt.cell[s->virt.ord];
-    a.fstart = f.start;
-    a.vstart = s->virt.start;
+    a.fstart = f.cur_start();
+    a.vstart = s->virt.cur_start();
     a.fpos = f.pos[f.st];
     a.fvel = f.vel[f.st];
     a.compute_normals = compute_normals;
@@ -345,10 +358,10 @@ FluidStepArgs fsa::fluid_args(const fsa_sim *s, const KindBuffers &f, const Kind
     a.fcvel_out = f.cvel[1 - f.st];
     a.fpk = f.pk;
     a.fcell = f.cell[f.ord];
-    a.fstart = f.start;
-    a.ffine = f.fstart;
-    a.sstart = so.start;
-    a.vstart = s->virt.start;
+    a.fstart = f.cur_start();
+    a.ffine = f.cur_fstart();
+    a.sstart = so.cur_start();
+    a.vstart = s->virt.cur_start();
     a.spos = so.pos[so.st];
     a.svel = so.vel[so.st];
     a.snp = s->sx.np;
@@ -459,6 +472,12 @@ int fsa_destroy(fsa_sim *s) {
             if (s->ev_begin[st][r]) (void)hipEventDestroy(s->ev_begin[st][r]);
             if (s->ev_end[st][r]) (void)hipEventDestroy(s->ev_end[st][r]);
         }
+    if (s->side) {
+        (void)hipStreamSynchronize(s->side);
+        (void)hipStreamDestroy(s->side);
+    }
+    if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+    if (s->ev_join) (void)hipEventDestroy(s->ev_join);
     if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return FSA_OK;
@@ -769,12 +788,8 @@ int fsa_fluid_calculation(fsa_sim *s) {
     return FSA_OK;
 }
 
-int fsa_solid_calculation(fsa_sim *s) {
-    CHECK_SIM(s);
-    int rc = require_uploaded(s);
-    if (rc) return rc;
-    if (s->P.ns == 0) return FSA_OK;
-    if ((rc = stage_open(s, FSA_STAGE_SOLID))) return rc;
+// the solid sub-steps of solidCalculation (src/solid.cu:241-260)
+static int solid_sub_steps(fsa_sim *s) {
     SolidArgs a = solid_args(s);
     const bool walk = (s->debug & FSA_DEBUG_WALK) != 0;
     a.fluid_rec = s->couple_fresh && !walk ? 1 : 0;
@@ -784,13 +799,85 @@ int fsa_solid_calculation(fsa_sim *s) {
         HIP_TRY(launch_solid_force(s->P, a, s->stream));
         HIP_TRY(launch_solid_update(s->P, a, s->stream));
     }
+    return FSA_OK;
+}
+
+int fsa_solid_calculation(fsa_sim *s) {
+    CHECK_SIM(s);
+    int rc = require_uploaded(s);
+    if (rc) return rc;
+    if (s->P.ns == 0) return FSA_OK;
+    if ((rc = stage_open(s, FSA_STAGE_SOLID))) return rc;
+    if ((rc = solid_sub_steps(s))) return rc;
     stage_end(s, FSA_STAGE_SOLID);
+    return FSA_OK;
+}
+
+// The side stream and its fork/join events, made on first use.
+static int ensure_side(fsa_sim *s) {
+    if (s->side) return FSA_OK;
+    HIP_TRY(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
+    return FSA_OK;
+}
+
+// Steps in the reference's stage order (src/calculation.cu:23-33), with the
+// head of step i+1 moved beside the tail of step i: the fluid cell build and
+// the wall pass of the next step need only the fluid state the fluid pass
+// has just written, so they run on the side stream while the solid
+// sub-steps (which read that state and the current fluid cells: the build
+// writes the other state buffer and the other start arrays) and the solid
+// cell build run on the context's stream.  The two join before the next
+// coupling pass.  Every kernel sees the same inputs as in the sequential
+// order, so the results are the same bits.  Stage timing keeps the overlap:
+// the side stream's fluid cell build and wall pass are timed there, and the
+// solid kind's cell build is timed with the solid stage it follows.
+static int step_overlapped(fsa_sim *s, int n_steps) {
+    int rc;
+    if ((rc = require_uploaded(s)) || (rc = ensure_side(s))) return rc;
+    if ((rc = fsa_particle_cell_update(s)) || (rc = fsa_fill_cell(s))) return rc;
+    bool walls_done = false;
+    for (int i = 0; i < n_steps; ++i) {
+        if ((rc = fsa_couple_calculation(s))) return rc;
+        if (!walls_done && (rc = fsa_virt_update(s))) return rc;
+        if ((rc = fsa_fluid_calculation(s))) return rc;
+        if (i == n_steps - 1) return fsa_solid_calculation(s);
+        HIP_TRY(hipEventRecord(s->ev_fork, s->stream));
+        HIP_TRY(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+        // context stream: this step's solid sub-steps (their arguments name
+        // the current fluid buffers), then the next solid cell build
+        if (s->P.ns) {
+            if ((rc = stage_open(s, FSA_STAGE_SOLID)) || (rc = solid_sub_steps(s))) return rc;
+            HIP_TRY(launch_cells_one(make_cell_job(s->P, s->solid, s->jumps, true), s->stream));
+            flip(s->solid);
+            stage_end(s, FSA_STAGE_SOLID);
+        }
+        // side stream: the next fluid cell build, then the next wall pass
+        if (s->fluid.n) {
+            if ((rc = stage_open(s, FSA_STAGE_CELL, s->side))) return rc;
+            HIP_TRY(launch_cells_one(make_cell_job(s->P, s->fluid, s->jumps, true), s->side));
+            flip(s->fluid);
+            stage_end(s, FSA_STAGE_CELL, s->side);
+        }
+        if (s->P.nv) {
+            if ((rc = stage_open(s, FSA_STAGE_VIRT, s->side))) return rc;
+            HIP_TRY(launch_virt(s->P, virt_args(s, s->vx.normals_ready ? 0 : 1), s->side));
+            s->vx.normals_ready = true;
+            stage_end(s, FSA_STAGE_VIRT, s->side);
+        }
+        HIP_TRY(hipEventRecord(s->ev_join, s->side));
+        HIP_TRY(hipStreamWaitEvent(s->stream, s->ev_join, 0));
+        s->couple_fresh = false;  // the coupling records describe the old order
+        walls_done = true;
+    }
     return FSA_OK;
 }
 
 int fsa_step(fsa_sim *s, int n_steps) {
     CHECK_SIM(s);
     if (n_steps < 0) return set_err(FSA_ERR_INVALID, "negative step count");
+    if (n_steps > 1) return step_overlapped(s, n_steps);
     for (int i = 0; i < n_steps; ++i) {
         int rc;
         if ((rc = fsa_particle_cell_update(s)) || (rc = fsa_fill_cell(s)) || (rc = fsa_couple_calculation(s)) ||
@@ -833,7 +920,7 @@ int fsa_get_cell_start(fsa_sim *s, int kind, int32_t *out) {
     CHECK_SIM(s);
     KindBuffers *k = kind_of(s, kind);
     if (!k || !out) return set_err(FSA_ERR_INVALID, "bad kind or null output");
-    HIP_TRY(hipMemcpyAsync(out, k->start, sizeof(int32_t) * ((size_t)s->P.ncell + 1), hipMemcpyDeviceToHost,
+    HIP_TRY(hipMemcpyAsync(out, k->cur_start(), sizeof(int32_t) * ((size_t)s->P.ncell + 1), hipMemcpyDeviceToHost,
                            s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return FSA_OK;
@@ -911,7 +998,7 @@ static ObserveArgs observe_args(fsa_sim *s) {
     ObserveArgs a{};
     a.fpos = s->fluid.pos[s->fluid.st];
     a.fvel = s->fluid.vel[s->fluid.st];
-    a.fstart = s->fluid.start;
+    a.fstart = s->fluid.cur_start();
     a.spos = s->solid.pos[s->solid.st];
     a.svel = s->solid.vel[s->solid.st];
     a.sref = s->solid.ref[s->solid.ord];
@@ -965,9 +1052,9 @@ int fsa_pair_census(fsa_sim *s, int64_t counts[6]) {
     a.spos = s->solid.pos[s->solid.st];
     a.vpos = s->virt.pos[0];
     a.fcell = s->fluid.cell[s->fluid.ord];
-    a.fstart = s->fluid.start;
-    a.sstart = s->solid.start;
-    a.vstart = s->virt.start;
+    a.fstart = s->fluid.cur_start();
+    a.sstart = s->solid.cur_start();
+    a.vstart = s->virt.cur_start();
     HIP_TRY(launch_census(s->P, a, s->census, s->stream));
     unsigned long long v[6];
     HIP_TRY(hipMemcpyAsync(v, s->census, sizeof v, hipMemcpyDeviceToHost, s->stream));

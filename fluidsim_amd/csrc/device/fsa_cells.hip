@@ -278,8 +278,8 @@ CellJob make_cell_job(const DevParams &P, const KindBuffers &k, unsigned long lo
     j.key = k.key;
     j.rank = k.rank;
     j.count = k.count;
-    j.start = k.start;
-    j.fstart = k.fstart;
+    j.start = k.next_start();  // the build's starts become current when it flips the order
+    j.fstart = k.next_fstart();
     j.tmp_slot = k.tmp_slot;
     j.tmp_id = k.tmp_id;
     j.tile_sums = k.tile_sums;

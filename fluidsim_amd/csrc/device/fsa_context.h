@@ -32,6 +32,11 @@ struct fsa_sim {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int device = 0;
+    // fsa_step runs the next step's fluid cell build and wall pass here,
+    // beside the solid sub-steps on `stream` (forked from and joined back
+    // into it); created on the first step that uses it
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     fsa::KindBuffers fluid, solid, virt;
     fsa::SolidExtra sx;

@@ -102,9 +102,19 @@ struct KindBuffers {
     float4 *ref[2] = {nullptr, nullptr};  // solid only
     int32_t *id[2] = {nullptr, nullptr};
     int32_t *cell[2] = {nullptr, nullptr};
-    int nsub = 1;               // sort keys per cell: kSubZ (fluid) or 1
-    int32_t *start = nullptr;   // ncell + 1: slots of cell c are [start[c], start[c+1])
-    int32_t *fstart = nullptr;  // ncell * nsub + 1: slots of key k are [fstart[k], fstart[k+1]) (start if nsub is 1)
+    int nsub = 1;  // sort keys per cell: kSubZ (fluid) or 1
+    // per-cell starts, ncell + 1: slots of cell c are [start[c], start[c+1]);
+    // per-key starts, ncell * nsub + 1: slots of key k are [fstart[k],
+    // fstart[k+1]) (the per-cell ones if nsub is 1).  Indexed like id/cell
+    // by `ord`: a build writes the other pair, so that passes still reading
+    // the current order may run beside the next build.  Kinds whose passes
+    // never overlap a build keep one pair (both entries alike).
+    int32_t *start_buf[2] = {nullptr, nullptr};
+    int32_t *fstart_buf[2] = {nullptr, nullptr};
+    int32_t *cur_start() const { return start_buf[ord]; }
+    int32_t *cur_fstart() const { return fstart_buf[ord]; }
+    int32_t *next_start() const { return start_buf[1 - ord]; }
+    int32_t *next_fstart() const { return fstart_buf[1 - ord]; }
     int32_t *count = nullptr;   // ncell * nsub padded to kScanTile
     int32_t *key = nullptr;    // cell of each slot of the current order
     int32_t *rank = nullptr;   // arrival slot inside the cell (atomic)

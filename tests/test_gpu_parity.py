@@ -314,6 +314,36 @@ def test_dense_cells_long_runs(gpu_available):
     assert ok, (e_gpu, e_32)
 
 
+@pytest.mark.parametrize("scene,steps,form", [("tiny", 12, 0), ("c1", 20, 0), ("c1", 20, 3)])
+def test_overlapped_steps_match_the_stage_sequence(gpu_available, scene, steps, form):
+    """fsa_step runs the next step's fluid cell build and wall pass beside the
+    solid sub-steps; the six stage calls made one by one give the same bits."""
+    sc = (scenes.tiny_scene(seed=5) if scene == "tiny" else scenes.c1_cpu_scene(seed=2)).with_(precision_form=form)
+    a = sc.arrays()
+    p = sc.params((len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])))
+    print(f"case {scene} form {form}: {p.n_fluid} fluid, {p.n_solid} solid, {p.n_virt} walls, {steps} steps",
+          flush=True)
+    fused = DeviceSim(p)
+    fused.upload(a)
+    fused.step(steps)
+    staged = DeviceSim(p)
+    staged.upload(a)
+    for _ in range(steps):
+        staged.particle_cell_update()
+        staged.fill_cell()
+        staged.couple_calculation()
+        staged.virt_update()
+        staged.fluid_calculation()
+        staged.solid_calculation()
+    for got, want in ((fused.fluid(), staged.fluid()), (fused.solid(), staged.solid()), (fused.virt(), staged.virt())):
+        for key in want:
+            np.testing.assert_array_equal(got[key], want[key], err_msg=key)
+    for kind in (FLUID, SOLID):
+        np.testing.assert_array_equal(fused.sorted_ids(kind), staged.sorted_ids(kind))
+        np.testing.assert_array_equal(fused.cell_start(kind), staged.cell_start(kind))
+    assert fused.jump_count() == staged.jump_count()
+
+
 def test_teleported_particle_is_kept_and_counted(gpu_available, tiny):
     a = tiny.arrays()
     p = tiny.params((len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])))
