@@ -183,7 +183,9 @@ int alloc_all(fsa_sim *s) {
     // which read the fluid cells of the current order
     if ((rc = alloc_kind(s, s->fluid, s->P.nf, false, true, kSubZ, comp, packed, true))) return rc;
     if ((rc = alloc_kind(s, s->solid, s->P.ns, true, true, 1, comp, false))) return rc;
-    if ((rc = alloc_kind(s, s->virt, s->P.nv, false, false, 1))) return rc;
+    // walls never move: their one sort (at upload) takes the fluid's keys, so
+    // that the fluid pass reads wall partners through z-windows as well
+    if ((rc = alloc_kind(s, s->virt, s->P.nv, false, false, kSubZ))) return rc;
     const int ns = s->P.ns;
     if ((rc = s->alloc(&s->sx.np, (size_t)ns + kPosPad)) || (rc = s->alloc(&s->sx.acc, ns)) ||
         (rc = s->alloc(&s->sx.couple, ns)) ||
@@ -362,6 +364,7 @@ FluidStepArgs fsa::fluid_args(const fsa_sim *s, const KindBuffers &f, const Kind
     a.ffine = f.cur_fstart();
     a.sstart = so.cur_start();
     a.vstart = s->virt.cur_start();
+    a.vfine = s->virt.cur_fstart();
     a.spos = so.pos[so.st];
     a.svel = so.vel[so.st];
     a.snp = s->sx.np;

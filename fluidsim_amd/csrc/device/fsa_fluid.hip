@@ -405,10 +405,13 @@ __device__ __forceinline__ void z_window(const DevParams &P, const Columns &cols
 }
 
 // all partners of one kind (n particles): the 9 columns in order, then one
-// evaluation; fluid partners are read from each column's z-window only.  The
+// evaluation; fluid and wall partners are read from each column's z-window
+// only (solid partners, sorted by cell alone, from the whole column).  The
 // slot ranges of all 9 columns are taken first (their loads in flight
 // together); a kind with no candidate for any lane of the wave is skipped,
-// and so is a column without one.
+// and so is a column without one.  Walls are far from most particles: their
+// whole-column ranges (per-cell starts) decide that first, and only a wave
+// with walls in reach takes their z-windows.
 template <int KIND, int T, bool PACK>
 __device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cell, int n,
                                      const int32_t *__restrict__ start,
@@ -416,7 +419,14 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cel
                                      const float4 *__restrict__ vel, const float4 *__restrict__ np,
                                      const PackedSrc &ps, const Lists<T> &L, Acc &acc) {
     int cb[kColumns], ce[kColumns];
-    if (KIND == kFluidFluid) {
+    if (KIND != kFluidFluid) {
+        column_ranges(P, cell, start, cb, ce);
+        bool any = false;
+#pragma unroll
+        for (int col = 0; col < kColumns; ++col) any |= cb[col] < ce[col];
+        if (__ballot(any) == 0) return;
+    }
+    if (KIND != kFluidSolid) {
         const Columns cols(P, cell);
         Window w;
         const float tx = (me.x - P.cmin[0]) * P.inv[0], ty = (me.y - P.cmin[1]) * P.inv[1];
@@ -428,13 +438,13 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cel
         w.mz = fmaf(w.tzs, 3.0e-7f, 1.0e-3f);
 #pragma unroll
         for (int col = 0; col < kColumns; ++col) z_window(P, cols, fine, col, w, cb[col], ce[col]);
-    } else {
-        column_ranges(P, cell, start, cb, ce);
     }
-    bool any = false;
+    if (KIND == kFluidFluid) {
+        bool any = false;
 #pragma unroll
-    for (int col = 0; col < kColumns; ++col) any |= cb[col] < ce[col];
-    if (__ballot(any) == 0) return;
+        for (int col = 0; col < kColumns; ++col) any |= cb[col] < ce[col];
+        if (__ballot(any) == 0) return;
+    }
     int cnt = 0, spare = 0;
     L.base[kSentinelSlot * T] = slot_bytes(n);
 #pragma unroll 1
@@ -487,7 +497,7 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     const PackedSrc fsrc{a.fpk, a.fcell}, none{nullptr, nullptr};
     pass<kFluidFluid, T, PACK>(P, me, cell, P.nf, a.fstart, a.ffine, a.fpos, a.fvel, nullptr, fsrc, list, acc);
     pass<kFluidSolid, T, false>(P, me, cell, P.ns, a.sstart, nullptr, a.spos, a.svel, a.snp, none, list, acc);
-    pass<kFluidVirt, T, false>(P, me, cell, P.nv, a.vstart, nullptr, a.vpos, a.vvel, nullptr, none, list, acc);
+    pass<kFluidVirt, T, false>(P, me, cell, P.nv, a.vstart, a.vfine, a.vpos, a.vvel, nullptr, none, list, acc);
 
     // fluidUpdate (fluid.cu:234-269): NaN scrub, explicit step with the old
     // velocity for the position, density clamp, EOS, domain clamp, speed cap

@@ -48,11 +48,13 @@ constexpr float kFarPos = 1.0e15f;
 // slot byte offsets are 32-bit (slot * 16 B)
 constexpr int kMaxKindParticles = (1 << 28) - kPosPad;
 constexpr int kSolidLanes = 16;  // lanes serving one solid particle
-// Inside a cell, fluid particles are sorted by an eighth-cell z index (then
-// by .dat index): their sort key is cell * kSubZ + sub.  The per-cell start
-// array every pass uses is taken from the per-key one, which the fluid pass
-// also reads to test only the z-range of a column its cut-off reaches.
-// Solid and wall particles keep one key per cell (cell, then .dat index).
+// Inside a cell, fluid and wall particles are sorted by an eighth-cell z
+// index (then by .dat index): their sort key is cell * kSubZ + sub.  The
+// per-cell start array every pass uses is taken from the per-key one, which
+// the fluid pass also reads to test only the z-range of a column its cut-off
+// reaches.  Solid particles keep one key per cell (cell, then .dat index):
+// their cell build runs every step, and the per-key arrays would cost more
+// than the few fluid particles near the gate save.
 constexpr int kSubZ = 8;
 // solid-solid partners recorded per lane in the first sub-step of a step
 // (entries per lane; a lane with more partners walks its candidates instead)
@@ -219,7 +221,7 @@ struct FluidStepArgs {
     const uint4 *fpk;                // FP16 partner records (null unless packed)
     const int32_t *fcell;
     const int32_t *fstart, *sstart, *vstart;
-    const int32_t *ffine;            // per-key starts of the fluid (kSubZ keys per cell)
+    const int32_t *ffine, *vfine;    // per-key starts of the fluid and the walls (kSubZ keys per cell)
     const float4 *spos, *svel, *snp;
     const float4 *vpos, *vvel;
     float4 *fpos_out, *fvel_out;

@@ -5,10 +5,10 @@
 //
 // Templated on the arithmetic type T (float = the reference's FP32, double =
 // the FP64 precision oracle) and on the neighbour traversal:
-//   CANONICAL  fluid sorted by (cell id, eighth-cell z index inside the
-//              cell, .dat index) -- the eighth-cell index is (int)(8 t) -
-//              8 (int)t with t = (z - cell_min_z) / interval_z -- and solid
-//              and wall particles by (cell id, .dat index), the key order of
+//   CANONICAL  fluid and wall particles sorted by (cell id, eighth-cell z
+//              index inside the cell, .dat index) -- the eighth-cell index
+//              is (int)(8 t) - 8 (int)t with t = (z - cell_min_z) /
+//              interval_z -- and solid particles by (cell id, .dat index), the key order of
 //              the GPU's cell build; for a particle in cell
 //              (cx,cy,cz) the partners are walked kind by kind (fluid, solid,
 //              virt), and for each kind the 27 neighbour cells as 9 columns
@@ -455,7 +455,7 @@ template <class T> struct Oracle : OracleBase {
 #pragma omp parallel for schedule(static)
             for (int i = 0; i < n; ++i) {
                 cell[k][i] = find_cell(x[i]);
-                const int ns = k == KF ? kSubZ : 1;
+                const int ns = k == KS ? 1 : kSubZ;
                 const T tz = (x[i].z - P.cmin[2]) * P.inv[2];
                 const int iz = std::min(std::max((int)tz, 0), P.cn[2] - 1);
                 sub[k][i] = std::min(std::max((int)(tz * T(ns)) - ns * iz, 0), ns - 1);
@@ -472,7 +472,7 @@ template <class T> struct Oracle : OracleBase {
             // stable counting sort by (cell, eighth-cell) over ascending .dat index
             for (int k = 0; k < 3; ++k) {
                 const int n = count(k);
-                const int ns = k == KF ? kSubZ : 1;
+                const int ns = k == KS ? 1 : kSubZ;
                 const int nk = P.ncell * ns;
                 std::vector<int> fstart(nk + 1, 0);
                 for (int i = 0; i < n; ++i) fstart[cell[k][i] * ns + sub[k][i] + 1]++;

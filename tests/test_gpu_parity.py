@@ -241,7 +241,7 @@ def test_showcase_scale_integer_invariants(gpu_available):
         expect = (idx[:, 0] * n[1] + idx[:, 1]) * n[2] + idx[:, 2]
         cells = dev.cell_ids(kind)
         np.testing.assert_array_equal(cells, expect)
-        # slots: by cell, then (fluid) eighth-cell z index inside the cell, then .dat index
+        # slots: by cell, then (fluid and walls) eighth-cell z index inside the cell, then .dat index
         tz = (x[:, 2].astype(np.float32) - cmin[2]) * inv[2]
         sub = np.clip((tz * np.float32(8)).astype(np.int64) - 8 * idx[:, 2], 0, 7) if kind == FLUID else 0 * idx[:, 2]
         order = dev.sorted_ids(kind)

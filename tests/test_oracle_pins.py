@@ -60,7 +60,7 @@ def test_cell_ids_match_the_float32_formula(tiny):
 
 
 def test_canonical_cell_lists_are_a_stable_sort(tiny):
-    """Slots are sorted by cell, then (fluid only) by the eighth-cell z index
+    """Slots are sorted by cell, then (fluid and walls) by the eighth-cell z index
     inside the cell ((int)(8 t) - 8 (int)t, t the float32 z coordinate in
     cells), then by .dat index; the per-cell starts count each cell's members."""
     sc, p, a = tiny
@@ -76,10 +76,10 @@ def test_canonical_cell_lists_are_a_stable_sort(tiny):
         tz = (a[key][:, 2].astype(np.float32) - cmin) * inv
         iz = tz.astype(np.int32)
         sub = np.clip((tz * np.float32(8)).astype(np.int32) - 8 * iz, 0, 7)
-        if kind == FLUID:
-            assert sub.max() == 7
-        else:
+        if kind == SOLID:
             sub = np.zeros_like(sub)
+        else:
+            assert sub.max() == 7
         np.testing.assert_array_equal(o.sorted_ids(kind), np.lexsort((ids, sub, cell)))
         start = np.concatenate([[0], np.cumsum(np.bincount(cell, minlength=p.cell_number_total))])
         np.testing.assert_array_equal(o.cell_start(kind), start)
