@@ -200,6 +200,7 @@ int alloc_all(fsa_sim *s) {
     // the sentinel solid partner's normal and pressure are zero
     if (!s->dry && ns > 0) HIP_TRY(hipMemsetAsync(s->sx.np, 0, sizeof(float4) * ((size_t)ns + kPosPad), s->stream));
     if ((rc = s->alloc(&s->vx.nrm, s->P.nv)) || (rc = s->alloc(&s->virt_x, s->P.nv))) return rc;
+    if (s->P.nv > 0 && (rc = s->alloc(&s->vx.reach, (size_t)s->P.ncell))) return rc;
     if ((rc = s->alloc(&s->jumps, 1))) return rc;
     if ((rc = s->alloc(&s->obs_partial, (size_t)observe_partials())) || (rc = s->alloc(&s->obs_out, kObserveSums)))
         return rc;
@@ -373,6 +374,8 @@ FluidStepArgs fsa::fluid_args(const fsa_sim *s, const KindBuffers &f, const Kind
     a.fpos_out = f.pos[1 - f.st];
     a.fvel_out = f.vel[1 - f.st];
     a.rates = nullptr;
+    a.scell = so.n ? so.cell[so.ord] : nullptr;
+    a.vreach = s->P.nv ? s->vx.reach : nullptr;
     return a;
 }
 
@@ -599,6 +602,8 @@ int fsa_upload_virt(fsa_sim *s, const float *x, const int32_t *type) {
             hipMemcpyAsync(k.id[0], id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s->stream) != hipSuccess)
             err = set_err(FSA_ERR_HIP, "wall upload copy failed");
         if (!err) err = build_cells(s, k, false);  // gathers into pos[0]/vel[0]
+        if (!err && launch_reach(s->P, k.cur_start(), s->vx.reach, s->stream) != hipSuccess)
+            err = set_err(FSA_ERR_HIP, "wall reach launch failed");
         if (!err && hipMemcpyAsync(s->virt_x, k.pos[0], sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream) !=
                         hipSuccess)
             err = set_err(FSA_ERR_HIP, "wall position copy failed");

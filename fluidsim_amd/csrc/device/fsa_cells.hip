@@ -43,6 +43,19 @@ struct OpFillCountsImpl {
     }
 };
 
+// per cell: does one of its 27 cells hold a particle of the kind
+__global__ __launch_bounds__(kBlock) void k_reach(DevParams P, const int32_t *__restrict__ start,
+                                                  uint8_t *__restrict__ reach) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= P.ncell) return;
+    int b[9], e[9];
+    column_ranges(P, c, start, b, e);
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) any |= b[k] < e[k];
+    reach[c] = any ? 1 : 0;
+}
+
 struct OpHashImpl {
     static __device__ void run(const CellJob &j, int blk, int) {
         const int s = blk * kBlock + threadIdx.x;
@@ -296,6 +309,12 @@ hipError_t launch_fill_i32(int32_t *p, int32_t v, int n, hipStream_t s) {
 hipError_t launch_fill_f4(float4 *p, float4 v, int n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_f4, dim3(blocks_for(n)), dim3(kBlock), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_reach(const DevParams &P, const int32_t *start, uint8_t *reach, hipStream_t s) {
+    if (P.ncell <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reach, dim3(blocks_for(P.ncell)), dim3(kBlock), 0, s, P, start, reach);
     return hipGetLastError();
 }
 

@@ -409,15 +409,27 @@ __device__ __forceinline__ void z_window(const DevParams &P, const Columns &cols
 // only (solid partners, sorted by cell alone, from the whole column).  The
 // slot ranges of all 9 columns are taken first (their loads in flight
 // together); a kind with no candidate for any lane of the wave is skipped,
-// and so is a column without one.  Walls are far from most particles: their
-// whole-column ranges (per-cell starts) decide that first, and only a wave
+// and so is a column without one.  Solid and wall particles are far from
+// most fluid particles.  The solid's slots are sorted by cell id, whose
+// leading digit is the cell's x index, so its first and last slots bound the
+// solid in x; the walls have a per-cell reach flag.  Either decides without
+// loads of column ranges; then the whole-column ranges do, and only a wave
 // with walls in reach takes their z-windows.
 template <int KIND, int T, bool PACK>
 __device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cell, int n,
                                      const int32_t *__restrict__ start,
                                      const int32_t *__restrict__ fine, const float4 *__restrict__ pos,
                                      const float4 *__restrict__ vel, const float4 *__restrict__ np,
-                                     const PackedSrc &ps, const Lists<T> &L, Acc &acc) {
+                                     const PackedSrc &ps, const Lists<T> &L, Acc &acc,
+                                     const int32_t *__restrict__ sorted_cells = nullptr,
+                                     const uint8_t *__restrict__ reach = nullptr) {
+    if (KIND == kFluidSolid && sorted_cells) {
+        const int nyz = P.cn[1] * P.cn[2];
+        const int lo = sorted_cells[0] / nyz, hi = sorted_cells[n - 1] / nyz;
+        const int cx = cell / nyz;
+        if (__ballot(cx + 1 >= lo && cx - 1 <= hi) == 0) return;
+    }
+    if (KIND == kFluidVirt && reach && __ballot(reach[cell] != 0) == 0) return;
     int cb[kColumns], ce[kColumns];
     if (KIND != kFluidFluid) {
         column_ranges(P, cell, start, cb, ce);
@@ -496,8 +508,10 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     // partners kind by kind: fluid, solid, wall (the canonical order)
     const PackedSrc fsrc{a.fpk, a.fcell}, none{nullptr, nullptr};
     pass<kFluidFluid, T, PACK>(P, me, cell, P.nf, a.fstart, a.ffine, a.fpos, a.fvel, nullptr, fsrc, list, acc);
-    pass<kFluidSolid, T, false>(P, me, cell, P.ns, a.sstart, nullptr, a.spos, a.svel, a.snp, none, list, acc);
-    pass<kFluidVirt, T, false>(P, me, cell, P.nv, a.vstart, a.vfine, a.vpos, a.vvel, nullptr, none, list, acc);
+    pass<kFluidSolid, T, false>(P, me, cell, P.ns, a.sstart, nullptr, a.spos, a.svel, a.snp, none, list, acc,
+                                a.scell);
+    pass<kFluidVirt, T, false>(P, me, cell, P.nv, a.vstart, a.vfine, a.vpos, a.vvel, nullptr, none, list, acc,
+                               nullptr, a.vreach);
 
     // fluidUpdate (fluid.cu:234-269): NaN scrub, explicit step with the old
     // velocity for the position, density clamp, EOS, domain clamp, speed cap

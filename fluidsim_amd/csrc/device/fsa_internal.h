@@ -154,6 +154,9 @@ struct SolidExtra {
 struct VirtExtra {
     float4 *nrm = nullptr;
     bool normals_ready = false;
+    // per cell: 1 if a wall particle lies in one of its 27 cells (the walls
+    // never move: set once at their upload)
+    uint8_t *reach = nullptr;
 };
 
 __host__ __device__ inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
@@ -212,6 +215,8 @@ hipError_t launch_hash_one(const CellJob &j, hipStream_t s);     // fill + hash
 hipError_t launch_sort_one(const CellJob &j, hipStream_t s);     // scan + place + gather
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int n, hipStream_t s);
 hipError_t launch_fill_f4(float4 *p, float4 v, int n, hipStream_t s);
+// per cell: 1 if a particle of the kind with starts `start` lies in one of its 27 cells
+hipError_t launch_reach(const DevParams &P, const int32_t *start, uint8_t *reach, hipStream_t s);
 
 // ---- fluid pass (fsa_fluid.hip) -----------------------------------------------
 struct FluidStepArgs {
@@ -226,6 +231,8 @@ struct FluidStepArgs {
     const float4 *vpos, *vvel;
     float4 *fpos_out, *fvel_out;
     float4 *rates;  // (ax, ay, az, drho) or null
+    const int32_t *scell;   // the solid's cells in slot order (null without solid particles)
+    const uint8_t *vreach;  // per cell: walls within its 27 cells (null without walls)
 };
 struct FluidJob {
     DevParams P;
