@@ -191,7 +191,11 @@ int fsa_virt_update(fsa_sim *sim);
 int fsa_fluid_calculation(fsa_sim *sim);
 /* K10-K12 x solid_sub_step (src/solid.cu:10-260). */
 int fsa_solid_calculation(fsa_sim *sim);
-/* The whole step loop body of src/calculation.cu:23-33, n_steps times. */
+/* The whole step loop body of src/calculation.cu:23-33, n_steps times.  With
+ * n_steps > 1 the next step's fluid cell build and wall pass run on an
+ * internal second stream beside the solid sub-steps (forked from and joined
+ * back into the context's stream, so the call is still ordered on it); the
+ * results are those of the stage calls made one by one, bit for bit. */
 int fsa_step(fsa_sim *sim, int n_steps);
 /* Wait for all work enqueued on the context stream. */
 int fsa_synchronize(fsa_sim *sim);
