@@ -310,6 +310,19 @@ void stage_end(fsa_sim *s, int st, hipStream_t on = nullptr) {
         if (!(s)) return set_err(FSA_ERR_INVALID, "null context");                     \
     } while (0)
 
+// The side stream and its fork/join events.  fsa_create makes them right
+// after the context's stream: streams share the device's few hardware queues
+// in creation order, and a side stream made much later (after the streams of
+// many other contexts) can land on its context stream's queue, where the two
+// run one after the other.
+int ensure_side(fsa_sim *s) {
+    if (s->side) return FSA_OK;
+    HIP_TRY(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
+    return FSA_OK;
+}
+
 }  // namespace
 
 int fsa::require_uploaded(fsa_sim *s) {
@@ -435,6 +448,7 @@ int fsa_create(const fsa_params *params, void *stream, fsa_sim **out) {
         fsa_destroy(s);
         return code;
     };
+    if ((rc = ensure_side(s))) return fail(rc);
     if ((rc = alloc_all(s))) return fail(rc);
     if (hipMemsetAsync(s->jumps, 0, sizeof(unsigned long long), s->stream) != hipSuccess)
         return fail(set_err(FSA_ERR_HIP, "hipMemsetAsync failed"));
@@ -821,14 +835,6 @@ int fsa_solid_calculation(fsa_sim *s) {
     return FSA_OK;
 }
 
-// The side stream and its fork/join events, made on first use.
-static int ensure_side(fsa_sim *s) {
-    if (s->side) return FSA_OK;
-    HIP_TRY(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
-    return FSA_OK;
-}
 
 // Steps in the reference's stage order (src/calculation.cu:23-33), with the
 // head of step i+1 moved beside the tail of step i: the fluid cell build and
