@@ -358,6 +358,7 @@ VirtArgs fsa::virt_args(const fsa_sim *s, const KindBuffers &f, int compute_norm
     a.vcell = s->virt.cell[s->virt.ord];
     a.fstart = f.cur_start();
     a.vstart = s->virt.cur_start();
+    a.ffine = f.cur_fstart();
     a.fpos = f.pos[f.st];
     a.fvel = f.vel[f.st];
     a.compute_normals = compute_normals;

@@ -342,68 +342,6 @@ __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col
     }
 }
 
-// The 9 columns of cell c (the order of for_each_column) as cell ranges
-// [c0, c1) of one z-run each; columns outside the grid are the empty range
-// [0, 0).
-struct Columns {
-    int cx, cy, zlo, zhi1, ny, nz, nx;
-    __device__ __forceinline__ Columns(const DevParams &P, int c) {
-        ny = P.cn[1];
-        nz = P.cn[2];
-        nx = P.cn[0];
-        const int nyz = ny * nz;
-        cx = c / nyz;
-        const int rem = c - cx * nyz;
-        cy = rem / nz;
-        const int cz = rem - cy * nz;
-        zlo = max(cz - 1, 0);
-        zhi1 = min(cz + 1, nz - 1) + 1;
-    }
-    __device__ __forceinline__ bool inside(int col) const {
-        const int gx = cx + col / 3 - 1, gy = cy + col % 3 - 1;
-        return (unsigned)gx < (unsigned)nx && (unsigned)gy < (unsigned)ny;
-    }
-    __device__ __forceinline__ int base(int col) const { return ((cx + col / 3 - 1) * ny + (cy + col % 3 - 1)) * nz; }
-    __device__ __forceinline__ int c0(int col) const { return inside(col) ? base(col) + zlo : 0; }
-    __device__ __forceinline__ int c1(int col) const { return inside(col) ? base(col) + zhi1 : 0; }
-};
-
-// Where a fluid particle sits, for its z-windows: fx, fy its position in its
-// cell in cells, tzs its z in sub-cells (1/kSubZ cell), and the margins that
-// cover the rounding of cell coordinates.  A coordinate t = (x - cell_min) /
-// interval carries a relative error below 2^-23 (two roundings), on the
-// particle's side and on the partner's, so the margins grow with t: 1e-4
-// cell + 3e-7 t across x and y, 1e-3 sub-cell + 3e-7 t in z.
-struct Window {
-    float fx, fy, tzs, mx, my, mz;
-};
-
-// The z-window of column `col` (fluid partners): inside a cell the slots
-// are sorted by their sub-cell z index, so a column's slots with z in a
-// range are the contiguous key range of that z-range.  The column is at
-// least (gx, gy) away from the particle across x and y (its distance to the
-// cell faces the column lies beyond, less the margin), so partners within R
-// lie within rz = sqrt(R^2 (1 + 1e-5) - gx^2 - gy^2) in z; the window is the
-// sub-cells of [z - rz, z + rz], widened by the z margin.  The exact cut-off
-// test still runs on every slot of the window, so the partner list is the
-// whole column's, in the same order.
-__device__ __forceinline__ void z_window(const DevParams &P, const Columns &cols, const int32_t *__restrict__ fine,
-                                         int col, const Window &w, int &b, int &e) {
-    const int ox = col / 3 - 1, oy = col % 3 - 1;
-    const float ex = ox < 0 ? w.fx : (ox > 0 ? 1.0f - w.fx : 0.0f);
-    const float ey = oy < 0 ? w.fy : (oy > 0 ? 1.0f - w.fy : 0.0f);
-    const float gx = fmaxf(ex - w.mx, 0.0f) * P.ival[0], gy = fmaxf(ey - w.my, 0.0f) * P.ival[1];
-    const float r2 = P.R2w - gx * gx - gy * gy;
-    b = e = 0;
-    if (!cols.inside(col) || !(r2 > 0.0f)) return;
-    const float rzs = sqrtf(r2) * P.inv[2] * (float)kSubZ + w.mz;
-    const int lo = max((int)floorf(w.tzs - rzs), cols.zlo * kSubZ);
-    const int hi = max(min((int)floorf(w.tzs + rzs) + 1, cols.zhi1 * kSubZ), lo);
-    const int k0 = cols.base(col) * kSubZ;
-    b = fine[k0 + lo];
-    e = fine[k0 + hi];
-}
-
 // all partners of one kind (n particles): the 9 columns in order, then one
 // evaluation; fluid and wall partners are read from each column's z-window
 // only (solid partners, sorted by cell alone, from the whole column).  The
@@ -440,16 +378,9 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cel
     }
     if (KIND != kFluidSolid) {
         const Columns cols(P, cell);
-        Window w;
-        const float tx = (me.x - P.cmin[0]) * P.inv[0], ty = (me.y - P.cmin[1]) * P.inv[1];
-        w.fx = tx - (float)cols.cx;
-        w.fy = ty - (float)cols.cy;
-        w.tzs = (me.z - P.cmin[2]) * P.inv[2] * (float)kSubZ;
-        w.mx = fmaf(tx, 3.0e-7f, 1.0e-4f);
-        w.my = fmaf(ty, 3.0e-7f, 1.0e-4f);
-        w.mz = fmaf(w.tzs, 3.0e-7f, 1.0e-3f);
+        const Window w(P, cols, me.x, me.y, me.z);
 #pragma unroll
-        for (int col = 0; col < kColumns; ++col) z_window(P, cols, fine, col, w, cb[col], ce[col]);
+        for (int col = 0; col < kColumns; ++col) z_window(P, cols, fine, col, w, P.R2w, cb[col], ce[col]);
     }
     if (KIND == kFluidFluid) {
         bool any = false;

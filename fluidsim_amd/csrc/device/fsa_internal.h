@@ -249,6 +249,7 @@ struct VirtArgs {
     float4 *vpos, *vvel, *vnrm;
     const int32_t *vcell;
     const int32_t *fstart, *vstart;
+    const int32_t *ffine;  // per-key starts of the fluid (kSubZ keys per cell)
     const float4 *fpos, *fvel;
     int compute_normals;
 };

@@ -31,14 +31,26 @@ __device__ __forceinline__ void virt_body(const DevParams &P, const VirtArgs &a,
     // first in traversal order wins ties; kScan loads in flight, reads past
     // the run end stay in the padded array and are masked.  Most walls have
     // no fluid in reach: their column ranges come back empty in one round.
+    // Once a candidate is found, a later column is read only in its z-window
+    // within the distance found so far (the fluid is sorted by eighth-cell z
+    // inside a cell): the slots left out are farther, and a candidate that is
+    // not strictly nearer never replaces the one found, so the result is the
+    // whole columns' walk.
     {
         int cb[9], ce[9];
         column_ranges(P, cell, a.fstart, cb, ce);
+        bool any = false;
+#pragma unroll
+        for (int col = 0; col < 9; ++col) any |= cb[col] < ce[col];
+        const Columns cols(P, cell);
+        const Window w(P, cols, xi, yi, zi);
         constexpr int kScan = 8;
         static_assert(kPosPad >= kScan, "padded position arrays");
 #pragma unroll 1
-        for (int col = 0; col < 9; ++col) {
-            for (int j = cb[col], e = ce[col]; j < e; j += kScan) {
+        for (int col = 0; any && col < 9; ++col) {
+            int b = cb[col], e = ce[col];
+            if (b < e && nearest < kLargeFloat) z_window(P, cols, a.ffine, col, w, nearest * (1.0f + 1.0e-5f), b, e);
+            for (int j = b; j < e; j += kScan) {
                 float4 q[kScan];
                 const float4 *src = a.fpos + j;
 #pragma unroll
