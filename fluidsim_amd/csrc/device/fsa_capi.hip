@@ -46,6 +46,7 @@ DevParams make_dev_params(const fsa_params &p) {
     d.rho_max = p.fluid_max_density;
     d.viscid = p.fluid_is_viscid ? 1 : 0;
     d.rho_s = p.solid_reference_density;
+    d.rho_s_inv = p.solid_reference_density != 0.0f ? 1.0f / p.solid_reference_density : 0.0f;
     d.lambda = p.solid_lambda;
     d.mu = p.solid_nu;
     d.cs_s = p.solid_sound_speed;

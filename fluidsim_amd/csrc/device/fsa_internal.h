@@ -67,6 +67,7 @@ struct DevParams {
     float cs, rho0, rho0_inv, gamma, visc, rho2p, p_min, rho_min, rho_max;
     int viscid;
     float rho_s, lambda, mu, cs_s, alpha, beta;
+    float rho_s_inv;  // 1 / rho_s
     float dmin[3], dmax[3], cmin[3], inv[3];
     int cn[3];
     int ncell;

@@ -12,6 +12,12 @@ constexpr float kEpsForSum = 0.1f;
 constexpr float kEpsFloat = 1.0e-6f;
 constexpr float kLargeFloat = 1.0e6f;
 
+// hardware reciprocal, reciprocal square root and exp2 (about 1 ulp; the
+// IEEE forms are several instructions each)
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float hw_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
 // squared length fma(z,z,fma(y,y,x*x)): the one form used for every
 // glm::length2 of the reference, so candidate tests are bit-reproducible
 __device__ __forceinline__ float len2(float x, float y, float z) {

@@ -222,7 +222,7 @@ __device__ __forceinline__ void deformation_body(const DevParams &P, const Solid
     solid_partners<MODE>(P, a, i, L.sub, Xi, [&](int j, const float (&dX)[3], float d2) {
         const float4 xj = a.spos[j];
         const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
-        const float kc = expf(-(d2 * P.h_inv2)) * P.kd;
+        const float kc = hw_exp2(fmaf(-d2, P.kexp2, P.log2kd));  // kd exp(-|dX|^2/h^2)
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -266,7 +266,9 @@ __device__ __forceinline__ void deformation_body(const DevParams &P, const Solid
 #pragma unroll
         for (int r = 0; r < 3; ++r) a.x.PL[3 * (size_t)i + r] = make_float4(PL.m[r][0], PL.m[r][1], PL.m[r][2], 0.0f);
     }
-    // det(F) F^-T, the factor of the artificial-viscosity force
+    // det(F) F^-T L, the particle's factor of the artificial-viscosity force
+    // (whose pair term is ((det F F^-T visc) L) dX k, solid.cu:156: visc
+    // and k are scalars of the pair, so the matrix product is the particle's)
     const M3 Fi = minverse(F);
     const float dF = mdet(F);
     M3 A;
@@ -274,7 +276,7 @@ __device__ __forceinline__ void deformation_body(const DevParams &P, const Solid
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int c = 0; c < 3; ++c) A.m[r][c] = dF * Fi.m[c][r];
-    store9(a.x.A, n, i, A);
+    store9(a.x.A, n, i, mmul(A, Lc));
 }
 
 // ---- K11: solidForceCalculate (src/solid.cu:81-207) ----------------------
@@ -293,8 +295,7 @@ __device__ __forceinline__ void force_body(const DevParams &P, const SolidArgs &
         PLi.m[r][1] = row.y;
         PLi.m[r][2] = row.z;
     }
-    const M3 Li = load9(a.x.L, n, i);
-    const M3 Ai = load9(a.x.A, n, i);
+    const M3 ALi = load9(a.x.A, n, i);  // det(F_i) F_i^-T L_i
     float cx = 0.0f, cy = 0.0f, cz = 0.0f;  // couple_dudt
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     const float eps = 0.01f * P.h2;
@@ -308,19 +309,14 @@ __device__ __forceinline__ void force_body(const DevParams &P, const SolidArgs &
     const auto fluid_term = [&](const float4 &pj, const float4 &uj, float dx, float dy, float dz, float d2) {
         const float m = 2.0f * ((uj.x - ui.x) * np.x + (uj.y - ui.y) * np.y + (uj.z - ui.z) * np.z);
         const float uix = uj.x - m * np.x, uiy = uj.y - m * np.y, uiz = uj.z - m * np.z;
-        const float kc = expf(-(d2 * P.h_inv2)) * P.kd;
-        const float rhoj = pj.w, rhoi = rhoj;
-        float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-        if (sqrtf(d2) != 0.0f) {
-            const float r = 1.0f / sqrtf(d2);
-            ex = dx * r;
-            ey = dy * r;
-            ez = dz * r;
-        }
-        float ps = (rhoi * uj.w + rhoj * np.w) / (rhoi + rhoj);
-        const float du = (uix - uj.x) * ex + (uiy - uj.y) * ey + (uiz - uj.z) * ez;
-        if (du >= 0.0f) ps += rhoi * rhoj * du / (rhoi + rhoj) * P.cs;
-        const float s = -2.0f * ps / P.rho_s;
+        const float kc = hw_exp2(fmaf(-d2, P.kexp2, P.log2kd));
+        // rho_i = rho_j: p* = (p_f + p_s)/2 + [du>=0] rho du/2 c0
+        const float rho = pj.w;
+        const float r = d2 != 0.0f ? hw_rsq(d2) : 0.0f;  // 1/|dx|, e = dx r
+        float ps = 0.5f * (uj.w + np.w);
+        const float du = ((uix - uj.x) * dx + (uiy - uj.y) * dy + (uiz - uj.z) * dz) * r;
+        if (du >= 0.0f) ps += 0.5f * rho * du * P.cs;
+        const float s = -2.0f * ps * P.rho_s_inv;
         cx += (s * dx) * kc;
         cy += (s * dy) * kc;
         cz += (s * dz) * kc;
@@ -353,7 +349,8 @@ __device__ __forceinline__ void force_body(const DevParams &P, const SolidArgs &
     }
     // stress divergence + artificial viscosity (solidSolidForce, solid.cu:119-157)
     solid_partners<kReplay>(P, a, i, L.sub, Xi, [&](int j, const float (&dX)[3], float d2) {
-        const float kc = expf(-(d2 * P.h_inv2)) * P.kd;
+        const float kc = hw_exp2(fmaf(-d2, P.kexp2, P.log2kd));
+        const float ks = kc * P.rho_s_inv;
         float v[3];
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
@@ -362,7 +359,7 @@ __device__ __forceinline__ void force_body(const DevParams &P, const SolidArgs &
             float t = 0.0f;
 #pragma unroll
             for (int c = 0; c < 3; ++c) t += (pl[c] + PLi.m[r][c]) * dX[c];
-            v[r] = t * kc / P.rho_s;
+            v[r] = t * ks;
         }
         ax += v[0];
         ay += v[1];
@@ -370,20 +367,15 @@ __device__ __forceinline__ void force_body(const DevParams &P, const SolidArgs &
         const float4 xj = a.spos[j], uj = a.svel[j];
         const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
         const float dudx = (uj.x - ui.x) * dx + (uj.y - ui.y) * dy + (uj.z - ui.z) * dz;
-        const float nu = P.h * dudx / (len2(dx, dy, dz) + eps);
+        const float nu = P.h * dudx * hw_rcp(len2(dx, dy, dz) + eps);
         float visc = P.alpha * nu * P.cs_s;
         visc -= P.beta * nu * nu;
-        // ((((det F F^-T) visc) L_i) dX) k
-        M3 AV;
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) AV.m[r][c] = Ai.m[r][c] * visc;
-        const M3 AL = mmul(AV, Li);
+        // ((det F F^-T visc) L_i) dX k = (det F F^-T L_i) dX (visc k)
+        const float vk = visc * kc;
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-            const float t = AL.m[r][0] * dX[0] + AL.m[r][1] * dX[1] + AL.m[r][2] * dX[2];
-            v[r] = t * kc;
+            const float t = ALi.m[r][0] * dX[0] + ALi.m[r][1] * dX[1] + ALi.m[r][2] * dX[2];
+            v[r] = t * vk;
         }
         ax += v[0];
         ay += v[1];
