@@ -214,7 +214,8 @@ void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph
     t.solid_off = put(blob, solid.data(), solid.size());
 
     std::vector<int> blocks;
-    for (BatchOp op : {kBFillCounts, kBHash, kBScanTiles, kBScanSums, kBScanAdd, kBPlace, kBGather}) {
+    // (the counts are zero between builds: no fill)
+    for (BatchOp op : {kBHash, kBScanTiles, kBScanSums, kBScanAdd, kBPlace, kBGather}) {
         blocks.clear();
         for (const CellJob &j : cells) blocks.push_back(blocks_cells(op, j));
         add_launch(blob, t, Launch::kCells, op, blocks);
@@ -533,6 +534,11 @@ int fsa_batch_step(fsa_batch *b, fsa_sim *const *scenes, int n_scenes, int n_ste
         }
         b->chains.push_back(c);
         b->ev_chain.push_back(e);
+    }
+    // a scene hashed by a stage call but not sorted has counts to clear
+    for (fsa_sim *s : sims) {
+        int rc = drop_pending_hash(s);
+        if (rc) return rc;
     }
     // the chains start after the work already queued on the scenes' streams
     // and on the batch stream

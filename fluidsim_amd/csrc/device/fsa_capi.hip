@@ -142,6 +142,7 @@ int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_sta
         k.fstart_buf[1] = k.fstart_buf[0];
     }
     if ((rc = s->alloc(&k.count, padded))) return rc;
+    if (!s->dry) HIP_TRY(hipMemsetAsync(k.count, 0, sizeof(int32_t) * padded, s->stream));  // zero between builds
     if (n == 0) return FSA_OK;
     for (int b = 0; b < 2; ++b) {
         if (b == 0 || double_state) {
@@ -254,6 +255,8 @@ void flip(KindBuffers &k) {
 // full cell build of one kind: hash + count, scan, place + rank + gather
 int build_cells(fsa_sim *s, KindBuffers &k, bool count_jumps) {
     if (k.n == 0) return FSA_OK;
+    int rc = drop_pending_hash(s);
+    if (rc) return rc;
     HIP_TRY(launch_cells_one(make_cell_job(s->P, k, s->jumps, count_jumps), s->stream));
     flip(k);
     return FSA_OK;
@@ -325,6 +328,17 @@ int ensure_side(fsa_sim *s) {
 }
 
 }  // namespace
+
+// A stage-call hash (fsa_particle_cell_update) not yet consumed by a sort
+// leaves counts behind and ranks of the old order: zero the counts and drop
+// the hash (the next fill_cell then needs a new one).
+int fsa::drop_pending_hash(fsa_sim *s) {
+    if (!s->hashed) return FSA_OK;
+    for (KindBuffers *k : {&s->fluid, &s->solid})
+        if (k->n) HIP_TRY(launch_fill_i32(k->count, 0, padded_cells(s->P.ncell * k->nsub), s->stream));
+    s->hashed = false;
+    return FSA_OK;
+}
 
 int fsa::require_uploaded(fsa_sim *s) {
     const int n[3] = {s->P.nf, s->P.ns, s->P.nv};

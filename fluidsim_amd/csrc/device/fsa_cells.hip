@@ -112,55 +112,76 @@ __device__ __forceinline__ int wave_inclusive_scan(int v) {
     return v;
 }
 
-// exclusive scan of the per-thread totals of a 256-thread workgroup
+// exclusive scan of the per-thread totals of an NT-thread workgroup
+// (wave_tot: NT / 64 ints of LDS)
+template <int NT = kBlock>
 __device__ __forceinline__ int block_exclusive_scan(int v, int *wave_tot, int &block_total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int inc = wave_inclusive_scan(v);
     if (lane == 63) wave_tot[wave] = inc;
     __syncthreads();
-    int before = 0;
+    int before = 0, all = 0;
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) before += (w < wave) ? wave_tot[w] : 0;
-    block_total = wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+    for (int w = 0; w < NT / 64; ++w) {
+        const int t = wave_tot[w];
+        before += (w < wave) ? t : 0;
+        all += t;
+    }
+    block_total = all;
     return before + inc - v;
 }
 
-// pass 1: per-tile inclusive scan into start[1 + i], tile totals
+constexpr int kSumsThreads = 1024;  // the one workgroup that scans the tile totals
+
+constexpr int kPerThread = kScanTile / kBlock;  // counts per thread of a scan tile
+static_assert(kPerThread % 4 == 0, "scan tiles are read as int4");
+
+// the kPerThread counts of this thread in tile `blk` (count is padded to a
+// multiple of kScanTile with zeros)
+__device__ __forceinline__ void load_counts(const CellJob &j, int blk, int (&c)[kPerThread]) {
+    const int4 *src = reinterpret_cast<const int4 *>(j.count + (size_t)blk * kScanTile + threadIdx.x * kPerThread);
+#pragma unroll
+    for (int q = 0; q < kPerThread / 4; ++q) {
+        const int4 v = src[q];
+        c[4 * q] = v.x;
+        c[4 * q + 1] = v.y;
+        c[4 * q + 2] = v.z;
+        c[4 * q + 3] = v.w;
+    }
+}
+
+// pass 1: the total of each scan tile
 struct OpScanTilesImpl {
     static __device__ void run(const CellJob &j, int blk, int) {
         __shared__ int wave_tot[kBlock / 64];
-        const int n = j.nk;
         // a batched grid rounds each scene's block range up: whole blocks past
         // the last tile leave before the first barrier
-        if (blk >= scan_tiles(n)) return;
-        const int base = blk * kScanTile + threadIdx.x * 4;
-        // count is padded to a multiple of kScanTile with zeros
-        const int4 v = *reinterpret_cast<const int4 *>(j.count + base);
-        const int t0 = v.x, t1 = t0 + v.y, t2 = t1 + v.z, t3 = t2 + v.w;
-        int total;
-        const int ex = block_exclusive_scan(t3, wave_tot, total);
-        const int vals[4] = {ex + t0, ex + t1, ex + t2, ex + t3};
+        if (blk >= scan_tiles(j.nk)) return;
+        int c[kPerThread];
+        load_counts(j, blk, c);
+        int t = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (base + k < n) j.fstart[1 + base + k] = vals[k];
+        for (int k = 0; k < kPerThread; ++k) t += c[k];
+        int total;
+        (void)block_exclusive_scan(t, wave_tot, total);
         if (threadIdx.x == 0) j.tile_sums[blk] = total;
     }
 };
 
-// pass 2: one workgroup scans the tile totals (exclusive, in place)
+// pass 2: one workgroup of kSumsThreads scans the tile totals (exclusive, in place)
 struct OpScanSumsImpl {
     static __device__ void run(const CellJob &j, int blk, int) {
-        __shared__ int wave_tot[kBlock / 64];
+        __shared__ int wave_tot[kSumsThreads / 64];
         __shared__ int carry;
         if (blk != 0) return;  // padding blocks of a batched grid
         const int ntiles = scan_tiles(j.nk);
         if (threadIdx.x == 0) carry = 0;
         __syncthreads();
-        for (int b = 0; b < ntiles; b += kBlock) {
+        for (int b = 0; b < ntiles; b += kSumsThreads) {
             const int i = b + threadIdx.x;
             const int v = i < ntiles ? j.tile_sums[i] : 0;
             int total;
-            const int ex = block_exclusive_scan(v, wave_tot, total);
+            const int ex = block_exclusive_scan<kSumsThreads>(v, wave_tot, total);
             const int c = carry;
             if (i < ntiles) j.tile_sums[i] = c + ex;
             __syncthreads();
@@ -170,21 +191,39 @@ struct OpScanSumsImpl {
     }
 };
 
-// pass 3: add tile offsets; start[0] = 0.  With several keys per cell the
+// pass 3: the tile's inclusive scan plus the totals of the tiles before it
+// into fstart[1 + i]; fstart[0] = 0.  With several keys per cell the
 // per-cell starts are every nsub-th key start (cell c begins where its first
-// key does), written by the thread that finishes that key start.
+// key does), written by the thread that finishes that key start.  The counts
+// go back to zero, ready for the next build's hash.
 struct OpScanAddImpl {
     static __device__ void run(const CellJob &j, int blk, int) {
-        const int i = blk * kBlock + threadIdx.x;
-        if (i == 0) {
+        __shared__ int wave_tot[kBlock / 64];
+        if (blk >= scan_tiles(j.nk)) return;
+        int c[kPerThread];
+        load_counts(j, blk, c);
+        int t = 0;
+#pragma unroll
+        for (int k = 0; k < kPerThread; ++k) t += c[k];
+        int total;
+        int run = block_exclusive_scan(t, wave_tot, total) + j.tile_sums[blk];
+        const int base = blk * kScanTile + threadIdx.x * kPerThread;
+        if (blk == 0 && threadIdx.x == 0) {
             j.fstart[0] = 0;
             if (j.nsub > 1) j.start[0] = 0;
         }
-        if (i < j.nk) {
-            const int v = j.fstart[1 + i] + j.tile_sums[i / kScanTile];
-            j.fstart[1 + i] = v;
-            if (j.nsub > 1 && (i + 1) % j.nsub == 0) j.start[(i + 1) / j.nsub] = v;
+#pragma unroll
+        for (int k = 0; k < kPerThread; ++k) {
+            const int i = base + k;
+            run += c[k];
+            if (i < j.nk) {
+                j.fstart[1 + i] = run;
+                if (j.nsub > 1 && (i + 1) % j.nsub == 0) j.start[(i + 1) / j.nsub] = run;
+            }
         }
+        int4 *dst = reinterpret_cast<int4 *>(j.count + (size_t)base);
+#pragma unroll
+        for (int q = 0; q < kPerThread / 4; ++q) dst[q] = make_int4(0, 0, 0, 0);
     }
 };
 
@@ -240,7 +279,7 @@ int blocks_cells(BatchOp op, const CellJob &j) {
     case kBHash: return blocks_for(j.n);
     case kBScanTiles: return scan_tiles(j.nk);
     case kBScanSums: return 1;
-    case kBScanAdd: return blocks_for(j.nk);
+    case kBScanAdd: return scan_tiles(j.nk);
     case kBPlace: return blocks_for(j.n);
     case kBGather: return blocks_for(j.n);
     default: return 0;
@@ -255,7 +294,10 @@ hipError_t launch_many_cells(BatchOp op, const CellJob *jobs, const int *prefix,
     case kBFillCounts: hipLaunchKernelGGL((k_many<OpFillCountsImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
     case kBHash: hipLaunchKernelGGL((k_many<OpHashImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
     case kBScanTiles: hipLaunchKernelGGL((k_many<OpScanTilesImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
-    case kBScanSums: hipLaunchKernelGGL((k_many<OpScanSumsImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    case kBScanSums:
+        hipLaunchKernelGGL((k_many<OpScanSumsImpl, CellJob, kSumsThreads>), g, dim3(kSumsThreads), 0, s, jobs, prefix,
+                           njobs);
+        break;
     case kBScanAdd: hipLaunchKernelGGL((k_many<OpScanAddImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
     case kBPlace: hipLaunchKernelGGL((k_many<OpPlaceImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
     case kBGather: hipLaunchKernelGGL((k_many<OpGatherImpl, CellJob>), g, b, 0, s, jobs, prefix, njobs); break;
@@ -327,14 +369,17 @@ hipError_t launch_hash_one(const CellJob &j, hipStream_t s) {
 hipError_t launch_sort_one(const CellJob &j, hipStream_t s) {
     hipError_t e;
     if ((e = one<OpScanTilesImpl>(j, blocks_cells(kBScanTiles, j), s)) != hipSuccess) return e;
-    if ((e = one<OpScanSumsImpl>(j, blocks_cells(kBScanSums, j), s)) != hipSuccess) return e;
+    hipLaunchKernelGGL((k_one<OpScanSumsImpl, CellJob, kSumsThreads>), dim3(1), dim3(kSumsThreads), 0, s, j);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = one<OpScanAddImpl>(j, blocks_cells(kBScanAdd, j), s)) != hipSuccess) return e;
     if ((e = one<OpPlaceImpl>(j, blocks_cells(kBPlace, j), s)) != hipSuccess) return e;
     return one<OpGatherImpl>(j, blocks_cells(kBGather, j), s);
 }
 
+// the counts are zero between builds (allocation, then every sort's scan),
+// so a whole build starts with the hash
 hipError_t launch_cells_one(const CellJob &j, hipStream_t s) {
-    hipError_t e = launch_hash_one(j, s);
+    hipError_t e = one<OpHashImpl>(j, blocks_cells(kBHash, j), s);
     return e != hipSuccess ? e : launch_sort_one(j, s);
 }
 

@@ -103,6 +103,8 @@ namespace fsa {
 
 // FSA_ERR_STATE unless every kind with particles has been uploaded
 int require_uploaded(fsa_sim *s);
+// clears the counts a stage-call hash left without a sort (see fsa_capi.hip)
+int drop_pending_hash(fsa_sim *s);
 
 // stage arguments over given fluid/solid buffer indices (a batch plans a
 // step ahead of the context's own indices) ...

@@ -118,7 +118,7 @@ struct KindBuffers {
     int32_t *cur_fstart() const { return fstart_buf[ord]; }
     int32_t *next_start() const { return start_buf[1 - ord]; }
     int32_t *next_fstart() const { return fstart_buf[1 - ord]; }
-    int32_t *count = nullptr;   // ncell * nsub padded to kScanTile
+    int32_t *count = nullptr;   // ncell * nsub padded to kScanTile; zero between builds
     int32_t *key = nullptr;    // cell of each slot of the current order
     int32_t *rank = nullptr;   // arrival slot inside the cell (atomic)
     int32_t *tmp_slot = nullptr;
@@ -210,9 +210,10 @@ struct CellJob {
 };
 // the job of kind k reading the current buffers and writing the other ones
 CellJob make_cell_job(const DevParams &P, const KindBuffers &k, unsigned long long *jumps, bool count_jumps);
-// the whole build (fill, hash, scan, place, gather) for one job
+// the whole build (hash, scan, place, gather) for one job; the counts must
+// be zero on entry, as every sort leaves them
 hipError_t launch_cells_one(const CellJob &j, hipStream_t s);
-hipError_t launch_hash_one(const CellJob &j, hipStream_t s);     // fill + hash
+hipError_t launch_hash_one(const CellJob &j, hipStream_t s);     // fill (zero the counts) + hash
 hipError_t launch_sort_one(const CellJob &j, hipStream_t s);     // scan + place + gather
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int n, hipStream_t s);
 hipError_t launch_fill_f4(float4 *p, float4 v, int n, hipStream_t s);
