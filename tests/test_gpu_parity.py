@@ -13,7 +13,7 @@ import pytest
 import oracle as orc
 from fluidsim_amd import scenes
 from fluidsim_amd._native import FLUID, SOLID, VIRT
-from fluidsim_amd.sim import DeviceSim
+from fluidsim_amd.sim import DeviceBatch, DeviceSim
 
 from helpers import REL, assert_within, compare_fluid, max_abs, same_class, velocity_scale
 
@@ -342,6 +342,47 @@ def test_overlapped_steps_match_the_stage_sequence(gpu_available, scene, steps, 
         np.testing.assert_array_equal(fused.sorted_ids(kind), staged.sorted_ids(kind))
         np.testing.assert_array_equal(fused.cell_start(kind), staged.cell_start(kind))
     assert fused.jump_count() == staged.jump_count()
+
+
+def _same_state(got: DeviceSim, want: DeviceSim):
+    for g, w in ((got.fluid(), want.fluid()), (got.solid(), want.solid()), (got.virt(), want.virt())):
+        for key in w:
+            np.testing.assert_array_equal(g[key], w[key], err_msg=key)
+    for kind in (FLUID, SOLID):
+        np.testing.assert_array_equal(got.sorted_ids(kind), want.sorted_ids(kind))
+        np.testing.assert_array_equal(got.cell_start(kind), want.cell_start(kind))
+
+
+@pytest.mark.parametrize("then", ["hash_again", "upload", "batch"])
+def test_pending_hash_leaves_no_counts_behind(gpu_available, tiny, then):
+    """The per-key counts are zero between cell builds (every sort's scan
+    clears them).  A stage-call hash without its sort leaves counts behind:
+    hashing again, re-uploading or stepping in a batch must still give the
+    bits of a clean run."""
+    a = tiny.arrays()
+    p = tiny.params((len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])))
+    clean = DeviceSim(p)
+    clean.upload(a)
+    clean.step(4)
+    dev = DeviceSim(p)
+    dev.upload(a)
+    dev.particle_cell_update()
+    if then == "hash_again":
+        dev.particle_cell_update()
+        dev.fill_cell()
+        dev.couple_calculation()
+        dev.virt_update()
+        dev.fluid_calculation()
+        dev.solid_calculation()
+        dev.step(3)
+    elif then == "upload":
+        dev.upload(a)
+        dev.step(4)
+    else:
+        b = DeviceBatch()
+        b.step([dev], 4)
+        b.synchronize()
+    _same_state(dev, clean)
 
 
 def test_teleported_particle_is_kept_and_counted(gpu_available, tiny):
