@@ -203,7 +203,8 @@ def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: flo
     }
 
 
-def serve_bench(n_requests: int, seed: int, sim_time: float, load: float, chunk: int) -> dict:
+def serve_bench(n_requests: int, seed: int, sim_time: float, load: float, chunk: int,
+                saturation: int | None = None) -> dict:
     """Scene-solve requests with Poisson arrivals served by continuous batching
     (fluidsim_amd.serving.Server), against the same requests solved one at a
     time in arrival order (each scene's solve time measured alone, the FIFO
@@ -270,7 +271,8 @@ def serve_bench(n_requests: int, seed: int, sim_time: float, load: float, chunk:
     log(f"serve: batched rate {rate:.3g} particle-steps/s, mean service {mean_service:.2f}s, "
         f"lambda {lam:.3f}/s for load {load}")
 
-    srv = Server(chunk_steps=chunk, on_event=lambda kind, rid, t: log(f"serve: t={t:7.2f}s {kind} {rid}"))
+    srv = Server(chunk_steps=chunk, saturation=saturation,
+                 on_event=lambda kind, rid, t: log(f"serve: t={t:7.2f}s {kind} {rid}"))
     t0 = time.perf_counter()
     res = srv.run(reqs)
     wall = time.perf_counter() - t0
@@ -435,6 +437,8 @@ def main():
     ap.add_argument("--requests", type=int, default=16, help="serve workload: number of requests")
     ap.add_argument("--load", type=float, default=0.8, help="serve workload: offered load")
     ap.add_argument("--chunk", type=int, default=64, help="serve workload: steps between admission points")
+    ap.add_argument("--saturation", type=int, default=0,
+                    help="serve workload: particles a chunk steps together, oldest scenes first (0: all)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (3 times): the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -461,7 +465,8 @@ def main():
         args.warmup = 20 if args.warmup is None else args.warmup
 
     if args.workload == "serve":
-        r = serve_bench(args.requests, args.batch_seed, args.sim_time, args.load, args.chunk)
+        r = serve_bench(args.requests, args.batch_seed, args.sim_time, args.load, args.chunk,
+                        args.saturation or None)
         out = {
             "metric": "p99 latency",
             "value": r["p99_s"],

@@ -80,12 +80,17 @@ class Server:
 
     memory_budget: device bytes the resident scenes may hold (default 90 % of
     the free memory when the server starts).  max_scenes: scene slots.
-    chunk_steps: most steps between two admission points.  The clock of
-    run() is a callable giving the current time with a wait_until(t) method
-    (default WallClock); StepClock makes a run reproducible."""
+    chunk_steps: most steps between two admission points.  saturation: the
+    particles (fluid + solid) a chunk steps together: the resident scenes in
+    arrival order, each taken while it fits (the oldest always), so that a
+    chunk fills the device and the oldest requests are not slowed by newer
+    ones beyond that (None: every resident scene).  The clock of run() is a
+    callable giving the current time with a wait_until(t) method (default
+    WallClock); StepClock makes a run reproducible."""
 
     def __init__(self, memory_budget: int | None = None, max_scenes: int = 64, chunk_steps: int = 32,
-                 graphs: bool = True, autotune: bool = True, keep_state: bool = False, on_event=None):
+                 graphs: bool = True, autotune: bool = True, keep_state: bool = False, on_event=None,
+                 saturation: int | None = None):
         if memory_budget is None:
             free, _ = device_memory()
             memory_budget = int(0.9 * free)
@@ -95,6 +100,7 @@ class Server:
         self.graphs = graphs
         self.autotune = autotune  # time the fluid workgroup sizes whenever the batch is re-recorded
         self.keep_state = keep_state
+        self.saturation = saturation
         self.on_event = on_event  # called as on_event(kind, request id, time), kind "admit" or "finish"
         self.stats = ServerStats()
 
@@ -117,6 +123,20 @@ class Server:
             state = {"fluid": act.sim.fluid(), "solid": act.sim.solid() if act.counts[1] else None}
         act.sim.close()
         return Result(act.req.rid, act.req.arrival, act.admitted, now, act.req.steps, act.counts, obs, state)
+
+    def _chunk_scenes(self, active: list) -> list:
+        """The resident scenes one chunk steps: in arrival order, each while
+        the particles fit `saturation` (the oldest always)."""
+        if self.saturation is None:
+            return list(active)
+        chosen, total = [], 0
+        for x in active:
+            n = x.counts[0] + x.counts[1]
+            if chosen and total + n > self.saturation:
+                continue
+            chosen.append(x)
+            total += n
+        return chosen
 
     # ---- the loop -------------------------------------------------------------------
     def run(self, requests, clock=None) -> list:
@@ -160,8 +180,9 @@ class Server:
                     if pending:
                         clock.wait_until(pending[0].arrival)
                     continue
-                k = min(self.chunk_steps, min(x.remaining for x in active))
-                sims = [x.sim for x in active]
+                stepped = self._chunk_scenes(active)
+                k = min(self.chunk_steps, min(x.remaining for x in stepped))
+                sims = [x.sim for x in stepped]
                 t_chunk = clock()
                 batch.step(sims, k)
                 # arrivals during the chunk are prepared while the device works
@@ -171,14 +192,16 @@ class Server:
                     a, counts, params = self._inputs(req)
                     queue.append((req, a, counts, params, footprint(params)))
                 batch.synchronize()
-                st.chunk_log.append((t_chunk, clock(), len(active), k, sum(x.counts[0] + x.counts[1] for x in active)))
+                st.chunk_log.append((t_chunk, clock(), len(stepped), k,
+                                     sum(x.counts[0] + x.counts[1] for x in stepped)))
                 st.chunks += 1
                 st.steps += k
-                st.scene_steps += k * len(active)
+                st.scene_steps += k * len(stepped)
                 now = clock()
                 still = []
                 for x in active:
-                    x.remaining -= k
+                    if any(x is y for y in stepped):
+                        x.remaining -= k
                     if x.remaining == 0:
                         results.append(self._finish(x, now))
                         used -= x.resident
