@@ -323,13 +323,18 @@ __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col
             const float4 *nxt = pos + min(j + kScanBatch, se);
 #pragma unroll
             for (int k = 0; k < kScanBatch; ++k) qn[k] = nxt[k];
+            // the write position advances by one entry (T lanes) per hit: a
+            // select and an add per candidate, no address arithmetic
+            uint16_t *wp = L.list + cnt * T;
 #pragma unroll
             for (int k = 0; k < kScanBatch; ++k) {
                 const float d2 = len2(q[k].x - me.x, q[k].y - me.y, q[k].z - me.z);
                 const bool hit = (k < lim) & (d2 < P.R2);
-                L.list[cnt * T] = (uint16_t)(e0 + 16u * k);
-                cnt += hit ? 1 : 0;
+                *wp = (uint16_t)(e0 + 16u * k);
+                wp += hit ? T : 0;
             }
+            // (the low 32 bits of an LDS pointer are its LDS byte address)
+            cnt = (int)(((uint32_t)(uintptr_t)wp - (uint32_t)(uintptr_t)L.list) / (2u * T));
 #pragma unroll
             for (int k = 0; k < kScanBatch; ++k) q[k] = qn[k];
             // at most kScanBatch entries are added per round; the bases stay

@@ -209,7 +209,11 @@ struct Window {
 // so partners within reach lie within rz = sqrt(reach2 - gx^2 - gy^2) in z;
 // the window is the sub-cells of [z - rz, z + rz], widened by the z margin.
 // The caller still tests every slot of the window exactly, so what it finds
-// is the whole column's, in the same order.
+// is the whole column's, in the same order.  The square root is the
+// hardware one (within 1 ulp; the z margin is ~10^4 ulp of rz).  The code is
+// branch-free: a column outside the grid or out of reach reads key 0 twice
+// and yields the empty range, so that a caller's unrolled 9 windows stay
+// straight-line code.
 __device__ __forceinline__ void z_window(const DevParams &P, const Columns &cols, const int32_t *__restrict__ fine,
                                          int col, const Window &w, float reach2, int &b, int &e) {
     const int ox = col / 3 - 1, oy = col % 3 - 1;
@@ -217,14 +221,14 @@ __device__ __forceinline__ void z_window(const DevParams &P, const Columns &cols
     const float ey = oy < 0 ? w.fy : (oy > 0 ? 1.0f - w.fy : 0.0f);
     const float gx = fmaxf(ex - w.mx, 0.0f) * P.ival[0], gy = fmaxf(ey - w.my, 0.0f) * P.ival[1];
     const float r2 = reach2 - gx * gx - gy * gy;
-    b = e = 0;
-    if (!cols.inside(col) || !(r2 > 0.0f)) return;
-    const float rzs = sqrtf(r2) * P.inv[2] * (float)kSubZ + w.mz;
+    const bool ok = cols.inside(col) && r2 > 0.0f;
+    const float rzs = __builtin_amdgcn_sqrtf(fmaxf(r2, 0.0f)) * P.inv[2] * (float)kSubZ + w.mz;
     const int lo = max((int)floorf(w.tzs - rzs), cols.zlo * kSubZ);
     const int hi = max(min((int)floorf(w.tzs + rzs) + 1, cols.zhi1 * kSubZ), lo);
-    const int k0 = cols.base(col) * kSubZ;
-    b = fine[k0 + lo];
-    e = fine[k0 + hi];
+    const int k0 = ok ? cols.base(col) * kSubZ : 0;
+    const int bb = fine[k0 + (ok ? lo : 0)], ee = fine[k0 + (ok ? hi : 0)];
+    b = ok ? bb : 0;
+    e = ok ? ee : 0;
 }
 
 }  // namespace fsa
