@@ -359,8 +359,8 @@ __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col
 // loads of column ranges; then the whole-column ranges do, and only a wave
 // with walls in reach takes their z-windows.
 template <int KIND, int T, bool PACK>
-__device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cell, int n,
-                                     const int32_t *__restrict__ start,
+__device__ __forceinline__ void pass(const DevParams &P, const Self &me, const Columns &cols, const Window &w,
+                                     int n, const int32_t *__restrict__ start,
                                      const int32_t *__restrict__ fine, const float4 *__restrict__ pos,
                                      const float4 *__restrict__ vel, const float4 *__restrict__ np,
                                      const PackedSrc &ps, const Lists<T> &L, Acc &acc,
@@ -369,21 +369,18 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, int cel
     if (KIND == kFluidSolid && sorted_cells) {
         const int nyz = P.cn[1] * P.cn[2];
         const int lo = sorted_cells[0] / nyz, hi = sorted_cells[n - 1] / nyz;
-        const int cx = cell / nyz;
-        if (__ballot(cx + 1 >= lo && cx - 1 <= hi) == 0) return;
+        if (__ballot(cols.cx + 1 >= lo && cols.cx - 1 <= hi) == 0) return;
     }
-    if (KIND == kFluidVirt && reach && __ballot(reach[cell] != 0) == 0) return;
+    if (KIND == kFluidVirt && reach && __ballot(reach[me.cell] != 0) == 0) return;
     int cb[kColumns], ce[kColumns];
     if (KIND != kFluidFluid) {
-        column_ranges(P, cell, start, cb, ce);
+        column_ranges(cols, start, cb, ce);
         bool any = false;
 #pragma unroll
         for (int col = 0; col < kColumns; ++col) any |= cb[col] < ce[col];
         if (__ballot(any) == 0) return;
     }
     if (KIND != kFluidSolid) {
-        const Columns cols(P, cell);
-        const Window w(P, cols, me.x, me.y, me.z);
 #pragma unroll
         for (int col = 0; col < kColumns; ++col) z_window(P, cols, fine, col, w, P.R2w, cb[col], ce[col]);
     }
@@ -442,11 +439,14 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     }
 
     // partners kind by kind: fluid, solid, wall (the canonical order)
+    // (the cell's columns and the particle's place in its cell serve all three kinds)
     const PackedSrc fsrc{a.fpk, a.fcell}, none{nullptr, nullptr};
-    pass<kFluidFluid, T, PACK>(P, me, cell, P.nf, a.fstart, a.ffine, a.fpos, a.fvel, nullptr, fsrc, list, acc);
-    pass<kFluidSolid, T, false>(P, me, cell, P.ns, a.sstart, nullptr, a.spos, a.svel, a.snp, none, list, acc,
+    const Columns cols(P, cell);
+    const Window w(P, cols, me.x, me.y, me.z);
+    pass<kFluidFluid, T, PACK>(P, me, cols, w, P.nf, a.fstart, a.ffine, a.fpos, a.fvel, nullptr, fsrc, list, acc);
+    pass<kFluidSolid, T, false>(P, me, cols, w, P.ns, a.sstart, nullptr, a.spos, a.svel, a.snp, none, list, acc,
                                 a.scell);
-    pass<kFluidVirt, T, false>(P, me, cell, P.nv, a.vstart, a.vfine, a.vpos, a.vvel, nullptr, none, list, acc,
+    pass<kFluidVirt, T, false>(P, me, cols, w, P.nv, a.vstart, a.vfine, a.vpos, a.vvel, nullptr, none, list, acc,
                                nullptr, a.vreach);
 
     // fluidUpdate (fluid.cu:234-269): NaN scrub, explicit step with the old

@@ -132,34 +132,12 @@ __device__ __forceinline__ void for_each_column(const DevParams &P, int c, F &&b
     }
 }
 
-// The slot ranges [b[k], e[k]) of the 9 columns of cell c (the order of
-// for_each_column; empty outside the grid) in the starts of one kind.  All
-// 18 loads are issued together, where a column-by-column walk waits for
-// each column's pair in turn.
-__device__ __forceinline__ void column_ranges(const DevParams &P, int c, const int32_t *__restrict__ start,
-                                              int (&b)[9], int (&e)[9]) {
-    const int ny = P.cn[1], nz = P.cn[2];
-    const int nyz = ny * nz;
-    const int cx = c / nyz;
-    const int rem = c - cx * nyz;
-    const int cy = rem / nz;
-    const int cz = rem - cy * nz;
-    const int zlo = max(cz - 1, 0), zhi1 = min(cz + 1, nz - 1) + 1;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        const int gx = cx + k / 3 - 1, gy = cy + k % 3 - 1;
-        const bool in = (unsigned)gx < (unsigned)P.cn[0] && (unsigned)gy < (unsigned)ny;
-        const int base = (gx * ny + gy) * nz;
-        b[k] = start[in ? base + zlo : 0];
-        e[k] = start[in ? base + zhi1 : 0];
-    }
-}
-
 // The 9 columns of cell c (the order of for_each_column) as cell ranges
 // [c0, c1) of one z-run each; columns outside the grid are the empty range
-// [0, 0).
+// [0, 0).  Column (ox, oy) starts at the particle's own column start
+// (c - cz) plus ox ny nz + oy nz, wave-uniform offsets: one add per column.
 struct Columns {
-    int cx, cy, zlo, zhi1, ny, nz, nx;
+    int cx, cy, zlo, zhi1, ny, nz, nx, cbase;
     __device__ __forceinline__ Columns(const DevParams &P, int c) {
         ny = P.cn[1];
         nz = P.cn[2];
@@ -169,6 +147,7 @@ struct Columns {
         const int rem = c - cx * nyz;
         cy = rem / nz;
         const int cz = rem - cy * nz;
+        cbase = c - cz;
         zlo = max(cz - 1, 0);
         zhi1 = min(cz + 1, nz - 1) + 1;
     }
@@ -176,10 +155,29 @@ struct Columns {
         const int gx = cx + col / 3 - 1, gy = cy + col % 3 - 1;
         return (unsigned)gx < (unsigned)nx && (unsigned)gy < (unsigned)ny;
     }
-    __device__ __forceinline__ int base(int col) const { return ((cx + col / 3 - 1) * ny + (cy + col % 3 - 1)) * nz; }
+    __device__ __forceinline__ int base(int col) const { return cbase + (col / 3 - 1) * (ny * nz) + (col % 3 - 1) * nz; }
     __device__ __forceinline__ int c0(int col) const { return inside(col) ? base(col) + zlo : 0; }
     __device__ __forceinline__ int c1(int col) const { return inside(col) ? base(col) + zhi1 : 0; }
 };
+
+// The slot ranges [b[k], e[k]) of the 9 columns (the order of
+// for_each_column; empty outside the grid) in the starts of one kind.  All
+// 18 loads are issued together, where a column-by-column walk waits for
+// each column's pair in turn.
+__device__ __forceinline__ void column_ranges(const Columns &cols, const int32_t *__restrict__ start, int (&b)[9],
+                                              int (&e)[9]) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const bool in = cols.inside(k);
+        const uint32_t base = (uint32_t)cols.base(k);
+        b[k] = start[in ? base + (uint32_t)cols.zlo : 0u];
+        e[k] = start[in ? base + (uint32_t)cols.zhi1 : 0u];
+    }
+}
+__device__ __forceinline__ void column_ranges(const DevParams &P, int c, const int32_t *__restrict__ start,
+                                              int (&b)[9], int (&e)[9]) {
+    column_ranges(Columns(P, c), start, b, e);
+}
 
 // Where a particle sits, for its z-windows: fx, fy its position in its
 // cell in cells, tzs its z in sub-cells (1/kSubZ cell), and the margins that
@@ -225,8 +223,8 @@ __device__ __forceinline__ void z_window(const DevParams &P, const Columns &cols
     const float rzs = __builtin_amdgcn_sqrtf(fmaxf(r2, 0.0f)) * P.inv[2] * (float)kSubZ + w.mz;
     const int lo = max((int)floorf(w.tzs - rzs), cols.zlo * kSubZ);
     const int hi = max(min((int)floorf(w.tzs + rzs) + 1, cols.zhi1 * kSubZ), lo);
-    const int k0 = ok ? cols.base(col) * kSubZ : 0;
-    const int bb = fine[k0 + (ok ? lo : 0)], ee = fine[k0 + (ok ? hi : 0)];
+    const uint32_t k0 = ok ? (uint32_t)(cols.base(col) * kSubZ) : 0u;
+    const int bb = fine[k0 + (ok ? (uint32_t)lo : 0u)], ee = fine[k0 + (ok ? (uint32_t)hi : 0u)];
     b = ok ? bb : 0;
     e = ok ? ee : 0;
 }
