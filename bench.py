@@ -457,7 +457,7 @@ def main():
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1 "
-                                   "(a scene is one GPU's share; see DESIGN.md for the sharding plan)"}))
+                                   "(a scene is one GPU's share; batched and served scenes spread over GPUs as whole scenes)"}))
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
     if args.workload == "step":

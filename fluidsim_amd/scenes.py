@@ -260,7 +260,8 @@ def _write_rows(path, rows: np.ndarray, ints) -> None:
             f.write(vals + (f" {int(ints[r])}" if ints is not None else "") + "\n")
 
 
-# BASELINE scenes (y-up mapping of the z-up specs)
+# The benchmark scenes: dam break onto an elastic gate at CPU, showcase and
+# precision scale (y-up mapping of the z-up specifications)
 def c1_cpu_scene(seed: int = 0) -> DamBreak:
     """43,750 fluid (50x35x25, dp 4 mm) + 2,000 solid (4x20x25) + 3 wall layers, S = 4."""
     return DamBreak(dp=0.004, fluid=(50, 35, 25), tank=(100, 70, 25), gate=(4, 20, 25), gate_x0=75,
