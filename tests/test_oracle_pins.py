@@ -546,3 +546,182 @@ def test_fp32_oracle_tracks_fp64_over_ten_steps(tiny):
     assert np.abs(f32["density"] - f64["density"]).max() <= 1e-5 * sc.fluid_density
     s32, s64 = o32.solid(), o64.solid()
     assert np.abs(s32["position"] - s64["position"]).max() <= 1e-6 * L
+
+
+# ---- TLSPH artificial viscosity (solid.cu:139-156) and sub-step integrator (solid.cu:209-260)
+
+@pytest.fixture(scope="module")
+def falling_box():
+    """The open box with gravity and 3 solid sub-steps per step."""
+    return scenes.DamBreak(dp=0.01, fluid=(1, 1, 1), tank=(20, 20, 20), gate=None, wall_layers=0, gravity=9.81,
+                           sound_speed=20.0, solid_sub_step=3)
+
+
+def _viscous_acc(box, X, x, u, **kw):
+    """du_dt of a solid block (FP64 oracle) at positions x, velocities u; scene keys in kw."""
+    params, a = custom_case(box.with_(**kw) if kw else box,
+                            solid={"position": x, "reference_position": X, "velocity": u})
+    o = orc.Oracle(params, 64)
+    o.upload(a)
+    run_stages(o, PRE + ("solid_deformation", "solid_force"))
+    return params, a, o.solid()
+
+
+def test_solid_substeps_integrate_rigid_translation(falling_box):
+    """A block translating rigidly feels no stress (F = I) and no artificial
+    viscosity (du = 0), so each of the S sub-steps of a step advances it by
+    dt_s = dt / S with the explicit order x += u dt_s; u += (a + g) dt_s
+    (solid.cu:220-228, 241-260): after one step u = u0 + g dt and
+    x = x0 + u0 dt + g dt_s^2 S (S - 1) / 2.  FIXED particles get u = 0 and
+    keep x (solid.cu:215-219), whatever velocity they were given."""
+    free = _block(6) + [0.0, 0.02, 0.0]
+    fixed = _block(3) + [0.06, -0.07, 0.05]  # farther than the cut-off from the free block
+    X = np.concatenate([free, fixed]).astype(np.float32)
+    nfree = len(free)
+    u = np.zeros_like(X)
+    u[:nfree] = [0.3, -0.2, 0.1]
+    u[nfree:] = [0.5, 0.5, -0.5]
+    t = np.zeros(len(X), np.int32)
+    t[nfree:] = 1  # FSA_FIXED_SOLID
+    params, a = custom_case(falling_box, solid={"position": X, "reference_position": X, "velocity": u, "type": t})
+    S = int(params.solid_sub_step)
+    assert S == 3
+    o = orc.Oracle(params, 64)
+    o.upload(a)
+    o.step(1)
+    s = o.solid()
+    dt = float(params.dt)
+    dts = float(np.float32(params.dt) / np.float32(S))  # dt / (float)S, solid.cu:213
+    g = np.array(params.gravity, np.float64)
+    x0 = X.astype(np.float64)
+    u0 = u.astype(np.float64)
+    x_exp = x0[:nfree] + u0[:nfree] * dts * S + g * dts * dts * S * (S - 1) / 2
+    u_exp = u0[:nfree] + g * dts * S
+    np.testing.assert_allclose(s["position"][:nfree], x_exp, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(s["velocity"][:nfree], u_exp, rtol=0, atol=1e-12)
+    # the sub-steps are what the step took: one step of dt would land elsewhere
+    assert np.abs(s["position"][:nfree] - (x0[:nfree] + u0[:nfree] * dt)).max() > 1e-8
+    assert abs(dts * S - dt) < 1e-9 * dt
+    np.testing.assert_array_equal(s["position"][nfree:], x0[nfree:])
+    np.testing.assert_array_equal(s["velocity"][nfree:], 0.0)
+    np.testing.assert_array_equal(s["du_dt"][nfree:], 0.0)
+    assert np.abs(s["du_dt"][:nfree]).max() < 1e-8 * 9.81  # rigid: no stress, no viscosity (FP64 rounding only)
+
+
+def test_solid_update_clamps_positions_not_velocities(falling_box):
+    """solidUpdate clamps x into the domain and leaves u (solid.cu:229-236)."""
+    hi = falling_box.tank[0] * falling_box.dp
+    g3 = (np.arange(3) - 1) * 0.01
+    blk = np.stack(np.meshgrid(g3, g3, g3, indexing="ij"), -1).reshape(-1, 3)
+    X = np.concatenate([blk + [hi - 0.012, 0.1, 0.1], blk + [0.012, 0.1, 0.1]]).astype(np.float32)
+    u = np.zeros_like(X)
+    u[:27] = [60.0, 0.0, 0.0]   # the front layer crosses xmax within one sub-step
+    u[27:] = [-60.0, 0.0, 0.0]  # and here xmin
+    params, a = custom_case(falling_box, solid={"position": X, "reference_position": X, "velocity": u})
+    o = orc.Oracle(params, 64)
+    o.upload(a)
+    run_stages(o, PRE + ("solid_deformation", "solid_force", "solid_update"))
+    s = o.solid()
+    dts = float(np.float32(params.dt) / np.float32(params.solid_sub_step))
+    g = np.array(params.gravity, np.float64)
+    x_free = X.astype(np.float64) + u.astype(np.float64) * dts
+    lo, hi_ = np.array(params.domain_min, np.float64), np.array(params.domain_max, np.float64)
+    x_exp = np.minimum(np.maximum(x_free, lo), hi_)
+    assert (x_exp != x_free).sum() >= 9 + 9  # a layer of each block is clamped
+    np.testing.assert_allclose(s["position"], x_exp, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(s["velocity"], u.astype(np.float64) + g * dts, rtol=0, atol=1e-9)
+
+
+def test_artificial_viscosity_vanishes_under_rigid_motion(open_box):
+    """The viscosity acts on nu = h (du.dx)/(|dx|^2 + 0.01 h^2) (solid.cu:139-156):
+    zero for a uniform velocity and for a rigid rotation u = w x (x - c),
+    whose du is normal to dx, while a compressive field of the same speed
+    gives a force.  Using dX for dx or |du| for du.dx fails this."""
+    X = _block(8)
+    c = X.mean(axis=0)
+    w = np.array([3.0, -2.0, 5.0])
+    rot = np.cross(w, X - c).astype(np.float32)
+    comp = (-np.linalg.norm(w) * (X - c)).astype(np.float32)
+    _, _, s_c = _viscous_acc(open_box, X, X, comp)
+    scale = np.abs(s_c["du_dt"]).max()
+    assert scale > 0
+    for u in (np.tile(np.float32([0.4, -0.3, 0.2]), (len(X), 1)), rot):
+        _, _, s = _viscous_acc(open_box, X, X, u)
+        assert np.abs(s["du_dt"]).max() < 1e-6 * scale
+
+
+def test_artificial_viscosity_dissipates_and_scales_with_alpha_and_beta(open_box):
+    """With F = I the force is the artificial viscosity alone,
+    a_i = sum_j (alpha nu c_s - beta nu^2) det(F) F^-T L_i dX k.  It removes
+    kinetic energy (sum u.a < 0) from a compressive and (at beta = 0) an
+    expanding flow; at beta = 0 it is linear in the velocity and grows with
+    c_s = sqrt(E / (3 (1 - 2 nu)) / rho_s) (4 E: twice the force); at
+    alpha = 0 it is quadratic in the velocity.  A flipped sign, alpha and
+    beta swapped, or another speed of sound fail one of these."""
+    X = _block(8)
+    c = X.mean(axis=0)
+    d = X - c
+    flows = {"compress": -2.0 * d, "expand": 2.0 * d,
+             "swirl": np.stack([-0.5 * d[:, 0] ** 2 * 40, d[:, 1] * d[:, 2] * 30, -d[:, 2]], 1)}
+    acc = {}
+    for name, u in flows.items():
+        for k, kw in (("a", dict(arti_vis_beta=0.0)), ("b", dict(arti_vis_alpha=0.0)),
+                      ("ab", {}), ("a4E", dict(arti_vis_beta=0.0, youngs_modulus=4 * open_box.youngs_modulus))):
+            for m in (1, 2):
+                _, _, s = _viscous_acc(open_box, X, X, (m * u).astype(np.float32), **kw)
+                acc[name, k, m] = s["du_dt"]
+    for name, u in flows.items():
+        a1 = acc[name, "a", 1]
+        scale = np.abs(a1).max()
+        floor = 2e-8  # the stress of F = I to FP64 rounding: ~1e-8 m/s^2
+        assert scale > 0
+        np.testing.assert_allclose(acc[name, "a", 2], 2 * a1, rtol=0, atol=1e-7 * scale + floor)
+        np.testing.assert_allclose(acc[name, "b", 2], 4 * acc[name, "b", 1], rtol=0,
+                                   atol=1e-7 * np.abs(acc[name, "b", 1]).max() + floor)
+        np.testing.assert_allclose(acc[name, "a4E", 1], 2 * a1, rtol=0, atol=1e-6 * scale)
+        # alpha and beta terms add
+        np.testing.assert_allclose(acc[name, "ab", 1], a1 + acc[name, "b", 1], rtol=0, atol=1e-7 * scale + floor)
+        if name != "swirl":
+            power = (u.astype(np.float32).astype(np.float64) * a1).sum()
+            assert power < 0, (name, power)
+    # approaching pairs (nu < 0) make both terms negative: dissipative at any alpha, beta
+    u = flows["compress"].astype(np.float32).astype(np.float64)
+    assert (u * acc["compress", "b", 1]).sum() < 0
+    assert (u * acc["compress", "ab", 1]).sum() < 0
+
+
+def test_solid_forces_are_objective(open_box):
+    """Rotating a deformed, moving block by Q (x -> Q x, u -> Q u) rotates every
+    force: stress P = F S(E) (E is rotation-free) and the viscous term
+    det(F) F^-T visc L_i dX k both turn with F = Q F0.  F^-1 in place of
+    F^-T, or L_i on the wrong side, breaks this for a non-symmetric F0."""
+    X = _block(8)
+    c = X.mean(axis=0)
+    F0 = np.array([[1.01, 0.03, 0.0], [0.0, 0.995, 0.02], [0.01, 0.0, 1.0]])
+    d = X - c
+    u0 = np.stack([-3 * d[:, 0] + 20 * d[:, 1] ** 2, 2 * d[:, 2], -d[:, 0] * d[:, 1] * 50], 1)
+    t = 0.7
+    Q = np.array([[math.cos(t), 0, math.sin(t)], [0, 1, 0], [-math.sin(t), 0, math.cos(t)]]) @ \
+        np.array([[1, 0, 0], [0, math.cos(0.4), -math.sin(0.4)], [0, math.sin(0.4), math.cos(0.4)]])
+    accs = []
+    for R in (np.eye(3), Q):
+        x = (d @ F0.T) @ R.T + c
+        u = u0 @ R.T
+        _, a, s = _viscous_acc(open_box, X, x, u)
+        accs.append((a, s))
+    (a0, s0), (a1, s1) = accs
+    scale = np.abs(s0["du_dt"]).max()
+    assert scale > 0
+    # float32 uploads of the rotated state perturb it by ~1e-8 relative
+    np.testing.assert_allclose(s1["du_dt"], s0["du_dt"] @ Q.T, rtol=0, atol=2e-4 * scale)
+    # and the same with the viscosity alone: F = Q (rigidly rotated block; the
+    # float32 positions leave a strain of ~1e-7, whose stress a soft solid,
+    # E / 1e4, makes negligible against the viscosity, which scales as sqrt(E))
+    accs = []
+    for R in (np.eye(3), Q):
+        _, _, s = _viscous_acc(open_box, X, d @ R.T + c, u0 @ R.T, youngs_modulus=open_box.youngs_modulus / 1e4)
+        accs.append(s["du_dt"])
+    scale = np.abs(accs[0]).max()
+    np.testing.assert_allclose(accs[1], accs[0] @ Q.T, rtol=0, atol=1e-3 * scale)
+    # ... which F^-1 would not give: Q^T != Q here
+    assert np.abs(accs[0] @ Q - accs[0] @ Q.T).max() > 0.1 * scale
