@@ -28,6 +28,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 from fluidsim_amd import scenes  # noqa: E402
+from fluidsim_amd.metrics import field_errors, tip_displacement  # noqa: E402
 from fluidsim_amd.sim import DeviceSim  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
@@ -313,34 +314,6 @@ def serve_bench(n_requests: int, seed: int, sim_time: float, load: float, chunk:
         "chunks": srv.stats.chunks, "chunk_steps": chunk, "plans_built": srv.stats.plans_built,
         "same_alone_and_in_mix": "tests/test_gpu_serving.py (bit-identical)",
     }
-
-
-def tip_displacement(solid_pos, ref_pos, types, dp) -> np.ndarray:
-    """Mean x - X over the gate tip row (unconstrained solids whose reference y is
-    within dp/2 of the largest one), in float64."""
-    free = types == 0
-    if not free.any():
-        return np.zeros(3)
-    X = np.asarray(ref_pos, np.float64)
-    row = free & (X[:, 1] >= X[free, 1].max() - 0.5 * dp)
-    return (np.asarray(solid_pos, np.float64)[row] - X[row]).mean(axis=0)
-
-
-def field_errors(sc, got: dict, ref: dict) -> dict:
-    """Max and RMS errors of a fluid state against the FP64 oracle's, each divided
-    by its scale: L (largest tank extent), U = max(max |u|, sqrt(2 g H)), rho0,
-    and rho0 g H (hydrostatic pressure at the column foot)."""
-    H = sc.fluid[1] * sc.dp
-    L = max(sc.tank) * sc.dp
-    uref = np.asarray(ref["velocity"], np.float64)
-    U = max(float(np.sqrt((uref ** 2).sum(axis=1)).max()), math.sqrt(2.0 * sc.gravity * H))
-    scales = {"position": L, "velocity": U, "density": sc.fluid_density,
-              "pressure": sc.fluid_density * sc.gravity * H}
-    out = {}
-    for k, s in scales.items():
-        d = np.abs(np.asarray(got[k], np.float64) - np.asarray(ref[k], np.float64))
-        out[k] = {"max": float(d.max()) / s, "rms": float(np.sqrt((d ** 2).mean())) / s}
-    return out
 
 
 def precision_bench(steps: int, seed: int, threads: int) -> dict:

@@ -26,6 +26,8 @@ FLUID, SOLID, VIRT = 0, 1, 2
 UNCONSTRAINED_SOLID, FIXED_SOLID = 0, 1
 DEBUG_PAIR_SUMS = 1
 DEBUG_WALK = 2
+HEALTH_FLUID_NONFINITE = 1
+HEALTH_SOLID_NONFINITE = 2
 FORM_PACKED = 1
 FORM_COMPENSATED = 2
 
@@ -164,6 +166,9 @@ DEVICE_API = {
     "fsa_batch_set_streams": (C.c_int, [_vp, C.c_int]),
     "fsa_batch_synchronize": (C.c_int, [_vp]),
     "fsa_batch_stats": (C.c_int, [_vp, _P(C.c_int64), _P(C.c_int64), _P(C.c_int)]),
+    "fsa_batch_tuning": (C.c_int, [_vp, _P(C.c_int64), _P(C.c_double)]),
+    "fsa_get_health": (C.c_int, [_vp, _P(C.c_uint)]),
+    "fsa_download_pair_counts": (C.c_int, [_vp, _P(C.c_int32)]),
     "fsa_set_fluid_threads": (C.c_int, [_vp, C.c_int]),
     "fsa_get_fluid_threads": (C.c_int, [_vp, _P(C.c_int)]),
     "fsa_autotune_fluid": (C.c_int, [_vp, C.c_int, _P(C.c_int)]),

@@ -39,6 +39,8 @@
 // next step overwrites, so the scenes do not advance -- and keeps the
 // fastest; otherwise it takes the first listed scene's size.
 #include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -118,7 +120,19 @@ struct Chain {
     int fluid_threads = 0;  // workgroup size of the recorded fluid launches
     bool packed = false;    // the scenes use the packed precision form (all or none)
     int64_t plans_built = 0;
+    // the autotuner's last choice and the chain's fluid total when it was
+    // made: a plan rebuild re-times only when the total has moved by more
+    // than kRetuneFraction (continuous batching rebuilds on every admit and
+    // finish); its device tables persist between timings
+    int tuned_threads = 0;
+    int64_t tuned_fluid = 0;
+    char *tune_blob = nullptr;
+    size_t tune_cap = 0;
+    int64_t tunes = 0;
+    double tune_ms = 0.0;  // host wall time spent timing
 };
+
+constexpr double kRetuneFraction = 0.25;
 
 }  // namespace
 
@@ -293,18 +307,25 @@ int tune_fluid(Chain *b, const std::vector<fsa_sim *> &sims) {
         poff[c] = put(blob, prefix.data(), prefix.size());
     }
     if (nblk[0] == 0) return kFluidThreadsDefault;  // no fluid anywhere
-    char *dev = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     auto fail = [&](hipError_t e) {
         if (t0) (void)hipEventDestroy(t0);
         if (t1) (void)hipEventDestroy(t1);
-        if (dev) (void)hipFree(dev);
         set_err(e == hipErrorOutOfMemory ? FSA_ERR_NOMEM : FSA_ERR_HIP,
                 std::string("batch fluid autotune: ") + hipGetErrorString(e));
         return 0;
     };
     hipError_t e;
-    if ((e = hipMalloc(&dev, blob.size())) != hipSuccess) return fail(e);
+    if (blob.size() > b->tune_cap) {
+        if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return fail(e);  // the old tables may be read
+        if (b->tune_blob) (void)hipFree(b->tune_blob);
+        b->tune_blob = nullptr;
+        b->tune_cap = 0;
+        const size_t cap = std::max(blob.size(), 2 * b->tune_cap);
+        if ((e = hipMalloc(&b->tune_blob, cap)) != hipSuccess) return fail(e);
+        b->tune_cap = cap;
+    }
+    char *dev = b->tune_blob;
     if ((e = hipMemcpyAsync(dev, blob.data(), blob.size(), hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
         (e = hipEventCreate(&t0)) != hipSuccess || (e = hipEventCreate(&t1)) != hipSuccess)
         return fail(e);
@@ -332,7 +353,6 @@ int tune_fluid(Chain *b, const std::vector<fsa_sim *> &sims) {
     const int best_t = kChoices[best_c];
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    (void)hipFree(dev);
     return best_t;
 }
 
@@ -346,9 +366,18 @@ int build_plan(Chain *b, const std::vector<fsa_sim *> &sims) {
         if (((s->P.form & FSA_FORM_PACKED) != 0) != b->packed)
             return set_err(FSA_ERR_INVALID, "the scenes of one batch must all use the packed form or none");
     if (b->flags & FSA_BATCH_AUTOTUNE) {
-        const int t = tune_fluid(b, sims);
-        if (t == 0) return FSA_ERR_HIP;
-        b->fluid_threads = t;
+        int64_t total = 0;
+        for (const fsa_sim *s : sims) total += s->P.nf;
+        if (b->tuned_threads == 0 || std::abs((double)(total - b->tuned_fluid)) > kRetuneFraction * b->tuned_fluid) {
+            const auto w0 = std::chrono::steady_clock::now();
+            const int t = tune_fluid(b, sims);
+            if (t == 0) return FSA_ERR_HIP;
+            b->tune_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+            b->tunes += 1;
+            b->tuned_threads = t;
+            b->tuned_fluid = total;
+        }
+        b->fluid_threads = b->tuned_threads;
     } else {
         b->fluid_threads = sims[0]->fluid_threads;
     }
@@ -428,6 +457,7 @@ void free_chain(Chain *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graphs(c);
     if (c->dev_blob) (void)hipFree(c->dev_blob);
+    if (c->tune_blob) (void)hipFree(c->tune_blob);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -621,3 +651,16 @@ int fsa_batch_stats(fsa_batch *b, int64_t *plans_built, int64_t *steps_run, int 
 }
 
 }  // extern "C"
+
+int fsa_batch_tuning(fsa_batch *b, int64_t *tunes, double *tune_ms) {
+    if (!b) return set_err(FSA_ERR_INVALID, "null batch");
+    int64_t n = 0;
+    double ms = 0.0;
+    for (const Chain *c : b->chains) {
+        n += c->tunes;
+        ms += c->tune_ms;
+    }
+    if (tunes) *tunes = n;
+    if (tune_ms) *tune_ms = ms;
+    return FSA_OK;
+}

@@ -27,6 +27,26 @@ int fsa::set_err(int code, const std::string &msg) {
 #define HIP_TRY(expr) FSA_HIP_TRY(expr)
 
 namespace fsa {
+// The coupling pass records the fluid candidates of a solid particle within
+// R_rec of its step-start position x0, from the fluid positions before the
+// fluid update; the force passes replay them against the updated positions
+// while the solid is within d = 0.04 R of x0.  A fluid partner within R of the
+// solid then lay within R + d + f of x0 before the update, where f bounds the
+// update's move |u| dt: |u| is at most the speed cap 0.3 c0 after any update
+// (src/fluid.cu:265-268) and at most the largest uploaded speed before the
+// first one.  The clamp to the domain only shortens the move; the
+// compensated form adds at most an ulp of the coordinate, which the absolute
+// margin (8 ulp of the largest cell-grid coordinate) covers.
+void set_record_reach(DevParams &d, const fsa_params &p, double vmax) {
+    const double R = std::sqrt((double)p.impact_distance_square), disp = 0.04 * R;
+    double xmax = 0.0;
+    for (int a = 0; a < 3; ++a)
+        xmax = std::max({xmax, std::fabs((double)p.cell_min[a]),
+                         std::fabs((double)p.cell_min[a] + (double)p.interval[a] * p.cell_number[a])});
+    const double f = vmax * (double)p.dt * (1.0 + 1.0e-4) + 8.0 * xmax * 0x1p-23;
+    d.R2_rec = (float)((R + disp + f) * (R + disp + f) * (1.0 + 1.0e-5));
+}
+
 DevParams make_dev_params(const fsa_params &p) {
     DevParams d{};
     d.nf = p.n_fluid;
@@ -80,9 +100,9 @@ DevParams make_dev_params(const fsa_params &p) {
     {
         const double R = std::sqrt((double)p.impact_distance_square), disp = 0.04 * R;
         d.disp2_max = (float)(disp * disp);
-        d.R2_rec = (float)((R + disp) * (R + disp) * (1.0 + 1.0e-5));
         d.R2w = (float)(R * R * (1.0 + 1.0e-5));
     }
+    set_record_reach(d, p, d.max_speed);
     d.form = p.precision_form;
     for (int a = 0; a < 3; ++a) d.ival[a] = p.interval[a];
     d.B_inv = p.fluid_coefficient_rho2p != 0.0f ? 1.0f / p.fluid_coefficient_rho2p : 0.0f;
@@ -203,7 +223,8 @@ int alloc_all(fsa_sim *s) {
     if (!s->dry && ns > 0) HIP_TRY(hipMemsetAsync(s->sx.np, 0, sizeof(float4) * ((size_t)ns + kPosPad), s->stream));
     if ((rc = s->alloc(&s->vx.nrm, s->P.nv)) || (rc = s->alloc(&s->virt_x, s->P.nv))) return rc;
     if (s->P.nv > 0 && (rc = s->alloc(&s->vx.reach, (size_t)s->P.ncell))) return rc;
-    if ((rc = s->alloc(&s->jumps, 1))) return rc;
+    if ((rc = s->alloc(&s->jumps, 2))) return rc;
+    s->health = s->jumps ? s->jumps + 1 : nullptr;
     if ((rc = s->alloc(&s->obs_partial, (size_t)observe_partials())) || (rc = s->alloc(&s->obs_out, kObserveSums)))
         return rc;
     return FSA_OK;
@@ -361,6 +382,7 @@ SolidArgs fsa::solid_args(const fsa_sim *s, const KindBuffers &f, const KindBuff
     a.fpos = f.pos[f.st];
     a.fvel = f.vel[f.st];
     a.x = s->sx;
+    a.health = s->health;
     return a;
 }
 
@@ -405,6 +427,7 @@ FluidStepArgs fsa::fluid_args(const fsa_sim *s, const KindBuffers &f, const Kind
     a.rates = nullptr;
     a.scell = so.n ? so.cell[so.ord] : nullptr;
     a.vreach = s->P.nv ? s->vx.reach : nullptr;
+    a.health = s->health;
     return a;
 }
 
@@ -429,6 +452,14 @@ int fsa_device_count(int *count) {
     return FSA_OK;
 }
 
+namespace {
+// unique per context state over the process lifetime (see fsa_sim::serial)
+uint64_t next_serial() {
+    static std::atomic<uint64_t> counter{1};
+    return counter.fetch_add(1);
+}
+}  // namespace
+
 int fsa_create(const fsa_params *params, void *stream, fsa_sim **out) {
     if (!params || !out) return set_err(FSA_ERR_INVALID, "null argument");
     *out = nullptr;
@@ -441,8 +472,7 @@ int fsa_create(const fsa_params *params, void *stream, fsa_sim **out) {
     if (!s) return set_err(FSA_ERR_NOMEM, "host allocation failed");
     s->params = *params;
     s->P = make_dev_params(*params);
-    static std::atomic<uint64_t> next_serial{1};
-    s->serial = next_serial.fetch_add(1);
+    s->serial = next_serial();
     {
         hipDeviceProp_t prop;
         if (hipGetDevice(&s->device) != hipSuccess || hipGetDeviceProperties(&prop, s->device) != hipSuccess ||
@@ -466,7 +496,7 @@ int fsa_create(const fsa_params *params, void *stream, fsa_sim **out) {
     };
     if ((rc = ensure_side(s))) return fail(rc);
     if ((rc = alloc_all(s))) return fail(rc);
-    if (hipMemsetAsync(s->jumps, 0, sizeof(unsigned long long), s->stream) != hipSuccess)
+    if (hipMemsetAsync(s->jumps, 0, 2 * sizeof(unsigned long long), s->stream) != hipSuccess)
         return fail(set_err(FSA_ERR_HIP, "hipMemsetAsync failed"));
     if ((rc = zero_starts(s, s->fluid)) || (rc = zero_starts(s, s->solid)) || (rc = zero_starts(s, s->virt)))
         return fail(rc);
@@ -527,12 +557,25 @@ int fsa_upload_fluid(fsa_sim *s, const float *x, const float *u, const float *rh
     if (rc) return rc;
     std::vector<float4> pos(n), vel(n);
     std::vector<int32_t> id(n);
+    double vmax = s->P.max_speed;
     for (int i = 0; i < n; ++i) {
         pos[i] = make_float4(x[3 * i], x[3 * i + 1], x[3 * i + 2], rho[i]);
         vel[i] = make_float4(u[3 * i], u[3 * i + 1], u[3 * i + 2], p[i]);
         id[i] = i;
+        const double v2 = (double)u[3 * i] * u[3 * i] + (double)u[3 * i + 1] * u[3 * i + 1] +
+                          (double)u[3 * i + 2] * u[3 * i + 2];
+        if (std::isfinite(v2) && v2 > vmax * vmax) vmax = std::sqrt(v2);
     }
+    // the first fluid update moves a particle by its uploaded speed, which may
+    // exceed the cap: the coupling records reach that far (until the next
+    // upload).  A non-finite velocity is a diverged state (fsa_get_health),
+    // not a bounded move.
+    set_record_reach(s->P, s->params, vmax);
+    // a batch keys its recorded launches on the serial: a re-uploaded scene is a new one
+    s->serial = next_serial();
     KindBuffers &k = s->fluid;
+    s->couple_fresh = false;  // the coupling records name slots of the old order
+    HIP_TRY(hipMemsetAsync(s->health, 0, sizeof(unsigned long long), s->stream));  // a new state
     if (n > 0) {
         k.st = 0;
         k.ord = 0;
@@ -566,6 +609,8 @@ int fsa_upload_solid(fsa_sim *s, const float *x, const float *u, const float *X,
         id[i] = i;
     }
     KindBuffers &k = s->solid;
+    s->couple_fresh = false;  // the coupling records name slots of the old order
+    HIP_TRY(hipMemsetAsync(s->health, 0, sizeof(unsigned long long), s->stream));  // a new state
     if (n > 0) {
         k.st = 0;
         k.ord = 0;
@@ -682,6 +727,28 @@ int fsa_download_fluid_rates(fsa_sim *s, float *drho, float *a) {
         const int i = ids[t];
         if (drho) drho[i] = r[t].w;
         if (a) { a[3 * i] = r[t].x; a[3 * i + 1] = r[t].y; a[3 * i + 2] = r[t].z; }
+    }
+    return FSA_OK;
+}
+
+int fsa_download_pair_counts(fsa_sim *s, int32_t *counts) {
+    CHECK_SIM(s);
+    if (!s->pair_counts) return set_err(FSA_ERR_STATE, "pair counts are recorded only with FSA_DEBUG_PAIR_SUMS");
+    if (!counts) return set_err(FSA_ERR_INVALID, "null output");
+    const KindBuffers &k = s->fluid;
+    std::vector<int32_t> ids;
+    std::vector<float4> r;
+    int rc;
+    if ((rc = download_sorted(s, reinterpret_cast<const float4 *>(s->pair_counts), k.n, r)) ||
+        (rc = sorted_ids(s, k, ids)))
+        return rc;
+    for (int t = 0; t < k.n; ++t) {
+        const int i = ids[t];
+        int4 c;
+        std::memcpy(&c, &r[t], sizeof c);
+        counts[3 * i] = c.x;
+        counts[3 * i + 1] = c.y;
+        counts[3 * i + 2] = c.z;
     }
     return FSA_OK;
 }
@@ -820,6 +887,7 @@ int fsa_fluid_calculation(fsa_sim *s) {
     KindBuffers &f = s->fluid;
     FluidStepArgs a = fluid_args(s);
     a.rates = (s->debug & FSA_DEBUG_PAIR_SUMS) ? s->rates : nullptr;
+    a.counts = (s->debug & FSA_DEBUG_PAIR_SUMS) ? s->pair_counts : nullptr;
     HIP_TRY(launch_fluid(s->P, a, s->fluid_threads, s->stream));
     f.st = 1 - f.st;
     stage_end(s, FSA_STAGE_FLUID);
@@ -977,11 +1045,21 @@ int fsa_get_jump_count(fsa_sim *s, int64_t *count) {
     return FSA_OK;
 }
 
+int fsa_get_health(fsa_sim *s, unsigned *flags) {
+    CHECK_SIM(s);
+    if (!flags) return set_err(FSA_ERR_INVALID, "null output");
+    unsigned long long v = 0;
+    HIP_TRY(hipMemcpyAsync(&v, s->health, sizeof v, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    *flags = (unsigned)v;
+    return FSA_OK;
+}
+
 int fsa_set_debug(fsa_sim *s, unsigned flags) {
     CHECK_SIM(s);
     if ((flags & FSA_DEBUG_PAIR_SUMS) && !s->rates && s->P.nf > 0) {
         int rc = s->alloc(&s->rates, s->P.nf);
-        if (rc) return rc;
+        if (rc || (rc = s->alloc(&s->pair_counts, s->P.nf))) return rc;
     }
     s->debug = flags;
     return FSA_OK;

@@ -43,7 +43,9 @@ struct fsa_sim {
     fsa::VirtExtra vx;
     float4 *virt_x = nullptr;  // constant wall positions (x, y, z, 0)
     float4 *rates = nullptr;   // fluid (ax, ay, az, drho) when debugging
-    unsigned long long *jumps = nullptr;
+    int4 *pair_counts = nullptr;  // and the partners each found per kind
+    unsigned long long *jumps = nullptr;   // [0]: particles that moved more than a cell in a step
+    unsigned long long *health = nullptr;  // = jumps + 1: FSA_HEALTH_* bits since the last upload
     // observables
     double *obs_partial = nullptr, *obs_out = nullptr;
     float tip_y = 3.0e38f;  // reference y of the gate tip row (none: above everything)

@@ -230,10 +230,11 @@ struct PackedSrc {
 // cnt with weight 0 (the sentinel partner; in the packed form the last entry
 // again, with d2 = kPadD2).  PACK: fluid partners come from their FP16
 // records.
-template <int KIND, int T, bool PACK>
+template <int KIND, int T, bool PACK, bool COUNT>
 __device__ __forceinline__ void evaluate(const DevParams &P, const Self &me, const float4 *__restrict__ pos,
                                          const float4 *__restrict__ vel, const float4 *__restrict__ np,
-                                         const PackedSrc &ps, const Lists<T> &L, int cnt, Acc &acc) {
+                                         const PackedSrc &ps, const Lists<T> &L, int cnt, Acc &acc, int &found) {
+    if (COUNT) found += cnt;
     if (__ballot(cnt > 0) == 0) return;
     if constexpr (PACK && KIND == kFluidFluid) {
         uint4 rq[kPairBatch];
@@ -281,18 +282,18 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const Self &me, con
 // end, masked out), and a full list or a full spare table is evaluated by
 // all lanes together.  The next batch of candidates is loaded while the
 // current one is tested.
-template <int KIND, int T, bool PACK>
+template <int KIND, int T, bool PACK, bool COUNT>
 __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col, int b, int e,
                                      const float4 *__restrict__ pos, const float4 *__restrict__ vel,
                                      const float4 *__restrict__ np, const PackedSrc &ps, const Lists<T> &L, int &cnt,
-                                     int &spare, Acc &acc) {
+                                     int &spare, Acc &acc, int &found) {
     int sb = b;
     for (int sub = 0; __ballot(sb < e) != 0; ++sub, sb += kRunMax) {
         const bool has = sb < e;
         int slot = col;
         if (sub > 0) {
             if (__ballot(has && spare == kSpareSlots) != 0) {
-                evaluate<KIND, T, PACK>(P, me, pos, vel, np, ps, L, cnt, acc);
+                evaluate<KIND, T, PACK, COUNT>(P, me, pos, vel, np, ps, L, cnt, acc, found);
                 cnt = 0;
                 spare = 0;
             }
@@ -340,7 +341,7 @@ __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col
             // at most kScanBatch entries are added per round; the bases stay
             // valid across an evaluation
             if (__ballot(cnt > kListCap - kScanBatch) != 0) {
-                evaluate<KIND, T, PACK>(P, me, pos, vel, np, ps, L, cnt, acc);
+                evaluate<KIND, T, PACK, COUNT>(P, me, pos, vel, np, ps, L, cnt, acc, found);
                 cnt = 0;
             }
         }
@@ -358,12 +359,12 @@ __device__ __forceinline__ void scan(const DevParams &P, const Self &me, int col
 // solid in x; the walls have a per-cell reach flag.  Either decides without
 // loads of column ranges; then the whole-column ranges do, and only a wave
 // with walls in reach takes their z-windows.
-template <int KIND, int T, bool PACK>
+template <int KIND, int T, bool PACK, bool COUNT>
 __device__ __forceinline__ void pass(const DevParams &P, const Self &me, const Columns &cols, const Window &w,
                                      int n, const int32_t *__restrict__ start,
                                      const int32_t *__restrict__ fine, const float4 *__restrict__ pos,
                                      const float4 *__restrict__ vel, const float4 *__restrict__ np,
-                                     const PackedSrc &ps, const Lists<T> &L, Acc &acc,
+                                     const PackedSrc &ps, const Lists<T> &L, Acc &acc, int &found,
                                      const int32_t *__restrict__ sorted_cells = nullptr,
                                      const uint8_t *__restrict__ reach = nullptr) {
     if (KIND == kFluidSolid && sorted_cells) {
@@ -397,9 +398,9 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, const C
         const int b = cb[col], e = ce[col];
         if (__ballot(b < e) == 0) continue;
         L.base[col * T] = slot_bytes(b);
-        scan<KIND, T, PACK>(P, me, col, b, e, pos, vel, np, ps, L, cnt, spare, acc);
+        scan<KIND, T, PACK, COUNT>(P, me, col, b, e, pos, vel, np, ps, L, cnt, spare, acc, found);
     }
-    evaluate<KIND, T, PACK>(P, me, pos, vel, np, ps, L, cnt, acc);
+    evaluate<KIND, T, PACK, COUNT>(P, me, pos, vel, np, ps, L, cnt, acc, found);
 }
 
 template <bool RATES, int T, bool PACK>
@@ -443,11 +444,15 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     const PackedSrc fsrc{a.fpk, a.fcell}, none{nullptr, nullptr};
     const Columns cols(P, cell);
     const Window w(P, cols, me.x, me.y, me.z);
-    pass<kFluidFluid, T, PACK>(P, me, cols, w, P.nf, a.fstart, a.ffine, a.fpos, a.fvel, nullptr, fsrc, list, acc);
-    pass<kFluidSolid, T, false>(P, me, cols, w, P.ns, a.sstart, nullptr, a.spos, a.svel, a.snp, none, list, acc,
-                                a.scell);
-    pass<kFluidVirt, T, false>(P, me, cols, w, P.nv, a.vstart, a.vfine, a.vpos, a.vvel, nullptr, none, list, acc,
-                               nullptr, a.vreach);
+    // (debug runs count each kind's partners: the neighbour sets the walk found)
+    int found[3] = {0, 0, 0};
+    pass<kFluidFluid, T, PACK, RATES>(P, me, cols, w, P.nf, a.fstart, a.ffine, a.fpos, a.fvel, nullptr, fsrc, list,
+                                      acc, found[0]);
+    pass<kFluidSolid, T, false, RATES>(P, me, cols, w, P.ns, a.sstart, nullptr, a.spos, a.svel, a.snp, none, list,
+                                       acc, found[1], a.scell);
+    pass<kFluidVirt, T, false, RATES>(P, me, cols, w, P.nv, a.vstart, a.vfine, a.vpos, a.vvel, nullptr, none, list,
+                                      acc, found[2], nullptr, a.vreach);
+    if (RATES) a.counts[i] = make_int4(found[0], found[1], found[2], 0);
 
     // fluidUpdate (fluid.cu:234-269): NaN scrub, explicit step with the old
     // velocity for the position, density clamp, EOS, domain clamp, speed cap
@@ -498,6 +503,10 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
         uz *= f;
         cv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
+    // a non-finite velocity survives every clamp above (the scrub covers the
+    // pair sums only): the scene has diverged
+    if (!(__builtin_isfinite(ux) && __builtin_isfinite(uy) && __builtin_isfinite(uz)))
+        atomicOr(a.health, (unsigned long long)FSA_HEALTH_FLUID_NONFINITE);
     a.fpos_out[i] = make_float4(xc, yc, zc, rho);
     a.fvel_out[i] = make_float4(ux, uy, uz, p);
     if (a.fcpos) {

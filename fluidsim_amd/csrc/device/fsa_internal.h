@@ -94,6 +94,8 @@ struct DevParams {
 };
 
 DevParams make_dev_params(const fsa_params &p);
+// R_rec of the coupling records, for fluid speeds up to vmax (see fsa_capi.hip)
+void set_record_reach(DevParams &d, const fsa_params &p, double vmax);
 
 // buffers of one particle kind
 struct KindBuffers {
@@ -233,8 +235,10 @@ struct FluidStepArgs {
     const float4 *vpos, *vvel;
     float4 *fpos_out, *fvel_out;
     float4 *rates;  // (ax, ay, az, drho) or null
+    int4 *counts;   // with rates: the partners found per kind (fluid, solid, wall, 0)
     const int32_t *scell;   // the solid's cells in slot order (null without solid particles)
     const uint8_t *vreach;  // per cell: walls within its 27 cells (null without walls)
+    unsigned long long *health;  // FSA_HEALTH_* bits of the scene (an updated velocity not finite)
 };
 struct FluidJob {
     DevParams P;
@@ -272,6 +276,7 @@ struct SolidArgs {
     SolidExtra x;  // device pointers, passed by value
     int fluid_rec;  // the coupling pass of this step (this order) recorded the fluid partners
     int solid_rec;  // later passes replay the solid partners the first deformation pass recorded
+    unsigned long long *health;  // FSA_HEALTH_* bits of the scene
 };
 struct SolidJob {
     DevParams P;

@@ -434,6 +434,8 @@ __device__ __forceinline__ void update_body(const DevParams &P, const SolidArgs 
             py = fminf(fmaxf(py, P.dmin[1]), P.dmax[1]);
             pz = fminf(fmaxf(pz, P.dmin[2]), P.dmax[2]);
         }
+        if (!(__builtin_isfinite(un.x) && __builtin_isfinite(un.y) && __builtin_isfinite(un.z)))
+            atomicOr(a.health, (unsigned long long)FSA_HEALTH_SOLID_NONFINITE);
         a.spos[i] = make_float4(px, py, pz, 0.0f);
         a.svel[i] = un;
     }

@@ -11,7 +11,11 @@ uploading) overlaps the chunk in flight on the device, since each context
 uploads on its own stream.
 
 A scene's result does not depend on which other scenes shared its chunks:
-a batched step is bit-identical to stepping the scene on its own.
+a batched step is bit-identical to stepping the scene on its own.  A scene
+whose state diverges (a non-finite velocity, fsa_get_health) is evicted
+after the chunk in which it happened, with status "diverged", so that it
+does not slow its neighbours (a torn-apart gate crowds its particles into a
+few cells, where the per-cell work grows with the square of the count).
 
 The reference has no counterpart (it runs one scene per process,
 src/main.cu); the step each chunk repeats is its step loop body
@@ -46,7 +50,8 @@ class Result:
     counts: tuple
     observables: dict
     state: dict | None = None
-    status: str = "done"        # or "rejected" (larger than the whole budget)
+    status: str = "done"        # or "rejected" (larger than the whole budget), or "diverged"
+    health: int = 0             # FSA_HEALTH_* bits of a diverged scene
 
     @property
     def latency(self) -> float:
@@ -71,6 +76,9 @@ class ServerStats:
     max_resident_bytes: int = 0
     max_active: int = 0
     plans_built: int = 0
+    tunes: int = 0          # autotuner timings (FSA_BATCH_AUTOTUNE)
+    tune_ms: float = 0.0    # their host wall time
+    diverged: int = 0
     events: list = field(default_factory=list)
     chunk_log: list = field(default_factory=list)  # (start, end, active scenes, steps, particles)
 
@@ -101,7 +109,7 @@ class Server:
         self.autotune = autotune  # time the fluid workgroup sizes whenever the batch is re-recorded
         self.keep_state = keep_state
         self.saturation = saturation
-        self.on_event = on_event  # called as on_event(kind, request id, time), kind "admit" or "finish"
+        self.on_event = on_event  # on_event(kind, request id, time), kind "admit", "finish" or "diverged"
         self.stats = ServerStats()
 
     def _event(self, kind: str, rid: int, t: float):
@@ -202,6 +210,15 @@ class Server:
                 for x in active:
                     if any(x is y for y in stepped):
                         x.remaining -= k
+                        bad = x.sim.health()
+                        if bad:
+                            x.sim.close()
+                            results.append(Result(x.req.rid, x.req.arrival, x.admitted, now, x.req.steps - x.remaining,
+                                                  x.counts, {}, None, "diverged", bad))
+                            used -= x.resident
+                            st.diverged += 1
+                            self._event("diverged", x.req.rid, now)
+                            continue
                     if x.remaining == 0:
                         results.append(self._finish(x, now))
                         used -= x.resident
@@ -209,7 +226,8 @@ class Server:
                     else:
                         still.append(x)
                 active = still
-            st.plans_built = batch.stats()["plans_built"]
+            bs = batch.stats()
+            st.plans_built, st.tunes, st.tune_ms = bs["plans_built"], bs["tunes"], bs["tune_ms"]
         finally:
             for x in active:
                 x.sim.close()

@@ -109,6 +109,12 @@ class DeviceSim:
         check(self.lib.fsa_download_fluid_rates(self.handle, _ptr(drho), _ptr(a)))
         return {"drho_dt": drho, "du_dt": a}
 
+    def pair_counts(self) -> np.ndarray:
+        """(N, 3) partners per fluid particle and kind found by the last fluid stage (debug runs)."""
+        out = np.zeros((self.nf, 3), np.int32)
+        check(self.lib.fsa_download_pair_counts(self.handle, out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
+
     def solid(self) -> dict:
         n = self.ns
         out = {k: np.zeros((n, 3), np.float32) for k in ("position", "velocity", "normal", "du_dt", "couple_dudt")}
@@ -152,6 +158,12 @@ class DeviceSim:
     def jump_count(self) -> int:
         v = C.c_int64()
         check(self.lib.fsa_get_jump_count(self.handle, C.byref(v)))
+        return int(v.value)
+
+    def health(self) -> int:
+        """FSA_HEALTH_* bits: a non-finite velocity was written since the last upload."""
+        v = C.c_uint()
+        check(self.lib.fsa_get_health(self.handle, C.byref(v)))
         return int(v.value)
 
     def observe(self) -> dict:
@@ -251,7 +263,10 @@ class DeviceBatch:
     def stats(self) -> dict:
         plans, steps, threads = C.c_int64(), C.c_int64(), C.c_int()
         check(self.lib.fsa_batch_stats(self.handle, C.byref(plans), C.byref(steps), C.byref(threads)))
-        return {"plans_built": int(plans.value), "steps_run": int(steps.value), "fluid_threads": int(threads.value)}
+        tunes, tune_ms = C.c_int64(), C.c_double()
+        check(self.lib.fsa_batch_tuning(self.handle, C.byref(tunes), C.byref(tune_ms)))
+        return {"plans_built": int(plans.value), "steps_run": int(steps.value), "fluid_threads": int(threads.value),
+                "tunes": int(tunes.value), "tune_ms": float(tune_ms.value)}
 
 
 def footprint(params: FsaParams) -> tuple:

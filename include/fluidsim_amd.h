@@ -211,10 +211,25 @@ int fsa_get_sorted_ids(fsa_sim *sim, int kind, int32_t *sorted_id);
 /* Fluid pair sums of the last fluid stage (drho_dt N floats, du_dt 3N),
  * in .dat order.  Only recorded while FSA_DEBUG_PAIR_SUMS is set. */
 int fsa_download_fluid_rates(fsa_sim *sim, float *drho_dt, float *du_dt);
+/* Partners the last fluid stage found per fluid particle and kind (3N ints:
+ * fluid, solid, wall; the particle itself counts as a fluid partner), in
+ * .dat order: the neighbour sets of findFluidPair (src/fluid.cu:188-227).
+ * Only recorded while FSA_DEBUG_PAIR_SUMS is set. */
+int fsa_download_pair_counts(fsa_sim *sim, int32_t *counts);
 /* Number of particles that moved more than one cell in one step since the
  * context was created (the reference silently dropped such particles from
  * its lists, src/pair.cu:87-150; this build keeps them). */
 int fsa_get_jump_count(fsa_sim *sim, int64_t *count);
+/* Divergence guard (no counterpart in the reference, which only zeroes NaN
+ * pair sums, src/fluid.cu:246-249): the fluid and solid updates set a bit
+ * when a velocity they write is not finite, which no later clamp repairs
+ * (a NaN position clamps to the domain box, a NaN velocity stays).  Bits
+ * accumulate from the last upload of the kind on; reading syncs the
+ * context's stream.  A batch leaves each scene's bits in its own context, so
+ * a caller finds a diverged scene of a batch by asking each. */
+#define FSA_HEALTH_FLUID_NONFINITE 1u
+#define FSA_HEALTH_SOLID_NONFINITE 2u
+int fsa_get_health(fsa_sim *sim, unsigned *flags);
 
 #define FSA_DEBUG_PAIR_SUMS 1u
 /* The solid sub-steps walk their candidate cells in every pass instead of
@@ -335,8 +350,10 @@ int fsa_device_memory(int64_t *free_bytes, int64_t *total_bytes);
 typedef struct fsa_batch fsa_batch;
 
 #define FSA_BATCH_NO_GRAPH 1u /* launch every stage directly instead of replaying a HIP graph */
-#define FSA_BATCH_AUTOTUNE 2u /* time the batched fluid pass at each workgroup size whenever the
-                                 launches are recorded, and keep the fastest */
+#define FSA_BATCH_AUTOTUNE 2u /* time the batched fluid pass at each workgroup size when the
+                                 launches are recorded, and keep the fastest; a chain re-times
+                                 only when its fluid particle total has moved by more than a
+                                 quarter since it last did */
 
 /* stream: hipStream_t to run on, or NULL for a batch-owned stream. */
 int fsa_batch_create(void *stream, unsigned flags, fsa_batch **out);
@@ -353,6 +370,9 @@ int fsa_batch_synchronize(fsa_batch *batch);
  * Without FSA_BATCH_AUTOTUNE a recording uses the first listed scene's size
  * (fsa_set_fluid_threads). */
 int fsa_batch_stats(fsa_batch *batch, int64_t *plans_built, int64_t *steps_run, int *fluid_threads);
+/* FSA_BATCH_AUTOTUNE: how many timings the chains ran and the host wall time
+ * they took, in ms (no counterpart in the reference). */
+int fsa_batch_tuning(fsa_batch *batch, int64_t *tunes, double *tune_ms);
 
 #ifdef __cplusplus
 }

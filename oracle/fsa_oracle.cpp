@@ -231,6 +231,7 @@ struct OracleBase {
     virtual int brute_neighbors(int kind_i, int i, unsigned mask, int reference_metric, int32_t *out_kind,
                                 int32_t *out_j, int cap) = 0;
     virtual int64_t lost_count() = 0;
+    virtual void pair_counts(int brute, int32_t *out) = 0;
     void step() {
         particle_cell_update();
         fill_cell();
@@ -824,6 +825,26 @@ template <class T> struct Oracle : OracleBase {
         });
         return n;
     }
+    // findFluidPair's neighbour sets (src/fluid.cu:188-227) as counts: per
+    // fluid particle, the fluid, solid and virtual particles within the
+    // cut-off (itself included), from the 27-cell walk or from every particle
+    void pair_counts(int brute, int32_t *out) override {
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int i = 0; i < P.nf; ++i) {
+            int32_t c[3] = {0, 0, 0};
+            const V3<T> xi = fx[i];
+            if (brute) {
+                for (int k = 0; k < 3; ++k)
+                    for (int j = 0; j < count(k); ++j)
+                        if (dist2(metric_pos(k, j, 0) - xi) < P.R2) ++c[k];
+            } else {
+                visit(cell[KF][i], 7u, [&](int k, int j) {
+                    if (dist2(metric_pos(k, j, 0) - xi) < P.R2) ++c[k];
+                });
+            }
+            for (int k = 0; k < 3; ++k) out[3 * (size_t)i + k] = c[k];
+        }
+    }
     int brute_neighbors(int ki, int i, unsigned mask, int ref, int32_t *ok, int32_t *oj, int cap) override {
         int n = 0;
         const V3<T> xi = metric_pos(ki, i, ref);
@@ -917,6 +938,7 @@ int orc_couple_calculation(orc_sim *s) { s->impl->couple_calculation(); return F
 int orc_virt_update(orc_sim *s) { s->impl->virt_update(); return FSA_OK; }
 int orc_fluid_calculation(orc_sim *s) { s->impl->fluid_calculation(); return FSA_OK; }
 int orc_solid_calculation(orc_sim *s) { s->impl->solid_calculation(); return FSA_OK; }
+int orc_pair_counts(orc_sim *s, int brute, int32_t *out) { s->impl->pair_counts(brute, out); return FSA_OK; }
 int orc_solid_deformation(orc_sim *s) { s->impl->solid_deformation(); return FSA_OK; }
 int orc_solid_force(orc_sim *s) { s->impl->solid_force(); return FSA_OK; }
 int orc_solid_update(orc_sim *s) { s->impl->solid_update(); return FSA_OK; }

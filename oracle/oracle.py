@@ -66,6 +66,7 @@ def lib():
             "orc_neighbors": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint, C.c_int, ip, ip, C.c_int]),
             "orc_brute_neighbors": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint, C.c_int, ip, ip, C.c_int]),
             "orc_lost_count": (C.c_int64, [vp]),
+            "orc_pair_counts": (C.c_int, [vp, C.c_int, ip]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -194,6 +195,14 @@ class Oracle:
                cap)
         assert n <= cap
         return list(zip(ok[:n].tolist(), oj[:n].tolist()))
+
+    def pair_counts(self, brute: bool = False) -> np.ndarray:
+        """(N, 3) fluid, solid and wall particles within the cut-off of each fluid
+        particle (itself included): the 27-cell walk of the current cells, or
+        every particle (brute)."""
+        out = np.zeros((self.nf, 3), np.int32)
+        _check(self.L.orc_pair_counts(self.h, 1 if brute else 0, _p(out, C.c_int32)))
+        return out
 
     def lost_count(self) -> int:
         return int(self.L.orc_lost_count(self.h))

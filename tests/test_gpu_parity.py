@@ -149,6 +149,41 @@ def test_replayed_partners_give_the_walk_bit_for_bit(gpu_available):
         d.close()
 
 
+def test_replay_holds_for_fast_fluid_and_a_moving_gate(gpu_available):
+    """The coupling pass records fluid candidates from the positions before
+    the fluid update; the force passes replay them after it.  Fluid driven
+    at the gate faster than the speed cap (first step, uploaded 0.5 c0) and
+    at the cap afterwards moves up to |u| dt in between, and a gate pushed
+    at 0.45 c0 leaves the 0.04 R replay zone within a step, so its later
+    sub-steps walk.  Replay must still give the walk's bits."""
+    from fluidsim_amd._native import DEBUG_WALK
+    sc = scenes.tiny_scene(seed=13, fluid=(16, 10, 6), gate=(4, 8, 6), gate_x0=11, solid_sub_step=4)
+    a = sc.arrays()
+    c0 = sc.sound_speed
+    a["fluid_velocity"][:, 0] = 0.5 * c0
+    a["fluid_velocity"][:, 1] = -0.05 * c0
+    free = a["solid_type"] == 0
+    a["solid_velocity"][free, 0] = -0.45 * c0
+    p = sc.params((len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])))
+    print("case fast fluid and moving gate, walk vs replay", p.n_fluid, p.n_solid, p.n_virt, flush=True)
+    devs = []
+    for flags in (0, DEBUG_WALK):
+        d = DeviceSim(p)
+        d.upload(a)
+        d.set_debug(flags)
+        devs.append(d)
+    for st in range(6):
+        for d in devs:
+            d.step(1)
+        s0, s1 = devs[0].solid(), devs[1].solid()
+        for key in ("position", "velocity", "du_dt", "couple_dudt", "deformation"):
+            np.testing.assert_array_equal(s0[key], s1[key], err_msg=f"{key} after {st + 1} steps")
+        np.testing.assert_array_equal(devs[0].fluid()["velocity"], devs[1].fluid()["velocity"])
+    assert np.abs(devs[0].solid()["couple_dudt"]).max() > 0, "the fluid must press on the gate"
+    for d in devs:
+        d.close()
+
+
 @pytest.mark.parametrize("steps", [1, 20])
 def test_multi_step_state(gpu_available, tiny, steps):
     a, p, dev, o = _setup(tiny)

@@ -87,3 +87,34 @@ def test_admission_respects_the_memory_budget(gpu_available):
     assert srv.stats.max_active < 5
     lat = [by[r.rid].latency for r in reqs[:5]]
     assert percentile(lat, 50) <= percentile(lat, 99)
+
+
+def test_a_diverged_scene_is_evicted_and_the_others_keep_their_bits(gpu_available):
+    """A NaN velocity in one gate (a scene that has torn apart) sets its
+    FSA_HEALTH bit; the server evicts it after that chunk with status
+    "diverged", and every other scene of the mix still gives the bits of its
+    solve alone."""
+    from fluidsim_amd._native import HEALTH_SOLID_NONFINITE
+    reqs = _requests()[:4]
+    bad = reqs[1]
+    a = bad.scene.arrays()
+    free = np.flatnonzero(a["solid_type"] == 0)
+    a["solid_velocity"][free[len(free) // 2]] = [np.nan, 0.0, 0.0]
+    reqs[1] = Request(bad.rid, bad.scene, bad.steps, bad.arrival, a)
+    srv = Server(memory_budget=1 << 34, max_scenes=3, chunk_steps=4, keep_state=True)
+    print("case serving mix with a diverged scene", [(r.rid, r.steps, r.arrival) for r in reqs], flush=True)
+    res = srv.run(reqs, clock=StepClock(srv))
+    by = {r.rid: r for r in res}
+    assert by[bad.rid].status == "diverged"
+    assert by[bad.rid].health & HEALTH_SOLID_NONFINITE
+    assert by[bad.rid].steps <= srv.chunk_steps  # evicted after its first chunk
+    assert srv.stats.diverged == 1
+    for req in reqs:
+        if req.rid == bad.rid:
+            continue
+        r = by[req.rid]
+        assert r.status == "done"
+        ref = _alone(req)
+        _same(r.state["fluid"], ref["fluid"], f"request {r.rid} fluid")
+        _same(r.state["solid"], ref["solid"], f"request {r.rid} solid")
+        assert r.observables == ref["obs"], f"request {r.rid} observables"

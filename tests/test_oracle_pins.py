@@ -106,6 +106,23 @@ def test_neighbour_sets_agree_three_ways(tiny):
                 assert len(s_can) == len(set(s_can))
 
 
+def test_pair_counts_of_the_cell_walk_equal_brute_force(tiny):
+    """findFluidPair's partner sets as counts per fluid particle and kind: the
+    27-cell walk finds exactly the particles an all-pairs scan finds, on the
+    initial lattice and after some steps of motion."""
+    sc, p, a = tiny
+    o = orc.Oracle(p, 32)
+    o.upload(a)
+    for steps in (0, 5):
+        o.step(steps)
+        o.particle_cell_update()
+        o.fill_cell()
+        walk, brute = o.pair_counts(), o.pair_counts(brute=True)
+        np.testing.assert_array_equal(walk, brute)
+        assert walk[:, 0].min() >= 1  # itself
+        assert walk[:, 2].max() > 0  # walls in reach (the gate stands clear of the column here)
+
+
 def test_reference_lists_lose_a_particle_that_jumps_two_cells(tiny):
     """The reference rebuilds a cell's list from its 27 neighbours' old lists
     (src/pair.cu:87-150), so a particle that moves more than one cell in a
