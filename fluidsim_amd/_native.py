@@ -28,6 +28,7 @@ DEBUG_PAIR_SUMS = 1
 DEBUG_WALK = 2
 HEALTH_FLUID_NONFINITE = 1
 HEALTH_SOLID_NONFINITE = 2
+PROBE_NOALLOC_MAX = 65536  # FSA_PROBE_NOALLOC_MAX
 FORM_PACKED = 1
 FORM_COMPENSATED = 2
 
@@ -153,6 +154,7 @@ DEVICE_API = {
     "fsa_set_debug": (C.c_int, [_vp, C.c_uint]),
     "fsa_observe": (C.c_int, [_vp, _P(FsaObservables)]),
     "fsa_probe_pressure": (C.c_int, [_vp, _f, C.c_int, _f]),
+    "fsa_probe_pressure_noalloc": (C.c_int, [_vp, _f, C.c_int, _f]),
     "fsa_pair_census": (C.c_int, [_vp, _P(C.c_int64)]),
     "fsa_set_timing": (C.c_int, [_vp, C.c_int]),
     "fsa_get_stage_times": (C.c_int, [_vp, _d, _P(C.c_int64)]),

@@ -5,6 +5,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -25,6 +26,37 @@ int fsa::set_err(int code, const std::string &msg) {
 }
 
 #define HIP_TRY(expr) FSA_HIP_TRY(expr)
+
+// ---- the staging area of fsa_probe_pressure_noalloc ----------------------------
+// Points (x, y, z) of up to FSA_PROBE_NOALLOC_MAX probes, then their
+// pressures: 4 floats per probe, 1 MiB, one copy on every device the library
+// runs on, held until the process ends.  In host code `scratch` names a
+// host-side stand-in, not the device array: host code uses only the device
+// address hipGetSymbolAddress returns for the current device, kept per device
+// below and per context in fsa_sim::probe_scratch.
+static __device__ __align__(256) float scratch[4 * FSA_PROBE_NOALLOC_MAX];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_addr[kMaxDevices] = {};
+
+// the device address of `scratch` on `device`, which must be current; taken
+// once per device (at the first fsa_create on it)
+int scratch_address(int device, float **out) {
+    if (device < 0 || device >= kMaxDevices) return set_err(FSA_ERR_NODEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(FSA_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_addr[device] = static_cast<float *>(p);
+    }
+    *out = g_scratch_addr[device];
+    return FSA_OK;
+}
+}  // namespace
 
 namespace fsa {
 // The coupling pass records the fluid candidates of a solid particle within
@@ -495,6 +527,7 @@ int fsa_create(const fsa_params *params, void *stream, fsa_sim **out) {
         return code;
     };
     if ((rc = ensure_side(s))) return fail(rc);
+    if (!s->dry && (rc = scratch_address(s->device, &s->probe_scratch))) return fail(rc);
     if ((rc = alloc_all(s))) return fail(rc);
     if (hipMemsetAsync(s->jumps, 0, 2 * sizeof(unsigned long long), s->stream) != hipSuccess)
         return fail(set_err(FSA_ERR_HIP, "hipMemsetAsync failed"));
@@ -1179,18 +1212,36 @@ int fsa_probe_pressure(fsa_sim *s, const float *points, int n, float *pressure) 
     if (n == 0) return FSA_OK;
     if ((rc = check_in_grid(s, points, n, "probe"))) return rc;
     if (n > s->probe_cap) {
-        float4 *p = nullptr;
-        float *o = nullptr;
-        if ((rc = s->alloc(&p, (size_t)n)) || (rc = s->alloc(&o, (size_t)n))) return rc;
+        float *p = nullptr, *o = nullptr;
+        if ((rc = s->alloc(&p, 3 * (size_t)n)) || (rc = s->alloc(&o, (size_t)n))) return rc;
         s->probe_pts = p;  // the old buffers stay owned by the context until fsa_destroy
         s->probe_out = o;
         s->probe_cap = n;
     }
-    std::vector<float4> pts(n);
-    for (int i = 0; i < n; ++i) pts[i] = make_float4(points[3 * i], points[3 * i + 1], points[3 * i + 2], 0.0f);
-    HIP_TRY(hipMemcpyAsync(s->probe_pts, pts.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->probe_pts, points, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(launch_probe(s->P, observe_args(s), s->probe_pts, n, s->probe_out, s->stream));
     HIP_TRY(hipMemcpyAsync(pressure, s->probe_out, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return FSA_OK;
+}
+
+int fsa_probe_pressure_noalloc(fsa_sim *s, const float *points, int n, float *pressure) {
+    CHECK_SIM(s);
+    if (n < 0 || (n > 0 && (!points || !pressure))) return set_err(FSA_ERR_INVALID, "bad probe arguments");
+    if (n > FSA_PROBE_NOALLOC_MAX)
+        return set_err(FSA_ERR_INVALID, "fsa_probe_pressure_noalloc takes at most " +
+                                            std::to_string(FSA_PROBE_NOALLOC_MAX) + " points per call");
+    int rc = require_uploaded(s);
+    if (rc) return rc;
+    if (n == 0) return FSA_OK;
+    if ((rc = check_in_grid(s, points, n, "probe"))) return rc;
+    // the call writes every point it reads and every pressure it returns:
+    // nothing in the staging area needs clearing
+    float *pts = s->probe_scratch;
+    float *out = s->probe_scratch + 3 * FSA_PROBE_NOALLOC_MAX;
+    HIP_TRY(hipMemcpyAsync(pts, points, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(launch_probe(s->P, observe_args(s), pts, n, out, s->stream));
+    HIP_TRY(hipMemcpyAsync(pressure, out, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return FSA_OK;
 }

@@ -50,9 +50,12 @@ struct fsa_sim {
     double *obs_partial = nullptr, *obs_out = nullptr;
     float tip_y = 3.0e38f;  // reference y of the gate tip row (none: above everything)
     unsigned long long *census = nullptr;
-    float4 *probe_pts = nullptr;
+    float *probe_pts = nullptr;  // (x, y, z) per point
     float *probe_out = nullptr;
     int probe_cap = 0;
+    // this device's staging area of fsa_probe_pressure_noalloc (a device
+    // address of the library's static array, shared by every context on the device)
+    float *probe_scratch = nullptr;
 
     int fluid_threads = fsa::kFluidThreadsDefault;  // workgroup size of the fluid pass
 

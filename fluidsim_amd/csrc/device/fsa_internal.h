@@ -327,7 +327,8 @@ struct CensusArgs {
 hipError_t launch_census(const DevParams &P, const CensusArgs &a, unsigned long long *out, hipStream_t s);
 int observe_partials();  // doubles of scratch the sums need
 hipError_t launch_observe(const DevParams &P, const ObserveArgs &a, double *partial, double *out, hipStream_t s);
-hipError_t launch_probe(const DevParams &P, const ObserveArgs &a, const float4 *pts, int n, float *out,
+// pts: n packed (x, y, z) triples
+hipError_t launch_probe(const DevParams &P, const ObserveArgs &a, const float *pts, int n, float *out,
                         hipStream_t s);
 
 }  // namespace fsa

@@ -110,12 +110,13 @@ __global__ __launch_bounds__(kBlock) void k_sums_final(const double *__restrict_
     out[k] = acc;
 }
 
-// Shepard pressure at probe points (the solidPressureCalculate weights)
-__global__ __launch_bounds__(kBlock) void k_probe(DevParams P, ObserveArgs a, const float4 *__restrict__ pts, int n,
+// Shepard pressure at probe points (the solidPressureCalculate weights);
+// the points are packed (x, y, z) triples
+__global__ __launch_bounds__(kBlock) void k_probe(DevParams P, ObserveArgs a, const float *__restrict__ pts, int n,
                                                   float *__restrict__ out) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const float4 xi = pts[i];
+    const float3 xi = make_float3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
     float pressure = 0.0f, wsum = 0.0f;
     for_each_column(P, find_cell(P, xi.x, xi.y, xi.z), [&](int c0, int c1) {
         for (int j = a.fstart[c0], e = a.fstart[c1]; j < e; ++j) {
@@ -141,7 +142,7 @@ hipError_t launch_observe(const DevParams &P, const ObserveArgs &a, double *part
     return hipGetLastError();
 }
 
-hipError_t launch_probe(const DevParams &P, const ObserveArgs &a, const float4 *pts, int n, float *out,
+hipError_t launch_probe(const DevParams &P, const ObserveArgs &a, const float *pts, int n, float *out,
                         hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_probe, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, a, pts, n, out);

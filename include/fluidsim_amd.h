@@ -268,6 +268,17 @@ int fsa_observe(fsa_sim *sim, fsa_observables *out);
  * p = sum p_j w_j / sum w_j, w = (exp(-q^2) - c1) c2 over fluid within K h;
  * 0 where the sum of weights is <= 1e-6.  Points must lie in the cell grid. */
 int fsa_probe_pressure(fsa_sim *sim, const float *points, int n, float *pressure);
+/* fsa_probe_pressure without allocating device memory.  No counterpart in
+ * the reference: this is for callers that cannot allocate (code between
+ * steps of a serving loop or a harness that forbids device allocation);
+ * fsa_probe_pressure allocates staging buffers when n grows.  The points and
+ * their pressures are staged in a static area of the library, one per
+ * device (1 MiB, held until the process ends), so n is at most
+ * FSA_PROBE_NOALLOC_MAX (else FSA_ERR_INVALID), and calls that overlap on
+ * one GPU -- from any context on it, on any thread -- race on that area.
+ * Results are bit-identical to fsa_probe_pressure. */
+#define FSA_PROBE_NOALLOC_MAX 65536
+int fsa_probe_pressure_noalloc(fsa_sim *sim, const float *points, int n, float *pressure);
 
 /* Work census of the fluid pass on the current state: counts[k] = candidate
  * tests and counts[3+k] = partners inside the cut-off, for partner kind k
