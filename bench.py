@@ -3,14 +3,18 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--scene c2|c1|tiny]
     python bench.py --workload batch [--batch-scenes 64] [--steps K]
-    python bench.py --workload serve [--requests 16] [--load 0.8] [--sim-time 0.5]
+    python bench.py --workload serve [--requests 64] [--load 0.8] [--sim-time 0.5]
+    python bench.py --workload precision [--steps 1000]
 
 One step is one pass of the reference's step loop body (src/calculation.cu:23-33:
 cell hash, cell lists, solid coupling fields, walls, fluid pair+update,
 solid_sub_step solid sub-steps) over the resident scene.  The default scene
 is the 4,000,000 fluid + 27,000 solid dam break (dp = 1 mm, 3 wall layers,
 h = 1.3 dp, dt = 0.25 h / c0, 4 solid sub-steps).  Inputs are uploaded once
-before the timed region.  Prints one JSON line on stdout.
+before the timed region.  The default line also carries the 500-step run of
+the 43,750 + 2,000 particle scene (`c1_gpu`), the 64-scene batched step
+(`batched`), 64 served requests of 0.05 s simulated time each (`serving`,
+p50/p99 time-to-result) and the CPU baseline.  Prints one JSON line on stdout.
 """
 from __future__ import annotations
 
@@ -96,8 +100,20 @@ def sync_all(dev):
         pass
 
 
+def host_threads() -> int:
+    """Threads for the CPU baseline: OMP_NUM_THREADS if set (the host share of
+    one GPU on a shared box), else every core this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
 def cpu_baseline(params, arrays, budget_s: float, threads: int) -> dict:
-    """The FP32 oracle (OpenMP) on the same scene: a bounded sample of whole steps."""
+    """The FP32 oracle (OpenMP over particles) on the same scene: three timed
+    samples of whole steps after one untimed step (median reported), each
+    about a third of `budget_s`; then the full 500-step run of the
+    43,750 + 2,000 particle scene."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
 
@@ -107,25 +123,39 @@ def cpu_baseline(params, arrays, budget_s: float, threads: int) -> dict:
     t0 = time.perf_counter()
     o.step(1)
     first = time.perf_counter() - t0
-    steps = 1
-    more = max(0, min(20, int(budget_s / max(first, 1e-3)) - 1))
-    if more:
-        t1 = time.perf_counter()
-        o.step(more)
-        total = time.perf_counter() - t1
-        steps_timed, secs = more, total
-    else:
-        steps_timed, secs = 1, first
-    steps += more
-    o.close()
+    per = max(1, min(20, int(budget_s / 3 / max(first, 1e-3))))
     n = params.n_fluid + params.n_solid
+    samples = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        o.step(per)
+        samples.append(n * per / (time.perf_counter() - t1))
+    o.close()
+    # the CPU-scale scene, run in full
+    c1 = scenes.c1_cpu_scene(seed=0)
+    a1 = c1.arrays()
+    p1 = c1.params((len(a1["fluid_position"]), len(a1["solid_position"]), len(a1["virt_position"])))
+    o = oracle.Oracle(p1, 32, oracle.CANONICAL)
+    o.upload(a1)
+    t1 = time.perf_counter()
+    o.step(c1.iend)
+    c1_s = time.perf_counter() - t1
+    o.close()
+    n1 = p1.n_fluid + p1.n_solid
     return {
-        "value": n * steps_timed / secs,
+        "value": float(np.median(samples)),
         "unit": "particle-steps/s",
         "cores": used,
         "kind": "port",
-        "sample": f"{steps_timed} step(s) of the same scene after 1 untimed step "
+        "sample": f"median of 3 samples of {per} step(s) of the same scene after 1 untimed step "
                   f"({n} particles, FP32 canonical traversal, OpenMP over particles)",
+        "samples": [round(v, 1) for v in samples],
+        "omp_max_threads": used,
+        "nproc": os.cpu_count(),
+        "affinity_cpus": len(os.sched_getaffinity(0)),
+        "c1_full_run": {"workload": f"{p1.n_fluid} fluid + {p1.n_solid} solid + {p1.n_virt} walls, "
+                                    f"{c1.iend} steps", "seconds": round(c1_s, 3),
+                        "particle_steps_per_s": n1 * c1.iend / c1_s},
     }
 
 
@@ -396,8 +426,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 20; batch workload 3)")
     ap.add_argument("--scene", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-budget", type=float, default=12.0,
+                    help="seconds of CPU oracle work on the headline scene (3 samples), before the 500-step CPU scene")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0: OMP_NUM_THREADS, else every core of the affinity mask)")
     ap.add_argument("--workload", choices=["step", "batch", "serve", "precision"], default="step",
                     help="step: one scene's step throughput (the headline line, plus a short batched "
                          "measurement); batch: only the many-scene batched measurement")
@@ -407,7 +439,13 @@ def main():
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--batch-streams", type=int, default=2, help="concurrent launch chains of the batch (1..8)")
     ap.add_argument("--sim-time", type=float, default=0.5, help="simulated seconds per solve (scenes/hour)")
-    ap.add_argument("--requests", type=int, default=16, help="serve workload: number of requests")
+    ap.add_argument("--requests", type=int, default=64, help="serve workload: number of requests")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement of the step workload")
+    ap.add_argument("--serve-requests", type=int, default=64,
+                    help="requests of the step workload's serving measurement")
+    ap.add_argument("--serve-sim-time", type=float, default=0.05,
+                    help="simulated seconds per request of the step workload's serving measurement "
+                         "(the serve workload uses --sim-time)")
     ap.add_argument("--load", type=float, default=0.8, help="serve workload: offered load")
     ap.add_argument("--chunk", type=int, default=64, help="serve workload: steps between admission points")
     ap.add_argument("--saturation", type=int, default=0,
@@ -461,7 +499,7 @@ def main():
 
     if args.workload == "precision":
         steps = args.steps or 100
-        r = precision_bench(steps, args.batch_seed, args.cpu_threads)
+        r = precision_bench(steps, args.batch_seed, args.cpu_threads or host_threads())
         pc = r["forms"]["packed+compensated"]["errors"][str(steps)]
         out = {
             "metric": "gate-tip displacement error vs FP64 oracle (packed+compensated form)",
@@ -599,13 +637,36 @@ def main():
         "roofline": roofline,
     }
     dev.close()
+    # the CPU-scale scene's full 500-step run on the GPU, beside the CPU's
+    c1 = scenes.c1_cpu_scene(seed=0)
+    a1 = c1.arrays()
+    d1 = DeviceSim(c1.params((len(a1["fluid_position"]), len(a1["solid_position"]), len(a1["virt_position"]))))
+    d1.upload(a1)
+    d1.step(1)
+    sync_all(d1)
+    t0 = time.perf_counter()
+    d1.step(c1.iend)
+    sync_all(d1)
+    c1_s = time.perf_counter() - t0
+    result["c1_gpu"] = {"workload": f"{d1.nf} fluid + {d1.ns} solid + {d1.nv} walls, {c1.iend} steps "
+                                    "(after 1 untimed step)", "seconds": round(c1_s, 4),
+                        "particle_steps_per_s": (d1.nf + d1.ns) * c1.iend / c1_s}
+    d1.close()
+    log(f"c1 scene on the GPU: {c1.iend} steps in {c1_s:.3f}s")
     if not args.no_batch:
         result["batched"] = batch_bench(args.batch_scenes, args.batch_steps, 3, args.batch_seed, args.sim_time,
                                         args.batch_streams)
+    if not args.no_serve:
+        result["serving"] = serve_bench(args.serve_requests, args.batch_seed, args.serve_sim_time, args.load,
+                                        args.chunk, args.saturation or None)
     if rank == 0 and not args.no_cpu_baseline:
-        log(f"cpu baseline: oracle FP32 on {args.cpu_threads} threads, ~{args.cpu_budget:.0f}s budget")
-        result["cpu_baseline"] = cpu_baseline(params, arrays, args.cpu_budget, args.cpu_threads)
-        log(f"cpu baseline: {result['cpu_baseline']['value']:.4g} particle-steps/s")
+        threads = args.cpu_threads or host_threads()
+        log(f"cpu baseline: oracle FP32 on {threads} threads (nproc {os.cpu_count()}), "
+            f"~{args.cpu_budget:.0f}s budget + the 500-step CPU scene")
+        cb = cpu_baseline(params, arrays, args.cpu_budget, threads)
+        result["cpu_baseline"] = cb
+        log(f"cpu baseline: {cb['value']:.4g} particle-steps/s (samples {cb['samples']}); "
+            f"500-step CPU scene {cb['c1_full_run']['seconds']:.1f}s")
     if rank == 0:
         print(json.dumps(result), flush=True)
 

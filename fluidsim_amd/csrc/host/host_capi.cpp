@@ -122,6 +122,8 @@ int fsh_run(const char *root, int verbose) {
         {
             fsa_host::Config c(std::string(root) + "/config/input.ini");
             sim.output_true_pressure = c.Read("output_fluid_pressure", false);
+            sim.vtk_crlf = c.Read("vtk_crlf", false);
+            sim.checkpoint_interval = c.Read("checkpoint_interval", 0);
         }
         sim.initiate();
         sim.calculation();

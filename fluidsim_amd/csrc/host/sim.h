@@ -52,6 +52,15 @@ public:
     // write the real fluid pressure into fluid_pressure-*.vtk instead of the
     // always-zero `color` field of the reference (src/output.cu:75)
     bool output_true_pressure = false;
+    // end .vtk lines with "\r\n", the bytes the reference's text-mode
+    // fstream writes on its Windows build (src/output.cu:32-35)
+    bool vtk_crlf = false;
+    // every checkpoint_interval steps (0: never), write the state after the
+    // step as a case directory <root>/checkpoint/step-<i>: the input.ini with
+    // istart = i and the three .dat files (include/host_data.cuh:25, 84, 152),
+    // which `fluidsim` resumes from
+    int checkpoint_interval = 0;
+    void writeCheckpoint(const std::string &dir);
 
 private:
     void readParticleData();
@@ -64,13 +73,18 @@ private:
     FluidHostList fluid_host_;
     SolidHostList solid_host_;
     VirtHostList virt_host_;
+    // the columns of the .dat files that never change on the device, kept
+    // for checkpoints: solid reference positions and types, walls
+    std::vector<float> solid_ref_, virt_pos_;
+    std::vector<int32_t> solid_type_, virt_type_;
 };
 
 // ASCII POLYDATA writers of src/output.cu:6-297 (std::to_string formatting)
 namespace vtk {
 void write_vectors(const std::string &path, const char *title, const char *name, const float *points,
-                   const float *vectors, int n);
-void write_scalars(const std::string &path, const char *name, const float *points, const float *scalars, int n);
+                   const float *vectors, int n, bool crlf = false);
+void write_scalars(const std::string &path, const char *name, const float *points, const float *scalars, int n,
+                   bool crlf = false);
 }  // namespace vtk
 
 }  // namespace fsa_host

@@ -113,7 +113,9 @@ typedef struct fsa_params {
     float kernel_coefficient_1;
     float kernel_coefficient_2;
     float kernel_differential;
-    int32_t thread_num; /* read for key compatibility; block size is fixed at 256 on gfx950 */
+    int32_t thread_num; /* read for key compatibility and not used: the fluid pass runs
+                           workgroups of 64, 128 or 256 threads (64 by default; see
+                           fsa_set_fluid_threads, fsa_autotune_fluid), the other passes 256 */
 
     /* Precision form: FSA_FORM_* bits, from the optional ini key
      * `precision_form` (no counterpart in the reference; 0 = its plain FP32

@@ -40,7 +40,13 @@ float fsh_kernel_coefficient(float impact_length_hsml_ratio);
 
 /* main() of the reference (src/main.cu:17-30) on a case directory: read,
  * run istart..iend with .vtk output every result_interval steps into
- * <root>/out.  verbose=0 silences the progress text. */
+ * <root>/out.  verbose=0 silences the progress text.  Optional ini keys
+ * with no counterpart in the reference: output_fluid_pressure = true
+ * (write p instead of the always-zero color into fluid_pressure-*.vtk),
+ * vtk_crlf = true (end .vtk lines with CR LF, as the reference's text-mode
+ * streams do on Windows), checkpoint_interval = N (every N steps write the
+ * state as a resumable case <root>/checkpoint/step-<i>: input.ini with
+ * istart = i and the three .dat files). */
 int fsh_run(const char *root, int verbose);
 
 #ifdef __cplusplus
