@@ -425,6 +425,11 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 1000; batch workload 30)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 20; batch workload 3)")
     ap.add_argument("--scene", default="c2")
+    ap.add_argument("--advance", type=int, default=0,
+                    help="step workload: untimed steps before the warm-up, to time a flow in motion "
+                         "(the default times the first steps from rest, as the configuration says)")
+    ap.add_argument("--skin", type=float, default=None,
+                    help="neighbour_skin of the step workload's scene (default: the library's; 0: walk every step)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0,
                     help="seconds of CPU oracle work on the headline scene (3 samples), before the 500-step CPU scene")
@@ -543,6 +548,8 @@ def main():
         return
 
     sc = scene_for(args.scene)
+    if args.skin is not None:
+        sc = sc.with_(neighbour_skin=args.skin)
     t = time.perf_counter()
     arrays = sc.arrays()
     nf, ns, nv = (len(arrays["fluid_position"]), len(arrays["solid_position"]), len(arrays["virt_position"]))
@@ -555,6 +562,12 @@ def main():
     sync_all(dev)
     log(f"uploaded; device memory {dev.memory_used() / 2**20:.0f} MiB")
 
+    if args.advance:
+        t = time.perf_counter()
+        dev.step(args.advance)
+        sync_all(dev)
+        log(f"advanced {args.advance} steps ({args.advance * sc.time_step:.4f} s simulated) in "
+            f"{time.perf_counter() - t:.1f}s; max fluid speed {dev.observe()['max_fluid_speed']:.3f} m/s")
     dev.step(args.warmup)
     sync_all(dev)
     threads = dev.autotune_fluid(3)
@@ -562,6 +575,7 @@ def main():
 
     dev.set_timing(True)
     dev.reset_stage_times()
+    lists0 = dev.list_stats()
     sync_all(dev)
     t0 = time.perf_counter()
     dev.step(args.steps)
@@ -569,10 +583,12 @@ def main():
     t1 = time.perf_counter()
     stage = dev.stage_times()
     dev.set_timing(False)
+    lists1 = dev.list_stats()
     secs = t1 - t0
     ms_per_step = 1e3 * secs / args.steps
     value = (nf + ns) * args.steps / secs
-    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.4g} particle-steps/s")
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.4g} particle-steps/s; "
+        f"neighbour lists {lists1}, {lists1['builds'] - lists0['builds']} builds in the timed steps")
     for k, (ms, n) in stage.items():
         if n:
             log(f"  stage {k:6s}: {ms / n:8.4f} ms avg over {n}")
@@ -582,13 +598,25 @@ def main():
     n_active = active_walls(params, fl["position"], arrays["virt_position"])
     fluid_ms, fluid_n = stage["fluid"]
     fluid_avg_s = fluid_ms / max(fluid_n, 1) / 1e3
-    bytes_per_launch = 68 * nf + 32 * n_active + 40 * ns + 8 * params.cell_number_total
-    achieved = bytes_per_launch / fluid_avg_s / 1e9
     # the pass is VALU-bound: price it in flops too (8 per candidate test,
     # 90 per in-range Riemann pair) from an exact census of this state
     census = dev.pair_census()
-    cand = sum(census["candidates"].values())
     pairs = sum(census["pairs"].values())
+    lists_on = params.neighbour_skin > 0 and lists1["builds"] > 0
+    if lists_on:
+        # with neighbour lists: per fluid particle its home state in (32 B),
+        # the update out to the home and the sorted buffers (64 B), its sorted
+        # slot, entry count and build position (24 B), its list (4 B per
+        # candidate); fluid candidates are the listed ones
+        entries = lists1["entries"]
+        bytes_per_launch = (120 * nf + 4 * entries + 32 * n_active + 40 * ns
+                            + 8 * params.cell_number_total)
+        cand = entries + census["candidates"]["solid"] + census["candidates"]["virt"]
+    else:
+        # per fluid particle its state in and out (64 B) and its cell (4 B)
+        bytes_per_launch = 68 * nf + 32 * n_active + 40 * ns + 8 * params.cell_number_total
+        cand = sum(census["candidates"].values())
+    achieved = bytes_per_launch / fluid_avg_s / 1e9
     flops = 8 * cand + 90 * pairs
     tflops = flops / fluid_avg_s / 1e12
     traffic = measured_traffic(args.scene)
@@ -632,9 +660,16 @@ def main():
             "dp": sc.dp, "h_over_dp": sc.h_ratio, "solid_sub_step": sc.solid_sub_step,
             "cells": params.cell_number_total,
             "fluid_threads": threads,
+            "advanced_steps": args.advance,
         },
         "stage_ms": {k: (ms / n if n else None) for k, (ms, n) in stage.items()},
         "roofline": roofline,
+        "neighbour_lists": {
+            "skin": round(float(params.neighbour_skin), 4),
+            "builds_in_timed_steps": lists1["builds"] - lists0["builds"],
+            "candidates_per_fluid_particle": round(lists1["entries"] / nf, 2),
+            "unlisted": lists1["unlisted"],
+        } if lists_on else None,
     }
     dev.close()
     # the CPU-scale scene's full 500-step run on the GPU, beside the CPU's

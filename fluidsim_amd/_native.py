@@ -83,6 +83,7 @@ class FsaParams(C.Structure):
         ("kernel_differential", C.c_float),
         ("thread_num", C.c_int32),
         ("precision_form", C.c_int32),
+        ("neighbour_skin", C.c_float),
     ]
 
     def as_dict(self) -> dict:
@@ -151,6 +152,7 @@ DEVICE_API = {
     "fsa_get_sorted_ids": (C.c_int, [_vp, C.c_int, _i]),
     "fsa_download_fluid_rates": (C.c_int, [_vp, _f, _f]),
     "fsa_get_jump_count": (C.c_int, [_vp, _P(C.c_int64)]),
+    "fsa_get_list_stats": (C.c_int, [_vp, _P(C.c_int64), _P(C.c_int64), _P(C.c_int64), _P(C.c_int64)]),
     "fsa_set_debug": (C.c_int, [_vp, C.c_uint]),
     "fsa_observe": (C.c_int, [_vp, _P(FsaObservables)]),
     "fsa_probe_pressure": (C.c_int, [_vp, _f, C.c_int, _f]),
@@ -207,7 +209,7 @@ def _bind(path: Path, api: dict):
     return lib
 
 
-ABI_VERSION = 2  # FSA_ABI_VERSION of the header these mirrors follow
+ABI_VERSION = 3  # FSA_ABI_VERSION of the header these mirrors follow
 
 
 def device_lib():

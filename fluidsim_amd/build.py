@@ -48,7 +48,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", f"-I{INCLUDE}", f"-I{HOST_SRC}"]
 
 DEVICE_FILES = ["fsa_cells.hip", "fsa_fluid.hip", "fsa_virt.hip", "fsa_solid.hip", "fsa_observe.hip", "fsa_capi.hip",
-                "fsa_batch.hip"]
+                "fsa_batch.hip", "fsa_pairs.hip"]
 HOST_FILES = ["config.cpp", "parameter.cpp", "sim.cpp", "host_capi.cpp"]
 
 

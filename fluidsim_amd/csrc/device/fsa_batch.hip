@@ -64,16 +64,20 @@ inline size_t align_up(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
 
 // indices of the double buffers of one scene at the start of a step
 struct Phase {
-    int fst, ford, sst, sord;
+    int fst, ford, fhst, sst, sord;
     bool operator==(const Phase &o) const {
-        return fst == o.fst && ford == o.ford && sst == o.sst && sord == o.sord;
+        return fst == o.fst && ford == o.ford && fhst == o.fhst && sst == o.sst && sord == o.sord;
     }
 };
-Phase phase_of(const fsa_sim *s) { return {s->fluid.st, s->fluid.ord, s->solid.st, s->solid.ord}; }
+Phase phase_of(const fsa_sim *s) {
+    return {s->fluid.st, s->fluid.ord, s->fluid.hst, s->solid.st, s->solid.ord};
+}
 // the indices one step later: the sort flips state and order of every kind
-// with particles, the fluid pass flips the fluid state back
+// with particles, the fluid pass flips the fluid state back (and the home
+// state of a scene with neighbour lists)
 Phase next_phase(const fsa_sim *s, Phase p) {
     if (s->P.nf > 0) p.ford ^= 1;
+    if (s->P.vlist) p.fhst ^= 1;
     if (s->P.ns > 0) {
         p.sst ^= 1;
         p.sord ^= 1;
@@ -83,14 +87,16 @@ Phase next_phase(const fsa_sim *s, Phase p) {
 void apply_phase(fsa_sim *s, const Phase &p) {
     s->fluid.st = p.fst;
     s->fluid.ord = p.ford;
+    s->fluid.hst = p.fhst;
     s->solid.st = p.sst;
     s->solid.ord = p.sord;
 }
 
 // one launch of a step: which op, which job table, its prefix table
 struct Launch {
-    enum Family { kCells, kCouple, kVirt, kFluid, kSolid } family;
+    enum Family { kCells, kList, kCouple, kVirt, kFluid, kSolid } family;
     BatchOp op;
+    ListOp lop;  // kList
     int njobs;
     size_t prefix_off;  // byte offset of the prefix table in the blob
     int nblocks;
@@ -98,7 +104,7 @@ struct Launch {
 
 // the launches and tables of one step
 struct StepTables {
-    size_t cells_off = 0, couple_off = 0, virt_off = 0, fluid_off = 0, solid_off = 0;
+    size_t cells_off = 0, list_off = 0, couple_off = 0, virt_off = 0, fluid_off = 0, solid_off = 0;
     int ncells = 0, nscenes = 0;
     std::vector<Launch> launches;
 };
@@ -168,13 +174,14 @@ template <class T> size_t put(std::vector<char> &blob, const T *data, size_t cou
 }
 
 void add_launch(std::vector<char> &blob, StepTables &t, Launch::Family fam, BatchOp op,
-                const std::vector<int> &blocks) {
+                const std::vector<int> &blocks, ListOp lop = kLDecide) {
     std::vector<int> prefix(blocks.size() + 1, 0);
     for (size_t i = 0; i < blocks.size(); ++i) prefix[i + 1] = prefix[i] + round_xcd(blocks[i]);
     if (prefix.back() == 0) return;  // no scene has work in this stage
     Launch l{};
     l.family = fam;
     l.op = op;
+    l.lop = lop;
     l.njobs = (int)blocks.size();
     l.nblocks = prefix.back();
     l.prefix_off = put(blob, prefix.data(), prefix.size());
@@ -186,6 +193,7 @@ void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph
                 std::vector<char> &blob, StepTables &t) {
     const int n = (int)sims.size();
     std::vector<CellJob> cells;
+    std::vector<ListJob> lists(n);
     std::vector<VirtJob> virt(n);
     std::vector<FluidJob> fluid(n);
     std::vector<SolidJob> couple(n), solid(n);
@@ -195,6 +203,7 @@ void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph
         KindBuffers f = s->fluid, so = s->solid;
         f.st = ph[i].fst;
         f.ord = ph[i].ford;
+        f.hst = ph[i].fhst;
         so.st = ph[i].sst;
         so.ord = ph[i].sord;
         cells.push_back(make_cell_job(s->P, f, s->jumps, true));
@@ -208,6 +217,7 @@ void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph
             so.st ^= 1;
             so.ord ^= 1;
         }
+        lists[i] = list_job(s, f);
         couple[i] = SolidJob{s->P, solid_args(s, f, so)};
         virt[i] = VirtJob{s->P, virt_args(s, f, 0)};
         fluid[i] = FluidJob{s->P, fluid_args(s, f, so)};
@@ -222,6 +232,7 @@ void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph
     t.nscenes = n;
     t.ncells = (int)cells.size();
     t.cells_off = put(blob, cells.data(), cells.size());
+    t.list_off = put(blob, lists.data(), lists.size());
     t.couple_off = put(blob, couple.data(), couple.size());
     t.virt_off = put(blob, virt.data(), virt.size());
     t.fluid_off = put(blob, fluid.data(), fluid.size());
@@ -233,6 +244,12 @@ void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph
         blocks.clear();
         for (const CellJob &j : cells) blocks.push_back(blocks_cells(op, j));
         add_launch(blob, t, Launch::kCells, op, blocks);
+    }
+    // neighbour lists: the build decision, then (if due) the new home order and the lists
+    for (ListOp op : {kLDecide, kLBuild}) {
+        blocks.clear();
+        for (const ListJob &j : lists) blocks.push_back(blocks_list(op, j));
+        add_launch(blob, t, Launch::kList, kBFillCounts, blocks, op);
     }
     // couple (src/solid.cu:346-359: only with both fluid and solid)
     blocks.clear();
@@ -261,6 +278,10 @@ hipError_t enqueue_step(Chain *b, const StepTables &t) {
         case Launch::kCells:
             e = launch_many_cells(l.op, reinterpret_cast<const CellJob *>(base + t.cells_off), prefix, l.njobs,
                                   l.nblocks, b->stream);
+            break;
+        case Launch::kList:
+            e = launch_many_list(l.lop, reinterpret_cast<const ListJob *>(base + t.list_off), prefix, l.njobs,
+                                 l.nblocks, b->stream);
             break;
         case Launch::kCouple:
             e = launch_many_solid(l.op, reinterpret_cast<const SolidJob *>(base + t.couple_off), prefix, l.njobs,

@@ -138,7 +138,38 @@ DevParams make_dev_params(const fsa_params &p) {
     d.form = p.precision_form;
     for (int a = 0; a < 3; ++a) d.ival[a] = p.interval[a];
     d.B_inv = p.fluid_coefficient_rho2p != 0.0f ? 1.0f / p.fluid_coefficient_rho2p : 0.0f;
+    // neighbour lists (fsa_pairs.hip); not with the packed form, whose
+    // partner records are written in sorted order by the sort
+    const double skin = p.neighbour_skin;
+    d.vlist = (skin > 0.0 && !(p.precision_form & FSA_FORM_PACKED) && p.n_fluid > 0) ? 1 : 0;
+    if (d.vlist) {
+        const double R = std::sqrt((double)p.impact_distance_square), reach = (1.0 + skin) * R;
+        // FP32 positions and squared distances carry relative errors of a few
+        // 1e-7: the margins keep every pair inside R now inside the reach of
+        // the build, with the moves bounded by the threshold
+        d.vreach2 = (float)(reach * reach * (1.0 + 2.0e-5));
+        d.vthr2 = (float)(0.25 * skin * skin * R * R * (1.0 - 2.0e-5));
+        // list rows: 1.5 x the lattice sites within the reach (room for a
+        // compressed or clustered neighbourhood; longer lists walk instead)
+        const double dp = p.particle_diameter > 0.0f ? (double)p.particle_diameter : R / 2.6;
+        const double sites = 4.0 / 3.0 * 3.141592653589793 * std::pow(reach / dp, 3.0);
+        d.vrows = std::min(std::max((int)std::ceil(1.5 * sites / 8.0) * 8, 32), kListRowsMax);
+    }
     return d;
+}
+
+ListJob list_job(const fsa_sim *s, const KindBuffers &f) {
+    ListJob j{};
+    j.P = s->P;
+    j.spos = f.pos[f.st];
+    j.svel = f.vel[f.st];
+    j.hpos = f.hpos[f.hst];
+    j.hvel = f.hvel[f.hst];
+    j.hidx = f.hidx[f.ord];
+    j.hslot = f.hslot;
+    j.ffine = f.cur_fstart();
+    j.L = s->lists;
+    return j;
 }
 }  // namespace fsa
 
@@ -164,6 +195,10 @@ int check_params(const fsa_params &p) {
         (p.cell_number[0] < 3 || p.cell_number[1] < 3 || p.cell_number[2] < 3))
         return set_err(FSA_ERR_INVALID, "the packed form needs at least 3 cells along every axis");
     if (!(p.impact_distance_square > 0.0f)) return set_err(FSA_ERR_INVALID, "impact_distance_square must be positive");
+    // the list build reads the 5x5 cells around a particle's: a reach of at
+    // most two cut-offs (two cells)
+    if (!(p.neighbour_skin >= 0.0f && p.neighbour_skin <= 1.0f))
+        return set_err(FSA_ERR_INVALID, "neighbour_skin must be in [0, 1]");
     // the 27-cell search needs cells at least one cut-off wide
     for (int a = 0; a < 3; ++a) {
         const float w = p.interval[a];
@@ -229,6 +264,36 @@ int alloc_kind(fsa_sim *s, KindBuffers &k, int n, bool with_ref, bool double_sta
     return FSA_OK;
 }
 
+// the home state and the neighbour lists of the fluid (fsa_pairs.hip)
+int alloc_lists(fsa_sim *s) {
+    KindBuffers &f = s->fluid;
+    NeighbourLists &L = s->lists;
+    const size_t n = (size_t)s->P.nf;
+    int rc;
+    for (int b = 0; b < 2; ++b) {
+        if ((rc = s->alloc(&f.hpos[b], n + kPosPad)) || (rc = s->alloc(&f.hvel[b], n + kPosPad)) ||
+            (rc = s->alloc(&f.hidx[b], n)))
+            return rc;
+        if (!s->dry) {
+            // the padding entries hold the sentinel partner
+            HIP_TRY(launch_fill_f4(f.hpos[b] + n, make_float4(kFarPos, kFarPos, kFarPos, s->P.rho0), kPosPad,
+                                   s->stream));
+            HIP_TRY(hipMemsetAsync(f.hvel[b] + n, 0, sizeof(float4) * kPosPad, s->stream));
+        }
+    }
+    L.nmove = (int)((n + 63) / 64);
+    if ((rc = s->alloc(&f.hslot, n)) || (rc = s->alloc(&L.ent, (size_t)(s->P.vrows + kListAhead) * n)) ||
+        (rc = s->alloc(&L.meta, n)) ||
+        (rc = s->alloc(&L.x0, n)) || (rc = s->alloc(&L.move, (size_t)L.nmove)) ||
+        (rc = s->alloc(&L.flag, (size_t)kFlagWords)))
+        return rc;
+    if (!s->dry) {
+        HIP_TRY(hipMemsetAsync(L.move, 0, sizeof(uint32_t) * (size_t)L.nmove, s->stream));
+        HIP_TRY(hipMemsetAsync(L.flag, 0, sizeof(uint32_t) * kFlagWords, s->stream));
+    }
+    return FSA_OK;
+}
+
 // every device buffer of a context (dry: only count the bytes)
 int alloc_all(fsa_sim *s) {
     int rc;
@@ -236,6 +301,7 @@ int alloc_all(fsa_sim *s) {
     // the next step's fluid cell build may run beside the solid sub-steps,
     // which read the fluid cells of the current order
     if ((rc = alloc_kind(s, s->fluid, s->P.nf, false, true, kSubZ, comp, packed, true))) return rc;
+    if (s->P.vlist && (rc = alloc_lists(s))) return rc;
     if ((rc = alloc_kind(s, s->solid, s->P.ns, true, true, 1, comp, false))) return rc;
     // walls never move: their one sort (at upload) takes the fluid's keys, so
     // that the fluid pass reads wall partners through z-windows as well
@@ -436,6 +502,14 @@ VirtArgs fsa::virt_args(const fsa_sim *s, const KindBuffers &f, int compute_norm
 
 FluidStepArgs fsa::fluid_args(const fsa_sim *s, const KindBuffers &f, const KindBuffers &so) {
     FluidStepArgs a{};
+    if (s->P.vlist) {
+        a.hpos = f.hpos[f.hst];
+        a.hvel = f.hvel[f.hst];
+        a.hpos_out = f.hpos[1 - f.hst];
+        a.hvel_out = f.hvel[1 - f.hst];
+        a.hslot = f.hslot;
+        a.L = s->lists;
+    }
     a.fpos = f.pos[f.st];
     a.fvel = f.vel[f.st];
     a.fcpos = f.cpos[f.st];
@@ -616,7 +690,18 @@ int fsa_upload_fluid(fsa_sim *s, const float *x, const float *u, const float *rh
         HIP_TRY(hipMemcpyAsync(k.vel[0], vel.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
         HIP_TRY(hipMemcpyAsync(k.id[0], id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s->stream));
         if ((rc = clear_compensation(s, k))) return rc;  // an uploaded state carries no rounding error
+        if (s->P.vlist) {
+            // any home order will do until the build below makes the sorted one home
+            k.hst = 0;
+            HIP_TRY(hipMemcpyAsync(k.hidx[0], id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s->stream));
+        }
         if ((rc = build_cells(s, k, false))) return rc;
+        if (s->P.vlist) {
+            // the counts start again and a build is due whatever the moves say
+            HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t *>(s->lists.flag), 0, kFlagWords, s->stream));
+            HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t *>(s->lists.flag) + kFlagForce, 1, 1, s->stream));
+            HIP_TRY(launch_list_build(list_job(s, k), s->stream));
+        }
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->uploaded[FSA_FLUID] = true;
@@ -881,6 +966,7 @@ int fsa_fill_cell(fsa_sim *s) {
         HIP_TRY(launch_sort_one(make_cell_job(s->P, *k, s->jumps, false), s->stream));
         flip(*k);
     }
+    if (s->P.vlist) HIP_TRY(launch_list_build(list_job(s, s->fluid), s->stream));
     stage_end(s, FSA_STAGE_CELL);
     return FSA_OK;
 }
@@ -923,6 +1009,7 @@ int fsa_fluid_calculation(fsa_sim *s) {
     a.counts = (s->debug & FSA_DEBUG_PAIR_SUMS) ? s->pair_counts : nullptr;
     HIP_TRY(launch_fluid(s->P, a, s->fluid_threads, s->stream));
     f.st = 1 - f.st;
+    if (s->P.vlist) f.hst = 1 - f.hst;
     stage_end(s, FSA_STAGE_FLUID);
     return FSA_OK;
 }
@@ -989,6 +1076,7 @@ static int step_overlapped(fsa_sim *s, int n_steps) {
             if ((rc = stage_open(s, FSA_STAGE_CELL, s->side))) return rc;
             HIP_TRY(launch_cells_one(make_cell_job(s->P, s->fluid, s->jumps, true), s->side));
             flip(s->fluid);
+            if (s->P.vlist) HIP_TRY(launch_list_build(list_job(s, s->fluid), s->side));
             stage_end(s, FSA_STAGE_CELL, s->side);
         }
         if (s->P.nv) {
@@ -1075,6 +1163,20 @@ int fsa_get_jump_count(fsa_sim *s, int64_t *count) {
     HIP_TRY(hipMemcpyAsync(&v, s->jumps, sizeof v, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     *count = (int64_t)v;
+    return FSA_OK;
+}
+
+int fsa_get_list_stats(fsa_sim *s, int64_t *builds, int64_t *unlisted, int64_t *entries, int64_t *walks) {
+    CHECK_SIM(s);
+    uint32_t v[kFlagWords] = {};
+    if (s->P.vlist) {
+        HIP_TRY(hipMemcpyAsync(v, s->lists.flag, sizeof v, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+    if (builds) *builds = (int64_t)v[kFlagBuilds];
+    if (unlisted) *unlisted = (int64_t)v[kFlagUnlisted];
+    if (entries) *entries = (int64_t)v[kFlagEntries];
+    if (walks) *walks = (int64_t)v[kFlagWalks];
     return FSA_OK;
 }
 

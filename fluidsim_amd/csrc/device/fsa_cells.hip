@@ -260,6 +260,11 @@ struct OpGatherImpl {
         if (j.pk_out) j.pk_out[d] = pack_partner(j.P, x, u, c);
         j.id_out[d] = my;
         j.cell_out[d] = c;
+        if (j.hidx_out) {
+            const int h = j.hidx_in[s];
+            j.hidx_out[d] = h;
+            j.hslot[h] = d;
+        }
     }
 };
 
@@ -339,6 +344,11 @@ CellJob make_cell_job(const DevParams &P, const KindBuffers &k, unsigned long lo
     j.tmp_id = k.tmp_id;
     j.tile_sums = k.tile_sums;
     j.jumps = jumps;
+    if (k.hslot) {
+        j.hidx_in = k.hidx[oo];
+        j.hidx_out = k.hidx[on];
+        j.hslot = k.hslot;
+    }
     return j;
 }
 

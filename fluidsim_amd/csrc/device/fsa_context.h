@@ -39,6 +39,7 @@ struct fsa_sim {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     fsa::KindBuffers fluid, solid, virt;
+    fsa::NeighbourLists lists;  // fluid neighbour lists (P.vlist)
     fsa::SolidExtra sx;
     fsa::VirtExtra vx;
     float4 *virt_x = nullptr;  // constant wall positions (x, y, z, 0)
@@ -116,6 +117,8 @@ int drop_pending_hash(fsa_sim *s);
 SolidArgs solid_args(const fsa_sim *s, const KindBuffers &f, const KindBuffers &so);
 VirtArgs virt_args(const fsa_sim *s, const KindBuffers &f, int compute_normals);
 FluidStepArgs fluid_args(const fsa_sim *s, const KindBuffers &f, const KindBuffers &so);  // rates off
+// the list-build job of the fluid buffers f (after a fluid sort)
+ListJob list_job(const fsa_sim *s, const KindBuffers &f);
 // ... and over the context's current ones
 inline SolidArgs solid_args(const fsa_sim *s) { return solid_args(s, s->fluid, s->solid); }
 inline VirtArgs virt_args(const fsa_sim *s, int compute_normals) { return virt_args(s, s->fluid, compute_normals); }

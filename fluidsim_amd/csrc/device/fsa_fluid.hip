@@ -78,9 +78,14 @@ __device__ __forceinline__ float exp2_hw(float x) { return __builtin_amdgcn_exp2
 // kPadD2 in the packed form): its k is exactly 0 and every other factor
 // finite, so its terms are exact zeros.  The self pair (d2 = 0) takes
 // 1/|dx| = rsq(FLT_MIN), finite, so its du and |dx| are 0 as well.
+// The term's factors: a += (dx, dy, dz) coef, rho' += kc dr.
+struct Term {
+    float coef, kc, dr;
+};
+
 template <int KIND>
-__device__ __forceinline__ void pair_term(const DevParams &P, const Self &me, float dx, float dy, float dz, float d2,
-                                          float ujx, float ujy, float ujz, float rhoj, float pj, Acc &acc) {
+__device__ __forceinline__ Term pair_factors(const DevParams &P, const Self &me, float dx, float dy, float dz,
+                                             float d2, float ujx, float ujy, float ujz, float rhoj, float pj) {
     const float kc = exp2_hw(fmaf(-d2, P.kexp2, P.log2kd));
     const float r = rsq(fmaxf(d2, 1.17549435e-38f));  // 1/|dx|, e = dx r
     const float inv_rsum = KIND == kFluidSolid ? me.half_inv_rho : rcp(me.rho + rhoj);
@@ -93,11 +98,37 @@ __device__ __forceinline__ void pair_term(const DevParams &P, const Self &me, fl
     if (KIND == kFluidVirt) ps = fmaxf(ps, 0.0f);
     float coef = fmaf(me.vis5 * udx, rcp(d2 + P.eps_vis), ps);
     coef *= kc;
-    acc.ax += dx * coef;
-    acc.ay += dy * coef;
-    acc.az += dz * coef;
     const float dpc = dpij * inv_rsum * P.cs_inv;
-    acc.drho += kc * (A * udx - dpc * (d2 * r));
+    return {coef, kc, A * udx - dpc * (d2 * r)};
+}
+
+template <int KIND>
+__device__ __forceinline__ void pair_term(const DevParams &P, const Self &me, float dx, float dy, float dz, float d2,
+                                          float ujx, float ujy, float ujz, float rhoj, float pj, Acc &acc) {
+    const Term t = pair_factors<KIND>(P, me, dx, dy, dz, d2, ujx, ujy, ujz, rhoj, pj);
+    acc.ax = fmaf(dx, t.coef, acc.ax);
+    acc.ay = fmaf(dy, t.coef, acc.ay);
+    acc.az = fmaf(dz, t.coef, acc.az);
+    acc.drho = fmaf(t.kc, t.dr, acc.drho);
+}
+
+// A listed fluid candidate: the pair term if it lies inside the cut-off,
+// else nothing -- its terms are selected away, so that a partner outside
+// the cut-off leaves the sums exactly as the walk, which never evaluates
+// it, leaves them (a non-finite one included).
+__device__ __forceinline__ void listed_pair(const DevParams &P, const Self &me, const float4 &pj, const float4 &vj,
+                                            Acc &acc, int &hits) {
+    const float dx = pj.x - me.x, dy = pj.y - me.y, dz = pj.z - me.z;
+    const float d2 = len2(dx, dy, dz);
+    const bool hit = d2 < P.R2;
+    const Term t = pair_factors<kFluidFluid>(P, me, dx, dy, dz, d2, vj.x, vj.y, vj.z, pj.w, vj.w);
+    const float ax = fmaf(dx, t.coef, acc.ax), ay = fmaf(dy, t.coef, acc.ay), az = fmaf(dz, t.coef, acc.az);
+    const float dr = fmaf(t.kc, t.dr, acc.drho);
+    acc.ax = hit ? ax : acc.ax;
+    acc.ay = hit ? ay : acc.ay;
+    acc.az = hit ? az : acc.az;
+    acc.drho = hit ? dr : acc.drho;
+    hits += hit ? 1 : 0;
 }
 
 // squared distance given to padding slots of the packed form: exp2 of its
@@ -154,8 +185,6 @@ constexpr int fluid_waves(int threads) { return threads > 128 ? 2 : 3; }
 // CU's 160 KiB (a lane more of either, and LDS allocation granules leave room for nine)
 constexpr int kListCap = 89;    // 16-bit list entries per lane
 constexpr int kListPad = 7;     // sentinel entries past the last one (the rounds and their prefetch read them)
-constexpr int kRunBits = 12;    // offset of an entry inside its run
-constexpr int kRunMax = 1 << kRunBits;
 constexpr int kColumns = 9;     // neighbour columns; column c's run starts in base slot c
 constexpr int kBaseSlots = 12;  // run bases per lane (4-bit ids): the columns, spares for long runs, the sentinel
 constexpr int kSentinelSlot = kBaseSlots - 1;  // holds the kind's particle count: the first padding slot
@@ -366,7 +395,7 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, const C
                                      const float4 *__restrict__ vel, const float4 *__restrict__ np,
                                      const PackedSrc &ps, const Lists<T> &L, Acc &acc, int &found,
                                      const int32_t *__restrict__ sorted_cells = nullptr,
-                                     const uint8_t *__restrict__ reach = nullptr) {
+                                     const uint8_t *__restrict__ reach = nullptr, bool active = true) {
     if (KIND == kFluidSolid && sorted_cells) {
         const int nyz = P.cn[1] * P.cn[2];
         const int lo = sorted_cells[0] / nyz, hi = sorted_cells[n - 1] / nyz;
@@ -388,7 +417,10 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, const C
     if (KIND == kFluidFluid) {
         bool any = false;
 #pragma unroll
-        for (int col = 0; col < kColumns; ++col) any |= cb[col] < ce[col];
+        for (int col = 0; col < kColumns; ++col) {
+            if (!active) ce[col] = cb[col];
+            any |= cb[col] < ce[col];
+        }
         if (__ballot(any) == 0) return;
     }
     int cnt = 0, spare = 0;
@@ -403,16 +435,69 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, const C
     evaluate<KIND, T, PACK, COUNT>(P, me, pos, vel, np, ps, L, cnt, acc, found);
 }
 
+// Fluid partners from the particle's neighbour list (home particle i): every
+// listed candidate is evaluated in list order (the walk order of the build)
+// and counts only inside the cut-off.  About nine in ten listed candidates
+// are partners, so this costs less than the walk's test-then-evaluate, and
+// each candidate's state is read once.  Rows past the lane's list read the
+// sentinel partner.  Three rounds are in flight: the entries of the round
+// after next, the partner state of the next, the current one's arithmetic.
+template <bool COUNT>
+__device__ __forceinline__ void list_pass(const DevParams &P, const Self &me, int i, int cnt,
+                                          const FluidStepArgs &a, Acc &acc, int &found) {
+    if (__ballot(cnt > 0) == 0) return;
+    const size_t nf = (size_t)P.nf;
+    const uint32_t sentinel = slot_bytes(P.nf);
+    const uint32_t *__restrict__ ent = a.L.ent + i;
+    uint32_t en[kPairBatch];
+    float4 pc[kPairBatch], vc[kPairBatch];
+#pragma unroll
+    for (int q = 0; q < kPairBatch; ++q) {
+        const uint32_t e = ent[(size_t)q * nf];
+        en[q] = ent[(size_t)(kPairBatch + q) * nf];
+        const uint32_t off = q < cnt ? e : sentinel;
+        pc[q] = at(a.hpos, off);
+        vc[q] = at(a.hvel, off);
+    }
+    int hits = 0;
+    for (int k = 0; __ballot(k < cnt) != 0; k += kPairBatch) {
+        const int kn = k + kPairBatch;
+        float4 pn[kPairBatch], vn[kPairBatch];
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) {
+            const uint32_t off = kn + q < cnt ? en[q] : sentinel;
+            pn[q] = at(a.hpos, off);
+            vn[q] = at(a.hvel, off);
+        }
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) en[q] = ent[(size_t)(kn + kPairBatch + q) * nf];
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) listed_pair(P, me, pc[q], vc[q], acc, hits);
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) {
+            pc[q] = pn[q];
+            vc[q] = vn[q];
+        }
+    }
+    if (COUNT) found += hits;
+}
+
+// With neighbour lists in use (P.vlist, never with PACK, and the step not a
+// walking one) one thread per home particle, else one per sorted slot;
+// outputs in sorted order go to slot o.  Whether lists are in use is decided
+// at run time, so that the fluid walk (for every particle, or for the
+// unlisted ones) has one call site: a second inlined copy of it spills
+// registers.  A walking step writes no home state: the next build makes
+// the sorted order home again.
 template <bool RATES, int T, bool PACK>
-__device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepArgs &a, int blk, int nblk) {
-    __shared__ uint16_t s_list[(kListCap + kListPad) * T];
-    __shared__ uint32_t s_base[kBaseSlots * T];
+__device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepArgs &a, const Lists<T> &list,
+                                           uint32_t *s_wave, int blk, int nblk) {
     const int i = xcd_tile(blk, nblk) * T + threadIdx.x;
     if (i >= P.nf) return;  // lists are lane-private: no barrier follows
-    const Lists<T> list{s_list + threadIdx.x, s_base + threadIdx.x};
+    const bool LIST = !PACK && P.vlist != 0 && a.L.flag[kFlagMode] != kModeWalk;
 
-    const float4 pi4 = a.fpos[i];
-    const float4 vi4 = a.fvel[i];
+    const float4 pi4 = (LIST ? a.hpos : a.fpos)[i];
+    const float4 vi4 = (LIST ? a.hvel : a.fvel)[i];
     Self me;
     me.x = pi4.x;
     me.y = pi4.y;
@@ -428,7 +513,8 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     me.vis5 = P.viscid ? 0.5f * P.visc10 : 0.0f;
     me.half_inv_rho = 0.5f * me.inv_rho;
     Acc acc{0.0f, 0.0f, 0.0f, 0.0f};
-    const int cell = a.fcell[i];
+    // (the cell the sort found: the hash took it from this same position)
+    const int cell = LIST ? find_cell(P, me.x, me.y, me.z) : a.fcell[i];
     me.i = i;
     me.cell = cell;
     me.fx = me.fy = me.fz = 0.0f;
@@ -446,13 +532,20 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     const Window w(P, cols, me.x, me.y, me.z);
     // (debug runs count each kind's partners: the neighbour sets the walk found)
     int found[3] = {0, 0, 0};
-    pass<kFluidFluid, T, PACK, RATES>(P, me, cols, w, P.nf, a.fstart, a.ffine, a.fpos, a.fvel, nullptr, fsrc, list,
-                                      acc, found[0]);
+    // listed particles test their lists; the others (all of them without
+    // lists) walk their cells, over the sorted state of this step (the same
+    // values as the home state)
+    const int meta = LIST ? a.L.meta[i] : -1;
+    if (!PACK && LIST) list_pass<RATES>(P, me, i, meta, a, acc, found[0]);
+    if (__ballot(meta < 0) != 0)
+        pass<kFluidFluid, T, PACK, RATES>(P, me, cols, w, P.nf, a.fstart, a.ffine, a.fpos, a.fvel, nullptr, fsrc,
+                                          list, acc, found[0], nullptr, nullptr, meta < 0);
     pass<kFluidSolid, T, false, RATES>(P, me, cols, w, P.ns, a.sstart, nullptr, a.spos, a.svel, a.snp, none, list,
                                        acc, found[1], a.scell);
     pass<kFluidVirt, T, false, RATES>(P, me, cols, w, P.nv, a.vstart, a.vfine, a.vpos, a.vvel, nullptr, none, list,
                                       acc, found[2], nullptr, a.vreach);
-    if (RATES) a.counts[i] = make_int4(found[0], found[1], found[2], 0);
+    const int o = LIST ? a.hslot[i] : i;
+    if (RATES) a.counts[o] = make_int4(found[0], found[1], found[2], 0);
 
     // fluidUpdate (fluid.cu:234-269): NaN scrub, explicit step with the old
     // velocity for the position, density clamp, EOS, domain clamp, speed cap
@@ -460,14 +553,14 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
           az = me.m2_inv_rho * acc.az;
     if (!(drho == drho)) drho = 0.0f;
     if (!(ax == ax && ay == ay && az == az)) ax = ay = az = 0.0f;
-    if (RATES) a.rates[i] = make_float4(ax, ay, az, drho);
+    if (RATES) a.rates[o] = make_float4(ax, ay, az, drho);
     const float dt = P.dt;
     float x, y, z, ux, uy, uz;
     float4 cp = make_float4(0.0f, 0.0f, 0.0f, 0.0f), cv = cp;
     if (a.fcpos) {
         // compensated form: x and u carry their rounding error (cp, cv)
-        cp = a.fcpos[i];
-        cv = a.fcvel[i];
+        cp = a.fcpos[o];
+        cv = a.fcvel[o];
         const float sx = me.ux * dt, sy = me.uy * dt, sz = me.uz * dt;
         x = kahan_add(me.x, sx, cp.x);
         y = kahan_add(me.y, sy, cp.y);
@@ -507,18 +600,39 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     // pair sums only): the scene has diverged
     if (!(__builtin_isfinite(ux) && __builtin_isfinite(uy) && __builtin_isfinite(uz)))
         atomicOr(a.health, (unsigned long long)FSA_HEALTH_FLUID_NONFINITE);
-    a.fpos_out[i] = make_float4(xc, yc, zc, rho);
-    a.fvel_out[i] = make_float4(ux, uy, uz, p);
+    const float4 xo = make_float4(xc, yc, zc, rho), uo = make_float4(ux, uy, uz, p);
+    a.fpos_out[o] = xo;
+    a.fvel_out[o] = uo;
     if (a.fcpos) {
-        a.fcpos_out[i] = cp;
-        a.fcvel_out[i] = cv;
+        a.fcpos_out[o] = cp;
+        a.fcvel_out[o] = cv;
+    }
+    if (LIST) {
+        a.hpos_out[i] = xo;
+        a.hvel_out[i] = uo;
+        // the wave's largest squared move since the last list build, for the
+        // next build decision (float bits of a nonnegative value or NaN:
+        // unsigned order); through the wave's first lane's base slot, which
+        // is free now.  Lane 0 is active whenever any lane of the wave is.
+        const float4 x0 = a.L.x0[i];
+        const uint32_t m2 = __float_as_uint(len2(xc - x0.x, yc - x0.y, zc - x0.z));
+        const bool lead = (threadIdx.x & 63) == 0;
+        if (lead) *s_wave = 0u;
+        __builtin_amdgcn_wave_barrier();
+        atomicMax(s_wave, m2);
+        __builtin_amdgcn_wave_barrier();
+        if (lead) a.L.move[i >> 6] = *s_wave;
     }
 }
 
 template <bool RATES, int T, bool PACK>
 struct OpFluid {
     static __device__ void run(const FluidJob &j, int blk, int nblk) {
-        fluid_body<RATES, T, PACK>(j.P, j.a, blk, nblk);
+        __shared__ uint16_t s_list[(kListCap + kListPad) * T];
+        __shared__ uint32_t s_base[kBaseSlots * T];
+        const Lists<T> list{s_list + threadIdx.x, s_base + threadIdx.x};
+        uint32_t *s_wave = s_base + (threadIdx.x & ~63u);  // base slot 0 of the wave's first lane
+        fluid_body<RATES, T, PACK>(j.P, j.a, list, s_wave, blk, nblk);
     }
 };
 

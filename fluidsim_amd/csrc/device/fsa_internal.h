@@ -48,6 +48,10 @@ constexpr float kFarPos = 1.0e15f;
 // slot byte offsets are 32-bit (slot * 16 B)
 constexpr int kMaxKindParticles = (1 << 28) - kPosPad;
 constexpr int kSolidLanes = 16;  // lanes serving one solid particle
+// 16-bit partner entries of the fluid pass (and of the neighbour lists):
+// offset inside a run of slots (kRunBits, high), run-base id (4 bits, low)
+constexpr int kRunBits = 12;
+constexpr int kRunMax = 1 << kRunBits;
 // Inside a cell, fluid and wall particles are sorted by an eighth-cell z
 // index (then by .dat index): their sort key is cell * kSubZ + sub.  The
 // per-cell start array every pass uses is taken from the per-key one, which
@@ -91,6 +95,11 @@ struct DevParams {
     int form;
     float ival[3];  // cell interval (the packed positions are in cells)
     float B_inv;    // 1 / B, B = rho0 c0^2 / gamma (packed pressure is p / B)
+    // fluid neighbour lists (vlist = 0: the fluid pass walks the cells)
+    int vlist;
+    int vrows;      // candidates one list holds
+    float vreach2;  // squared list reach ((1 + skin) R)^2, widened by a rounding margin
+    float vthr2;    // squared move since the last build that calls for a new one, (skin R / 2)^2 less a margin
 };
 
 DevParams make_dev_params(const fsa_params &p);
@@ -132,6 +141,57 @@ struct KindBuffers {
     float4 *cvel[2] = {nullptr, nullptr};
     // packed form (fluid): the FP16 partner records of the current order
     uint4 *pk = nullptr;
+    // neighbour lists (fluid): the fluid pass reads and writes the state in
+    // its home order -- the sorted order of the last list build -- in hpos /
+    // hvel (double-buffered by hst), and also writes it into the sorted
+    // buffers, which every other pass reads.  hidx holds the home index of
+    // each sorted slot (indexed by ord, moved by every sort), hslot the
+    // sorted slot of each home index (written by every sort).
+    float4 *hpos[2] = {nullptr, nullptr};
+    float4 *hvel[2] = {nullptr, nullptr};
+    int hst = 0;
+    int32_t *hidx[2] = {nullptr, nullptr};
+    int32_t *hslot = nullptr;
+};
+
+// Fluid neighbour lists.  A list holds the candidates within the list reach
+// in the walk order of its build (the 25 columns of the 5x5 cells around the
+// particle's, x-major, each column's z-window in slot order), each as the
+// byte offset of its home slot in the home state arrays.
+constexpr int kListAhead = 12;     // rows past the longest list the fluid pass may read (and discard)
+constexpr int kListRowsMax = 256;  // candidates a list holds at most
+enum ListFlag {
+    kFlagMode = 0,      // this step's ListMode (set by the decision, read by the build and fluid passes)
+    kFlagForce = 1,     // build at the next decision whatever the moves (set by the host)
+    kFlagBuilds = 2,    // builds since the context's last fluid upload
+    kFlagUnlisted = 3,  // particles the last build could not list
+    kFlagEntries = 4,   // candidates the last build listed (all particles)
+    kFlagAge = 5,       // steps the current lists have served
+    kFlagWalkLeft = 6,  // steps left to walk before the lists are tried again
+    kFlagWalks = 7,     // steps walked since the context's last fluid upload
+    kFlagRetry = 8,     // length of the next walking stretch
+    kFlagWords = 12
+};
+// What the fluid pass of a step does with lists: use the current ones,
+// build new ones first, or walk every particle's cells over the sorted order.
+enum ListMode { kModeList = 0, kModeBuild = 1, kModeWalk = 2 };
+// A list that has to be rebuilt after fewer than kListMinAge steps did not
+// pay for its build (a build costs about as much as the list saves in that
+// many steps at 4 M particles): the scene then walks its cells for
+// kListRetry steps before it builds again, twice as long after each failure
+// in a row, at most kListRetryMax.
+constexpr uint32_t kListMinAge = 8;
+constexpr uint32_t kListRetry = 64;
+constexpr uint32_t kListRetryMax = 1024;
+struct NeighbourLists {
+    uint32_t *ent = nullptr;  // entry k of home particle i at ent[k * nf + i]; vrows + kListAhead rows
+    int32_t *meta = nullptr;  // entries, or -1: not listed (the particle walks its cells)
+    float4 *x0 = nullptr;     // position at the last build
+    // per wavefront of the fluid pass: the largest squared move since the
+    // last build (float bits; the decision reduces and clears them)
+    uint32_t *move = nullptr;
+    int nmove = 0;
+    uint32_t *flag = nullptr;  // ListFlag words
 };
 
 struct SolidExtra {
@@ -207,6 +267,10 @@ struct CellJob {
     float4 *cpos_out, *cvel_out;
     uint4 *pk_out;  // FP16 partner records of the sorted state (null unless packed)
     int32_t *id_out, *cell_out;
+    // neighbour lists (else null): home index of each slot in and out, and
+    // the sorted slot of each home index
+    const int32_t *hidx_in;
+    int32_t *hidx_out, *hslot;
     int32_t *key, *rank, *count, *start, *fstart, *tmp_slot, *tmp_id, *tile_sums;
     unsigned long long *jumps;
 };
@@ -221,6 +285,28 @@ hipError_t launch_fill_i32(int32_t *p, int32_t v, int n, hipStream_t s);
 hipError_t launch_fill_f4(float4 *p, float4 v, int n, hipStream_t s);
 // per cell: 1 if a particle of the kind with starts `start` lies in one of its 27 cells
 hipError_t launch_reach(const DevParams &P, const int32_t *start, uint8_t *reach, hipStream_t s);
+
+// ---- neighbour lists (fsa_pairs.hip) -------------------------------------------
+// The two passes of a list build, run after every fluid sort: the decision
+// (one workgroup: a build when the host forced one or some particle has
+// moved more than half the skin), then, only if it says so, the new home
+// order (the sorted state copied into the home buffers the next fluid pass
+// reads, home index = sorted slot) and the lists.
+struct ListJob {
+    DevParams P;
+    const float4 *spos, *svel;  // sorted state of the current order
+    float4 *hpos, *hvel;        // home state the next fluid pass reads
+    int32_t *hidx;              // home index of each sorted slot of the current order
+    int32_t *hslot;             // sorted slot of each home index
+    const int32_t *ffine;       // per-key starts of the current order
+    NeighbourLists L;
+};
+enum ListOp { kLDecide, kLBuild };
+hipError_t launch_list_one(ListOp op, const ListJob &j, hipStream_t s);
+hipError_t launch_many_list(ListOp op, const ListJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s);
+int blocks_list(ListOp op, const ListJob &j);
+// both passes for one scene
+hipError_t launch_list_build(const ListJob &j, hipStream_t s);
 
 // ---- fluid pass (fsa_fluid.hip) -----------------------------------------------
 struct FluidStepArgs {
@@ -239,6 +325,14 @@ struct FluidStepArgs {
     const int32_t *scell;   // the solid's cells in slot order (null without solid particles)
     const uint8_t *vreach;  // per cell: walls within its 27 cells (null without walls)
     unsigned long long *health;  // FSA_HEALTH_* bits of the scene (an updated velocity not finite)
+    // neighbour lists (P.vlist): one thread per home particle, which reads
+    // its state from hpos/hvel, writes the update to hpos_out/hvel_out and to
+    // the sorted buffers at hslot (fpos_out, fvel_out, the compensation
+    // terms, rates and counts, all in sorted order)
+    const float4 *hpos, *hvel;
+    float4 *hpos_out, *hvel_out;
+    const int32_t *hslot;
+    NeighbourLists L;
 };
 struct FluidJob {
     DevParams P;

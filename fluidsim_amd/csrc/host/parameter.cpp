@@ -95,12 +95,17 @@ bool Kernel::readParameter(const Config &config) {
     impact_length_hsml_ratio = config.Read("impact_length_hsml_ratio", 3.0f);
     thread_num = config.Read("thread_num", 32);
     precision_form = config.Read("precision_form", 0);
+    neighbour_skin = config.Read("neighbour_skin", kDefaultNeighbourSkin);
     if (smoothing_length == 0.0f || particle_diameter == 0.0f) {
         std::cout << "failed to read kernel information" << std::endl;
         return true;
     }
     if (precision_form < 0 || precision_form > (FSA_FORM_PACKED | FSA_FORM_COMPENSATED)) {
         std::cout << "precision_form must be 0..3" << std::endl;
+        return true;
+    }
+    if (!(neighbour_skin >= 0.0f && neighbour_skin <= 1.0f)) {
+        std::cout << "neighbour_skin must be in [0, 1]" << std::endl;
         return true;
     }
     return false;
@@ -315,6 +320,7 @@ fsa_params Parameter::toDevice() const {
     p.kernel_differential = kernel.kernel_differential;
     p.thread_num = kernel.thread_num;
     p.precision_form = kernel.precision_form;
+    p.neighbour_skin = kernel.neighbour_skin;
     return p;
 }
 

@@ -17,6 +17,9 @@ namespace fsa_host {
 // include/global.h:4-10
 constexpr float kPi = 3.141593f;
 constexpr int kVirtLayer = 8;
+// skin of the fluid neighbour lists when the ini has no neighbour_skin key
+// (a fraction of the cut-off)
+constexpr float kDefaultNeighbourSkin = 0.05f;
 
 struct Physics {
     float gravity[3] = {0.0f, 0.0f, 0.0f};
@@ -58,6 +61,9 @@ struct Kernel {
     float smoothing_length = 0, particle_diameter = 0, particle_volume = 0, diameter_square = 0;
     int thread_num = 32;
     int precision_form = 0;  // FSA_FORM_* bits (optional key, not in the reference)
+    // fluid neighbour lists with this skin (fraction of the cut-off; optional
+    // key, not in the reference; 0 = walk the cells every step)
+    float neighbour_skin = kDefaultNeighbourSkin;
     float smoothing_length_square = 0, smoothing_length_inverse = 0, smoothing_length_inverse_square = 0;
     float impact_length_hsml_ratio = 0, impact_distance_square = 0;
     float kernel_coefficient_1 = 0, kernel_coefficient_2 = 0, kernel_differential = 0;

@@ -63,6 +63,7 @@ class DamBreak:
     iend: int = 1
     result_interval: int = 1000000
     precision_form: int = 0           # FSA_FORM_* bits (optional ini key)
+    neighbour_skin: float | None = None  # fluid neighbour-list skin (optional ini key; None: the library default)
 
     # ---- derived --------------------------------------------------------
     @property
@@ -106,6 +107,8 @@ class DamBreak:
         }
         if self.precision_form:
             keys["precision_form"] = self.precision_form
+        if self.neighbour_skin is not None:
+            keys["neighbour_skin"] = self.neighbour_skin
         if extra:
             keys.update(extra)
         sections = {

@@ -160,6 +160,14 @@ class DeviceSim:
         check(self.lib.fsa_get_jump_count(self.handle, C.byref(v)))
         return int(v.value)
 
+    def list_stats(self) -> dict:
+        """Neighbour-list builds since the last fluid upload; the particles the
+        last build could not list, the candidates it listed, and the steps
+        walked without lists (fsa_get_list_stats)."""
+        v = [C.c_int64(0) for _ in range(4)]
+        check(self.lib.fsa_get_list_stats(self.handle, *[C.byref(x) for x in v]))
+        return dict(zip(("builds", "unlisted", "entries", "walks"), (x.value for x in v)))
+
     def health(self) -> int:
         """FSA_HEALTH_* bits: a non-finite velocity was written since the last upload."""
         v = C.c_uint()

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FSA_ABI_VERSION 2
+#define FSA_ABI_VERSION 3
 
 enum fsa_status {
     FSA_OK = 0,
@@ -121,6 +121,16 @@ typedef struct fsa_params {
      * `precision_form` (no counterpart in the reference; 0 = its plain FP32
      * arithmetic). */
     int32_t precision_form;
+
+    /* Neighbour lists: the fluid pass tests each fluid particle's fluid
+     * partners from a list of the candidates within (1 + neighbour_skin) R
+     * (R the cut-off), built from the cell lists and rebuilt on the device
+     * once some particle has moved more than neighbour_skin R / 2 since the
+     * last build; the exact cut-off test still runs every step, so every
+     * step finds the same partner set as a walk of the 27 cells.  From the
+     * optional ini key `neighbour_skin` (no counterpart in the reference;
+     * 0 = walk the cells every step).  Not used with FSA_FORM_PACKED. */
+    float neighbour_skin;
 } fsa_params;
 
 /* The fluid pass gathers its fluid partners' attributes from 16-byte FP16
@@ -222,6 +232,18 @@ int fsa_download_pair_counts(fsa_sim *sim, int32_t *counts);
  * context was created (the reference silently dropped such particles from
  * its lists, src/pair.cu:87-150; this build keeps them). */
 int fsa_get_jump_count(fsa_sim *sim, int64_t *count);
+/* Neighbour lists (fsa_params.neighbour_skin > 0): how many times the
+ * lists were built since the last fluid upload (the upload builds them
+ * once); how many fluid particles the last build could not list (more
+ * candidates than a list holds: those walk their cells every step until the
+ * next build); the candidates the last build listed over all particles;
+ * and the steps in which every particle walked its cells instead, because
+ * the lists did not last long enough to pay for their builds (a fast flow:
+ * after a stretch of such steps -- 64, doubled each time the lists fail
+ * again, at most 1024 -- the lists are built and tried again).  Any pointer
+ * may be NULL.  All zero without lists.  Reading syncs the context's
+ * stream. */
+int fsa_get_list_stats(fsa_sim *sim, int64_t *builds, int64_t *unlisted, int64_t *entries, int64_t *walks);
 /* Divergence guard (no counterpart in the reference, which only zeroes NaN
  * pair sums, src/fluid.cu:246-249): the fluid and solid updates set a bit
  * when a velocity they write is not finite, which no later clamp repairs

@@ -48,7 +48,7 @@ def test_params_struct_layout_matches_the_header():
 
 def test_abi_version_and_error_paths_without_a_device():
     lib = _native.device_lib()
-    assert lib.fsa_abi_version() == 2
+    assert lib.fsa_abi_version() == 3
     n = C.c_int(-1)
     assert lib.fsa_device_count(C.byref(n)) == 0
     if n.value > 0:
@@ -86,10 +86,11 @@ def test_footprint_matches_a_hand_count_of_the_buffers():
     + 16 counts x 4 B) = 2176 B) and the step-start position (16 B) = 2560 B;
     per wall particle one
     position/velocity buffer (32 B), id and cell double buffers (16 B), sort
-    scratch (16 B), normal and constant position (32 B) = 96 B."""
+    scratch (16 B), normal and constant position (32 B) = 96 B.  (Without
+    neighbour lists; their buffers are counted in the next test.)"""
     from fluidsim_amd import scenes
     from fluidsim_amd.sim import footprint
-    sc = scenes.tiny_scene(seed=1)
+    sc = scenes.tiny_scene(seed=1).with_(neighbour_skin=0.0)
     base = sc.params((1000, 100, 2000))
     r0, p0 = footprint(base)
     r1, _ = footprint(sc.params((1500, 100, 2000)))
@@ -99,3 +100,27 @@ def test_footprint_matches_a_hand_count_of_the_buffers():
     assert r2 - r0 == 50 * 2560
     assert r3 - r0 == 400 * 96
     assert p0 - r0 == 2000 * 32 and p3 - r3 == 2400 * 32
+
+
+def test_footprint_counts_the_neighbour_lists():
+    """With neighbour lists a fluid particle also holds the home-order
+    position + velocity double buffers (64 B), home index double buffer and
+    sorted slot (12 B), its list -- (rows + 12) 32-bit entries and the
+    entry count (4 B) -- and its build position (16 B), plus one 4-byte move
+    word per 64 particles.  rows is 1.5 x the lattice sites within the list
+    reach (1 + skin) R, rounded up to a multiple of 8, at most 256."""
+    import math
+    from fluidsim_amd import scenes
+    from fluidsim_amd.sim import footprint
+    skin = 0.15
+    sc = scenes.tiny_scene(seed=1).with_(neighbour_skin=skin)
+    p0, p1 = sc.params((1000, 100, 2000)), sc.params((1640, 100, 2000))
+    r0, _ = footprint(p0)
+    r1, _ = footprint(p1)
+    reach = (1 + skin) * math.sqrt(p0.impact_distance_square) / p0.particle_diameter
+    rows = min(max(math.ceil(1.5 * 4 / 3 * math.pi * reach ** 3 / 8) * 8, 32), 256)
+    per = 96 + 64 + 12 + 4 * (rows + 12) + 4 + 16
+    assert r1 - r0 == 640 * per + 4 * (math.ceil(1640 / 64) - math.ceil(1000 / 64))
+    # without a skin, or with the packed form, there are no lists
+    assert footprint(sc.with_(neighbour_skin=0.0).params((1000, 100, 2000)))[0] < r0
+    assert footprint(sc.with_(precision_form=_native.FORM_PACKED).params((1000, 100, 2000)))[0] < r0

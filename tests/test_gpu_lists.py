@@ -1,0 +1,200 @@
+"""Fluid neighbour lists (fsa_params.neighbour_skin, fsa_pairs.hip).
+
+The fluid pass tests each fluid particle's listed candidates against the
+cut-off every step; a list holds the candidates within (1 + skin) R of the
+positions at its build and is rebuilt on the device once a particle has
+moved more than skin R / 2.  What must hold:
+
+  * the partner set of every fluid particle at every step is the walk's,
+    i.e. the oracle's all-pairs scan on the same positions, bit for bit --
+    also on the last step before a rebuild, where the lists are stalest
+    (fluid thrown in random directions, so builds come every ten steps or
+    so), when the scene walks because its lists would not last (a thin
+    skin and fast fluid), and with a skin too thick for the lists;
+  * particles with more candidates than a list holds walk their cells, and
+    their partner sets are the oracle's as well;
+  * only the order of the partner sums changes: the state stays within the
+    1e-5 tolerances of the walk's over 100 steps of the 43,750-particle
+    scene;
+  * a batch of scenes with and without lists gives each scene the bits of
+    stepping it alone.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from fluidsim_amd import scenes
+from fluidsim_amd._native import DEBUG_PAIR_SUMS, FORM_COMPENSATED
+from fluidsim_amd.sim import DeviceBatch, DeviceSim
+
+from helpers import REL, assert_within, compare_fluid, max_abs
+
+pytestmark = pytest.mark.gpu
+
+THREADS = 16
+
+
+def _counts(a):
+    return len(a["fluid_position"]), len(a["solid_position"]), len(a["virt_position"])
+
+
+def _thrown(sc, seed, frac=0.95):
+    """The scene's particles with every fluid velocity drawn uniformly in
+    direction and up to frac x the speed cap 0.3 c0 in magnitude."""
+    a = sc.arrays()
+    rng = np.random.default_rng(seed)
+    n = len(a["fluid_position"])
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    speed = rng.uniform(0.0, frac * 0.3 * sc.sound_speed, size=(n, 1))
+    a["fluid_velocity"] = np.ascontiguousarray((d * speed).astype(np.float32))
+    return a
+
+
+def _oracle_counts(p, dev, a):
+    """The oracle's all-pairs partner counts on the device's current positions."""
+    b = dict(a)
+    f = dev.fluid()
+    b["fluid_position"], b["fluid_velocity"] = f["position"], f["velocity"]
+    if p.n_solid:
+        s = dev.solid()
+        b["solid_position"], b["solid_velocity"] = s["position"], s["velocity"]
+    o = orc.Oracle(p, 32)
+    o.upload(b)
+    o.particle_cell_update()
+    o.fill_cell()
+    want = o.pair_counts(brute=True)
+    o.close()
+    return want
+
+
+def _stage_step(dev):
+    dev.particle_cell_update()
+    dev.fill_cell()
+    dev.couple_calculation()
+    dev.virt_update()
+    dev.fluid_calculation()
+    dev.solid_calculation()
+
+
+@pytest.mark.parametrize("skin,speed,steps", [(0.15, 0.1, 40), (0.02, 0.95, 24), (0.6, 0.95, 16)],
+                         ids=["lists-age", "fast-walks", "thick-unlisted"])
+def test_partner_sets_are_the_walks_at_every_step(gpu_available, skin, speed, steps):
+    """lists-age: fluid thrown at up to 0.1 x the speed cap, so a list
+    serves several steps before a particle has moved half the skin: every
+    step of its life is checked, the last ones the stalest.  fast-walks: a thin skin
+    and fast fluid, so the lists would not pay for their builds and the
+    scene walks its cells.  thick-unlisted: a skin whose reach holds more
+    candidates than a list takes, so most particles walk their cells."""
+    orc.set_threads(THREADS)
+    sc = scenes.tiny_scene(seed=31, fluid=(14, 10, 6), gate=(4, 8, 6), gate_x0=9).with_(neighbour_skin=skin)
+    a = _thrown(sc, 31, speed)
+    p = sc.params(_counts(a))
+    print(f"case thrown fluid, skin {skin}, speed {speed} x cap: {p.n_fluid} fluid", flush=True)
+    dev = DeviceSim(p)
+    dev.upload(a)
+    dev.set_debug(DEBUG_PAIR_SUMS)
+    for k in range(steps):
+        want = _oracle_counts(p, dev, a)
+        _stage_step(dev)
+        got = dev.pair_counts()
+        bad = np.flatnonzero((got != want).any(axis=1))
+        assert bad.size == 0, (f"step {k}: {bad.size} fluid particles differ, first {bad[:5]}: "
+                               f"gpu {got[bad[:5]].tolist()} oracle {want[bad[:5]].tolist()}")
+    st = dev.list_stats()
+    print(f"skin {skin}: {st} over {steps} steps", flush=True)
+    if skin == 0.15:
+        # one build at the upload, then one per list life of several steps
+        assert st["builds"] >= 3 and st["walks"] <= steps // 2 and st["unlisted"] == 0
+    elif skin == 0.02:
+        assert st["walks"] >= steps // 2 and st["unlisted"] == 0
+    else:
+        # a list holds at most 256 candidates: with the thick skin (about 300
+        # lattice sites within reach) most particles walk their cells
+        assert st["unlisted"] > 0
+    dev.close()
+
+
+def test_unlisted_particles_walk_their_cells(gpu_available):
+    """A clump three times denser than the lattice overflows its particles'
+    lists; they walk their cells while the rest use their lists."""
+    orc.set_threads(THREADS)
+    sc = scenes.tiny_scene(seed=33, fluid=(16, 12, 10), tank=(20, 14, 12), gate=None)
+    a = sc.arrays()
+    x = a["fluid_position"]
+    lo, hi = x.min(axis=0), x.max(axis=0)
+    c = (lo + hi) / 2
+    g = np.arange(-2, 3, dtype=np.float32) * np.float32(sc.dp / 3)
+    clump = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) + c + np.float32(sc.dp / 7)
+    a["fluid_position"] = np.ascontiguousarray(np.concatenate([x, clump]).astype(np.float32))
+    n = len(a["fluid_position"])
+    a["fluid_velocity"] = np.zeros((n, 3), np.float32)
+    a["fluid_density"] = np.full(n, sc.fluid_density, np.float32)
+    a["fluid_pressure"] = np.zeros(n, np.float32)
+    p = sc.params(_counts(a))
+    dev = DeviceSim(p)
+    dev.upload(a)
+    st = dev.list_stats()
+    print(f"case clump: {n} fluid, {st['unlisted']} unlisted", flush=True)
+    assert 0 < st["unlisted"] < n // 2
+    dev.set_debug(DEBUG_PAIR_SUMS)
+    for k in range(2):
+        want = _oracle_counts(p, dev, a)
+        _stage_step(dev)
+        got = dev.pair_counts()
+        bad = np.flatnonzero((got != want).any(axis=1))
+        assert bad.size == 0, f"step {k}: {bad.size} differ, first gpu {got[bad[:3]].tolist()} oracle {want[bad[:3]].tolist()}"
+    dev.close()
+
+
+def test_lists_change_only_the_summation_order(gpu_available):
+    """100 steps of the 43,750-particle scene with lists (the library's
+    default skin) and with the walk."""
+    sc = scenes.c1_cpu_scene(seed=2)
+    a = sc.arrays()
+    c = _counts(a)
+    runs = {}
+    for skin in (None, 0.0):
+        print(f"case c1 100 steps, skin {skin}", flush=True)
+        d = DeviceSim(sc.with_(neighbour_skin=skin).params(c))
+        d.upload(a)
+        d.step(100)
+        runs[skin] = (d.fluid(), d.solid(), d.list_stats())
+        d.close()
+    (f1, s1, st), (f0, s0, _) = runs[None], runs[0.0]
+    print(f"lists: {st}", flush=True)
+    assert st["builds"] >= 1 and st["unlisted"] == 0
+    assert_within(compare_fluid(sc, f1, f0), "lists against the walk")
+    L = max(sc.tank) * sc.dp
+    assert max_abs(s1["position"], s0["position"]) <= REL * L
+
+
+def test_batch_with_and_without_lists_is_bit_identical(gpu_available):
+    base = scenes.tiny_scene(seed=35)
+    cases = [base, base.with_(seed=36, neighbour_skin=0.0),
+             base.with_(seed=37, neighbour_skin=0.4, precision_form=FORM_COMPENSATED, solid_sub_step=3)]
+
+    def make(sc, thrown):
+        a = _thrown(sc, sc.seed, 0.5) if thrown else sc.arrays()
+        d = DeviceSim(sc.params(_counts(a)))
+        d.upload(a)
+        return d
+
+    for thrown in (False, True):
+        print(f"case batch lists/no lists, thrown {thrown}", flush=True)
+        alone = [make(sc, thrown) for sc in cases]
+        mixed = [make(sc, thrown) for sc in cases]
+        for d in alone:
+            d.step(9)
+        with DeviceBatch(graphs=True, streams=2) as b:
+            b.step(mixed, 9)
+            b.synchronize()
+        for k, (x, y) in enumerate(zip(alone, mixed)):
+            fx, fy = x.fluid(), y.fluid()
+            for key in fx:
+                assert np.array_equal(fx[key].view(np.uint32), fy[key].view(np.uint32)), (k, key)
+            if x.ns:
+                assert np.array_equal(x.solid()["position"].view(np.uint32), y.solid()["position"].view(np.uint32))
+            assert x.list_stats() == y.list_stats()
+        for d in alone + mixed:
+            d.close()
