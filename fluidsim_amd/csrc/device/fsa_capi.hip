@@ -154,6 +154,10 @@ DevParams make_dev_params(const fsa_params &p) {
         const double dp = p.particle_diameter > 0.0f ? (double)p.particle_diameter : R / 2.6;
         const double sites = 4.0 / 3.0 * 3.141592653589793 * std::pow(reach / dp, 3.0);
         d.vrows = std::min(std::max((int)std::ceil(1.5 * sites / 8.0) * 8, 32), kListRowsMax);
+        // walls: a fluid particle at a flat wall sees about a third of the
+        // sites within the reach in wall layers, at a corner two thirds;
+        // corner particles with more wall candidates walk the wall cells
+        d.vwrows = p.n_virt > 0 ? std::max(d.vrows / 2 / 8 * 8, 32) : 0;
     }
     return d;
 }
@@ -168,6 +172,8 @@ ListJob list_job(const fsa_sim *s, const KindBuffers &f) {
     j.hidx = f.hidx[f.ord];
     j.hslot = f.hslot;
     j.ffine = f.cur_fstart();
+    j.vpos = s->virt.pos[0];
+    j.vfine = s->virt.cur_fstart();
     j.L = s->lists;
     return j;
 }
@@ -284,6 +290,8 @@ int alloc_lists(fsa_sim *s) {
     L.nmove = (int)((n + 63) / 64);
     if ((rc = s->alloc(&f.hslot, n)) || (rc = s->alloc(&L.ent, (size_t)(s->P.vrows + kListAhead) * n)) ||
         (rc = s->alloc(&L.meta, n)) ||
+        (s->P.vwrows > 0 && ((rc = s->alloc(&L.went, (size_t)(s->P.vwrows + kListAhead) * n)) ||
+                             (rc = s->alloc(&L.wmeta, n)))) ||
         (rc = s->alloc(&L.x0, n)) || (rc = s->alloc(&L.move, (size_t)L.nmove)) ||
         (rc = s->alloc(&L.flag, (size_t)kFlagWords)))
         return rc;
@@ -808,6 +816,9 @@ int fsa_upload_virt(fsa_sim *s, const float *x, const int32_t *type) {
         if (err) return err;
         (void)type;  // FIXED_VIRT only; MOVING_VIRT is declared but unused by the reference
         s->vx.normals_ready = false;
+        // the fluid's neighbour lists hold wall candidates: the next step builds them again
+        if (s->P.vlist)
+            HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t *>(s->lists.flag) + kFlagForce, 1, 1, s->stream));
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->uploaded[FSA_VIRT] = true;

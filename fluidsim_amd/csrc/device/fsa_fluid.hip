@@ -112,16 +112,17 @@ __device__ __forceinline__ void pair_term(const DevParams &P, const Self &me, fl
     acc.drho = fmaf(t.kc, t.dr, acc.drho);
 }
 
-// A listed fluid candidate: the pair term if it lies inside the cut-off,
-// else nothing -- its terms are selected away, so that a partner outside
-// the cut-off leaves the sums exactly as the walk, which never evaluates
-// it, leaves them (a non-finite one included).
+// A listed fluid or wall candidate: the pair term if it lies inside the
+// cut-off, else nothing -- its terms are selected away, so that a partner
+// outside the cut-off leaves the sums exactly as the walk, which never
+// evaluates it, leaves them (a non-finite one included).
+template <int KIND>
 __device__ __forceinline__ void listed_pair(const DevParams &P, const Self &me, const float4 &pj, const float4 &vj,
                                             Acc &acc, int &hits) {
     const float dx = pj.x - me.x, dy = pj.y - me.y, dz = pj.z - me.z;
     const float d2 = len2(dx, dy, dz);
     const bool hit = d2 < P.R2;
-    const Term t = pair_factors<kFluidFluid>(P, me, dx, dy, dz, d2, vj.x, vj.y, vj.z, pj.w, vj.w);
+    const Term t = pair_factors<KIND>(P, me, dx, dy, dz, d2, vj.x, vj.y, vj.z, pj.w, vj.w);
     const float ax = fmaf(dx, t.coef, acc.ax), ay = fmaf(dy, t.coef, acc.ay), az = fmaf(dz, t.coef, acc.az);
     const float dr = fmaf(t.kc, t.dr, acc.drho);
     acc.ax = hit ? ax : acc.ax;
@@ -401,20 +402,21 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, const C
         const int lo = sorted_cells[0] / nyz, hi = sorted_cells[n - 1] / nyz;
         if (__ballot(cols.cx + 1 >= lo && cols.cx - 1 <= hi) == 0) return;
     }
-    if (KIND == kFluidVirt && reach && __ballot(reach[me.cell] != 0) == 0) return;
+    if (KIND == kFluidVirt && reach && __ballot(active && reach[me.cell] != 0) == 0) return;
     int cb[kColumns], ce[kColumns];
     if (KIND != kFluidFluid) {
         column_ranges(cols, start, cb, ce);
         bool any = false;
 #pragma unroll
         for (int col = 0; col < kColumns; ++col) any |= cb[col] < ce[col];
-        if (__ballot(any) == 0) return;
+        if (__ballot(active && any) == 0) return;
     }
     if (KIND != kFluidSolid) {
 #pragma unroll
         for (int col = 0; col < kColumns; ++col) z_window(P, cols, fine, col, w, P.R2w, cb[col], ce[col]);
     }
-    if (KIND == kFluidFluid) {
+    {
+        // (lanes that are not active, e.g. listed particles, take no part)
         bool any = false;
 #pragma unroll
         for (int col = 0; col < kColumns; ++col) {
@@ -435,20 +437,22 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, const C
     evaluate<KIND, T, PACK, COUNT>(P, me, pos, vel, np, ps, L, cnt, acc, found);
 }
 
-// Fluid partners from the particle's neighbour list (home particle i): every
-// listed candidate is evaluated in list order (the walk order of the build)
-// and counts only inside the cut-off.  About nine in ten listed candidates
-// are partners, so this costs less than the walk's test-then-evaluate, and
-// each candidate's state is read once.  Rows past the lane's list read the
+// Fluid (or wall) partners from the particle's neighbour list (home
+// particle i; n particles of the kind in pos, vel): every listed candidate
+// is evaluated in list order (the walk order of the build) and counts only
+// inside the cut-off.  About nine in ten listed fluid candidates are
+// partners, so this costs less than the walk's test-then-evaluate, and each
+// candidate's state is read once.  Rows past the lane's list read the
 // sentinel partner.  Three rounds are in flight: the entries of the round
 // after next, the partner state of the next, the current one's arithmetic.
-template <bool COUNT>
+template <int KIND, bool COUNT>
 __device__ __forceinline__ void list_pass(const DevParams &P, const Self &me, int i, int cnt,
-                                          const FluidStepArgs &a, Acc &acc, int &found) {
+                                          const uint32_t *__restrict__ list, const float4 *__restrict__ pos,
+                                          const float4 *__restrict__ vel, int n, Acc &acc, int &found) {
     if (__ballot(cnt > 0) == 0) return;
     const size_t nf = (size_t)P.nf;
-    const uint32_t sentinel = slot_bytes(P.nf);
-    const uint32_t *__restrict__ ent = a.L.ent + i;
+    const uint32_t sentinel = slot_bytes(n);
+    const uint32_t *__restrict__ ent = list + i;
     uint32_t en[kPairBatch];
     float4 pc[kPairBatch], vc[kPairBatch];
 #pragma unroll
@@ -456,8 +460,8 @@ __device__ __forceinline__ void list_pass(const DevParams &P, const Self &me, in
         const uint32_t e = ent[(size_t)q * nf];
         en[q] = ent[(size_t)(kPairBatch + q) * nf];
         const uint32_t off = q < cnt ? e : sentinel;
-        pc[q] = at(a.hpos, off);
-        vc[q] = at(a.hvel, off);
+        pc[q] = at(pos, off);
+        vc[q] = at(vel, off);
     }
     int hits = 0;
     for (int k = 0; __ballot(k < cnt) != 0; k += kPairBatch) {
@@ -466,13 +470,13 @@ __device__ __forceinline__ void list_pass(const DevParams &P, const Self &me, in
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q) {
             const uint32_t off = kn + q < cnt ? en[q] : sentinel;
-            pn[q] = at(a.hpos, off);
-            vn[q] = at(a.hvel, off);
+            pn[q] = at(pos, off);
+            vn[q] = at(vel, off);
         }
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q) en[q] = ent[(size_t)(kn + kPairBatch + q) * nf];
 #pragma unroll
-        for (int q = 0; q < kPairBatch; ++q) listed_pair(P, me, pc[q], vc[q], acc, hits);
+        for (int q = 0; q < kPairBatch; ++q) listed_pair<KIND>(P, me, pc[q], vc[q], acc, hits);
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q) {
             pc[q] = pn[q];
@@ -536,14 +540,19 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     // lists) walk their cells, over the sorted state of this step (the same
     // values as the home state)
     const int meta = LIST ? a.L.meta[i] : -1;
-    if (!PACK && LIST) list_pass<RATES>(P, me, i, meta, a, acc, found[0]);
+    if (!PACK && LIST) list_pass<kFluidFluid, RATES>(P, me, i, meta, a.L.ent, a.hpos, a.hvel, P.nf, acc, found[0]);
     if (__ballot(meta < 0) != 0)
         pass<kFluidFluid, T, PACK, RATES>(P, me, cols, w, P.nf, a.fstart, a.ffine, a.fpos, a.fvel, nullptr, fsrc,
                                           list, acc, found[0], nullptr, nullptr, meta < 0);
     pass<kFluidSolid, T, false, RATES>(P, me, cols, w, P.ns, a.sstart, nullptr, a.spos, a.svel, a.snp, none, list,
                                        acc, found[1], a.scell);
-    pass<kFluidVirt, T, false, RATES>(P, me, cols, w, P.nv, a.vstart, a.vfine, a.vpos, a.vvel, nullptr, none, list,
-                                      acc, found[2], nullptr, a.vreach);
+    // wall partners likewise (their lists are built with the fluid's)
+    const int wmeta = LIST && a.L.went ? a.L.wmeta[i] : -1;
+    if (!PACK && LIST && a.L.went)
+        list_pass<kFluidVirt, RATES>(P, me, i, wmeta, a.L.went, a.vpos, a.vvel, P.nv, acc, found[2]);
+    if (__ballot(wmeta < 0) != 0)
+        pass<kFluidVirt, T, false, RATES>(P, me, cols, w, P.nv, a.vstart, a.vfine, a.vpos, a.vvel, nullptr, none,
+                                          list, acc, found[2], nullptr, a.vreach, wmeta < 0);
     const int o = LIST ? a.hslot[i] : i;
     if (RATES) a.counts[o] = make_int4(found[0], found[1], found[2], 0);
 

@@ -97,7 +97,8 @@ struct DevParams {
     float B_inv;    // 1 / B, B = rho0 c0^2 / gamma (packed pressure is p / B)
     // fluid neighbour lists (vlist = 0: the fluid pass walks the cells)
     int vlist;
-    int vrows;      // candidates one list holds
+    int vrows;      // fluid candidates one list holds
+    int vwrows;     // wall candidates one list holds
     float vreach2;  // squared list reach ((1 + skin) R)^2, widened by a rounding margin
     float vthr2;    // squared move since the last build that calls for a new one, (skin R / 2)^2 less a margin
 };
@@ -170,6 +171,7 @@ enum ListFlag {
     kFlagWalkLeft = 6,  // steps left to walk before the lists are tried again
     kFlagWalks = 7,     // steps walked since the context's last fluid upload
     kFlagRetry = 8,     // length of the next walking stretch
+    kFlagWallEntries = 9,  // wall candidates the last build listed (all particles)
     kFlagWords = 12
 };
 // What the fluid pass of a step does with lists: use the current ones,
@@ -186,6 +188,10 @@ constexpr uint32_t kListRetryMax = 1024;
 struct NeighbourLists {
     uint32_t *ent = nullptr;  // entry k of home particle i at ent[k * nf + i]; vrows + kListAhead rows
     int32_t *meta = nullptr;  // entries, or -1: not listed (the particle walks its cells)
+    // wall candidates (byte offsets of wall slots), the same layout with
+    // vwrows + kListAhead rows; wmeta -1: the particle walks the wall cells
+    uint32_t *went = nullptr;
+    int32_t *wmeta = nullptr;
     float4 *x0 = nullptr;     // position at the last build
     // per wavefront of the fluid pass: the largest squared move since the
     // last build (float bits; the decision reduces and clears them)
@@ -299,6 +305,8 @@ struct ListJob {
     int32_t *hidx;              // home index of each sorted slot of the current order
     int32_t *hslot;             // sorted slot of each home index
     const int32_t *ffine;       // per-key starts of the current order
+    const float4 *vpos;         // wall positions (sorted once, at their upload)
+    const int32_t *vfine;       // per-key starts of the walls
     NeighbourLists L;
 };
 enum ListOp { kLDecide, kLBuild };

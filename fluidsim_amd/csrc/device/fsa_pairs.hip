@@ -28,7 +28,9 @@
 //              per particle, the candidates within the list reach over the
 //              5x5 columns of cells around its own (the reach exceeds one
 //              cell), each column's z-window in slot order, as the byte
-//              offsets of their home slots.
+//              offsets of their home slots; and the wall particles within
+//              the reach, as the byte offsets of their slots (the walls
+//              never move, so they stay candidates as long as the fluid's).
 // The build pass leaves at once when the current lists serve or the step
 // walks.  A particle with more
 // candidates than a list holds is marked unlisted and walks its cells in the
@@ -100,6 +102,7 @@ struct OpListDecide {
             f[kFlagBuilds] += 1u;
             f[kFlagUnlisted] = 0u;
             f[kFlagEntries] = 0u;
+            f[kFlagWallEntries] = 0u;
             f[kFlagAge] = 0u;
         } else if (mode == kModeList) {
             f[kFlagAge] = age + 1u;
@@ -138,15 +141,75 @@ __device__ __forceinline__ void wide_window(const DevParams &P, const Columns &c
 constexpr int kBuildThreads = 256;
 constexpr int kBuildGrid = 2048;  // workgroups of one scene's build (grid-stride over its particles)
 
+// The candidates of one kind (positions pos, per-key starts fine) within
+// the list reach of xi, in the walk order of the 5x5 columns, written to
+// rows 0.. of ent as the byte offsets of their slots while fewer than
+// `rows`; returns how many there are.
+__device__ __forceinline__ int list_kind(const DevParams &P, const Columns &cols, const Window &w, int zlo, int zhi1,
+                                         bool valid, const float4 &xi, const float4 *__restrict__ pos,
+                                         const int32_t *__restrict__ fine, uint32_t *__restrict__ ent, size_t nf,
+                                         int i, int rows) {
+    int cnt = 0;
+#pragma unroll 1
+    for (int ox = -2; ox <= 2; ++ox) {
+        // the row's 5 windows first, their loads in flight together
+        int cb[5], cend[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            wide_window(P, cols, fine, ox, c - 2, zlo, zhi1, w, P.vreach2, cb[c], cend[c]);
+            if (!valid) cend[c] = cb[c];
+        }
+#pragma unroll 1
+        for (int col = 0; col < 5; ++col) {
+            const int b = cb[col], e = cend[col];
+            if (__ballot(b < e) == 0) continue;
+            // the next batch is loaded while this one is tested (loads at or
+            // past the window end stay in the padded array)
+            float4 q[kBuildBatch];
+            {
+                const float4 *src = pos + min(b, e);
+#pragma unroll
+                for (int k = 0; k < kBuildBatch; ++k) q[k] = src[k];
+            }
+#pragma unroll 1
+            for (int s0 = b; __ballot(s0 < e) != 0; s0 += kBuildBatch) {
+                const int ss = min(s0, e);
+                const int lim = e - ss;
+                float4 qn[kBuildBatch];
+                const float4 *nxt = pos + min(s0 + kBuildBatch, e);
+#pragma unroll
+                for (int k = 0; k < kBuildBatch; ++k) qn[k] = nxt[k];
+#pragma unroll
+                for (int k = 0; k < kBuildBatch; ++k) {
+                    const float d2 = len2(q[k].x - xi.x, q[k].y - xi.y, q[k].z - xi.z);
+                    if (k < lim && d2 < P.vreach2) {
+                        if (cnt < rows) ent[(size_t)cnt * nf + i] = (uint32_t)(ss + k) << 4;
+                        ++cnt;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kBuildBatch; ++k) q[k] = qn[k];
+            }
+        }
+    }
+    return cnt;
+}
+
+// wave sum of v (every lane of the wave active)
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += (uint32_t)__shfl_xor((int)v, d, 64);
+    return v;
+}
+
 struct OpListBuild {
     static __device__ void run(const ListJob &j, int blk, int nblk) {
         const uint32_t mode = j.L.flag[kFlagMode];
-        if (mode == kModeList) return;
+        if (mode != kModeBuild) return;  // the current lists serve, or the fluid pass walks
         const DevParams &P = j.P;
         const int lane = threadIdx.x;
         const size_t nf = (size_t)P.nf;
-        if (mode == kModeWalk) return;  // the fluid pass walks over the sorted order
-        uint32_t ce = 0;  // listed entries of this thread's particles
+        uint32_t ce = 0, cw = 0;  // listed fluid and wall entries of this thread's particles
         for (int base = blk * kBuildThreads; base < P.nf; base += nblk * kBuildThreads) {
             const int i = base + lane;
             const bool valid = i < P.nf;
@@ -167,57 +230,30 @@ struct OpListBuild {
             const Window w(P, cols, xi.x, xi.y, xi.z);
             const int cz = cell - cols.cbase;
             const int zlo = max(cz - 2, 0), zhi1 = min(cz + 2, cols.nz - 1) + 1;
-            // the 25 windows first, their loads in flight together
-            int cb[25], cend[25];
-#pragma unroll
-            for (int col = 0; col < 25; ++col) {
-                wide_window(P, cols, j.ffine, col / 5 - 2, col % 5 - 2, zlo, zhi1, w, P.vreach2, cb[col], cend[col]);
-                if (!valid) cend[col] = cb[col];
-            }
-            int cnt = 0;
+            // the fluid's candidates, then the walls' (the walls never move:
+            // their candidates stay valid as long as the fluid's do); one
+            // loop, so that the walk is inlined once
+            bool unlisted = false;
 #pragma unroll 1
-            for (int col = 0; col < 25; ++col) {
-                const int b = cb[col], e = cend[col];
-                if (__ballot(b < e) == 0) continue;
-                // the next batch is loaded while this one is tested (loads at
-                // or past the window end stay in the padded array)
-                float4 q[kBuildBatch];
-                {
-                    const float4 *src = j.spos + min(b, e);
-#pragma unroll
-                    for (int k = 0; k < kBuildBatch; ++k) q[k] = src[k];
-                }
-#pragma unroll 1
-                for (int s0 = b; __ballot(s0 < e) != 0; s0 += kBuildBatch) {
-                    const int ss = min(s0, e);
-                    const int lim = e - ss;
-                    float4 qn[kBuildBatch];
-                    const float4 *nxt = j.spos + min(s0 + kBuildBatch, e);
-#pragma unroll
-                    for (int k = 0; k < kBuildBatch; ++k) qn[k] = nxt[k];
-#pragma unroll
-                    for (int k = 0; k < kBuildBatch; ++k) {
-                        const float d2 = len2(q[k].x - xi.x, q[k].y - xi.y, q[k].z - xi.z);
-                        if (k < lim && d2 < P.vreach2) {
-                            if (cnt < P.vrows) j.L.ent[(size_t)cnt * nf + i] = (uint32_t)(ss + k) << 4;
-                            ++cnt;
-                        }
-                    }
-#pragma unroll
-                    for (int k = 0; k < kBuildBatch; ++k) q[k] = qn[k];
-                }
+            for (int kind = 0; kind < (j.L.went ? 2 : 1); ++kind) {
+                const bool wall = kind == 1;
+                const int rows = wall ? P.vwrows : P.vrows;
+                const int cnt = list_kind(P, cols, w, zlo, zhi1, valid, xi, wall ? j.vpos : j.spos,
+                                          wall ? j.vfine : j.ffine, wall ? j.L.went : j.L.ent, nf, i, rows);
+                const bool listed = valid && cnt <= rows;
+                unlisted |= valid && !listed;
+                if (valid) (wall ? j.L.wmeta : j.L.meta)[i] = listed ? cnt : -1;
+                (wall ? cw : ce) += listed ? (uint32_t)cnt : 0u;
             }
-            const bool listed = valid && cnt <= P.vrows;
-            if (valid) {
-                j.L.meta[i] = listed ? cnt : -1;
-                if (!listed) atomicAdd(&j.L.flag[kFlagUnlisted], 1u);
-            }
-            ce += listed ? (uint32_t)cnt : 0u;
+            if (unlisted) atomicAdd(&j.L.flag[kFlagUnlisted], 1u);
         }
-        // the wave's listed entries, one atomic (every lane is still active)
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) ce += (uint32_t)__shfl_xor((int)ce, d, 64);
-        if ((lane & 63) == 0 && ce > 0u) atomicAdd(&j.L.flag[kFlagEntries], ce);
+        // the wave's listed entries, one atomic each (every lane is still active)
+        ce = wave_sum(ce);
+        cw = wave_sum(cw);
+        if ((lane & 63) == 0) {
+            if (ce > 0u) atomicAdd(&j.L.flag[kFlagEntries], ce);
+            if (cw > 0u) atomicAdd(&j.L.flag[kFlagWallEntries], cw);
+        }
     }
 };
 

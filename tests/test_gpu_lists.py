@@ -107,7 +107,9 @@ def test_partner_sets_are_the_walks_at_every_step(gpu_available, skin, speed, st
         # one build at the upload, then one per list life of several steps
         assert st["builds"] >= 3 and st["walks"] <= steps // 2 and st["unlisted"] == 0
     elif skin == 0.02:
-        assert st["walks"] >= steps // 2 and st["unlisted"] == 0
+        # (particles in the tank's corners may have more wall candidates than
+        # a list holds at this skin: they walk the wall cells)
+        assert st["walks"] >= steps // 2
     else:
         # a list holds at most 256 candidates: with the thick skin (about 300
         # lattice sites within reach) most particles walk their cells
