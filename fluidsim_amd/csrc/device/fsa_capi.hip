@@ -174,6 +174,7 @@ ListJob list_job(const fsa_sim *s, const KindBuffers &f) {
     j.ffine = f.cur_fstart();
     j.vpos = s->virt.pos[0];
     j.vfine = s->virt.cur_fstart();
+    j.vreach = s->P.nv ? s->vx.reach : nullptr;
     j.L = s->lists;
     return j;
 }
@@ -287,16 +288,15 @@ int alloc_lists(fsa_sim *s) {
             HIP_TRY(hipMemsetAsync(f.hvel[b] + n, 0, sizeof(float4) * kPosPad, s->stream));
         }
     }
-    L.nmove = (int)((n + 63) / 64);
     if ((rc = s->alloc(&f.hslot, n)) || (rc = s->alloc(&L.ent, (size_t)(s->P.vrows + kListAhead) * n)) ||
         (rc = s->alloc(&L.meta, n)) ||
         (s->P.vwrows > 0 && ((rc = s->alloc(&L.went, (size_t)(s->P.vwrows + kListAhead) * n)) ||
                              (rc = s->alloc(&L.wmeta, n)))) ||
-        (rc = s->alloc(&L.x0, n)) || (rc = s->alloc(&L.move, (size_t)L.nmove)) ||
+        (rc = s->alloc(&L.x0, n)) || (rc = s->alloc(&L.move, (size_t)kMoveWords)) ||
         (rc = s->alloc(&L.flag, (size_t)kFlagWords)))
         return rc;
     if (!s->dry) {
-        HIP_TRY(hipMemsetAsync(L.move, 0, sizeof(uint32_t) * (size_t)L.nmove, s->stream));
+        HIP_TRY(hipMemsetAsync(L.move, 0, sizeof(uint32_t) * kMoveWords, s->stream));
         HIP_TRY(hipMemsetAsync(L.flag, 0, sizeof(uint32_t) * kFlagWords, s->stream));
     }
     return FSA_OK;

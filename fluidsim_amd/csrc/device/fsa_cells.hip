@@ -43,17 +43,28 @@ struct OpFillCountsImpl {
     }
 };
 
-// per cell: does one of its 27 cells hold a particle of the kind
+// per cell: kReachNear if one of its 27 cells holds a particle of the kind,
+// kReachFar if one of the 125 cells within two cells of it does
 __global__ __launch_bounds__(kBlock) void k_reach(DevParams P, const int32_t *__restrict__ start,
                                                   uint8_t *__restrict__ reach) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= P.ncell) return;
     int b[9], e[9];
     column_ranges(P, c, start, b, e);
-    bool any = false;
+    bool near = false;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) any |= b[k] < e[k];
-    reach[c] = any ? 1 : 0;
+    for (int k = 0; k < 9; ++k) near |= b[k] < e[k];
+    bool far = near;
+    const int3 ci = cell_coords(P, c);
+    const int zlo = max(ci.z - 2, 0), zhi1 = min(ci.z + 2, P.cn[2] - 1) + 1;
+    for (int ox = -2; ox <= 2 && !far; ++ox)
+        for (int oy = -2; oy <= 2 && !far; ++oy) {
+            const int gx = ci.x + ox, gy = ci.y + oy;
+            if ((unsigned)gx >= (unsigned)P.cn[0] || (unsigned)gy >= (unsigned)P.cn[1]) continue;
+            const int base = (gx * P.cn[1] + gy) * P.cn[2];
+            far = start[base + zhi1] > start[base + zlo];
+        }
+    reach[c] = (uint8_t)((near ? kReachNear : 0) | (far ? kReachFar : 0));
 }
 
 struct OpHashImpl {

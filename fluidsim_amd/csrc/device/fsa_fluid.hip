@@ -402,7 +402,7 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, const C
         const int lo = sorted_cells[0] / nyz, hi = sorted_cells[n - 1] / nyz;
         if (__ballot(cols.cx + 1 >= lo && cols.cx - 1 <= hi) == 0) return;
     }
-    if (KIND == kFluidVirt && reach && __ballot(active && reach[me.cell] != 0) == 0) return;
+    if (KIND == kFluidVirt && reach && __ballot(active && (reach[me.cell] & kReachNear) != 0) == 0) return;
     int cb[kColumns], ce[kColumns];
     if (KIND != kFluidFluid) {
         column_ranges(cols, start, cb, ce);
@@ -630,7 +630,7 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
         __builtin_amdgcn_wave_barrier();
         atomicMax(s_wave, m2);
         __builtin_amdgcn_wave_barrier();
-        if (lead) a.L.move[i >> 6] = *s_wave;
+        if (lead) atomicMax(&a.L.move[(i >> 6) & (kMoveWords - 1)], *s_wave);
     }
 }
 

@@ -62,7 +62,7 @@ constexpr int kRunMax = 1 << kRunBits;
 constexpr int kSubZ = 8;
 // solid-solid partners recorded per lane in the first sub-step of a step
 // (entries per lane; a lane with more partners walks its candidates instead)
-constexpr int kSolidListCap = 16;
+constexpr int kSolidListCap = 256 / kSolidLanes;
 
 // the kernels' view of fsa_params
 struct DevParams {
@@ -160,6 +160,7 @@ struct KindBuffers {
 // particle's, x-major, each column's z-window in slot order), each as the
 // byte offset of its home slot in the home state arrays.
 constexpr int kListAhead = 12;     // rows past the longest list the fluid pass may read (and discard)
+constexpr int kMoveWords = 1024;   // words the fluid pass's wavefronts spread their largest moves over
 constexpr int kListRowsMax = 256;  // candidates a list holds at most
 enum ListFlag {
     kFlagMode = 0,      // this step's ListMode (set by the decision, read by the build and fluid passes)
@@ -193,10 +194,10 @@ struct NeighbourLists {
     uint32_t *went = nullptr;
     int32_t *wmeta = nullptr;
     float4 *x0 = nullptr;     // position at the last build
-    // per wavefront of the fluid pass: the largest squared move since the
-    // last build (float bits; the decision reduces and clears them)
+    // the largest squared move since the last build (float bits), in
+    // kMoveWords words: each wavefront of the fluid pass takes the maximum
+    // into one of them, the decision reduces them and clears them
     uint32_t *move = nullptr;
-    int nmove = 0;
     uint32_t *flag = nullptr;  // ListFlag words
 };
 
@@ -223,8 +224,9 @@ struct SolidExtra {
 struct VirtExtra {
     float4 *nrm = nullptr;
     bool normals_ready = false;
-    // per cell: 1 if a wall particle lies in one of its 27 cells (the walls
-    // never move: set once at their upload)
+    // per cell: kReachNear if a wall particle lies in one of its 27 cells,
+    // kReachFar if one lies within two cells (the walls never move: set once
+    // at their upload)
     uint8_t *reach = nullptr;
 };
 
@@ -289,7 +291,9 @@ hipError_t launch_hash_one(const CellJob &j, hipStream_t s);     // fill (zero t
 hipError_t launch_sort_one(const CellJob &j, hipStream_t s);     // scan + place + gather
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int n, hipStream_t s);
 hipError_t launch_fill_f4(float4 *p, float4 v, int n, hipStream_t s);
-// per cell: 1 if a particle of the kind with starts `start` lies in one of its 27 cells
+// per cell: whether a particle of the kind with starts `start` lies in one of
+// its 27 cells (kReachNear) or within two cells of it (kReachFar)
+constexpr uint8_t kReachNear = 1, kReachFar = 2;
 hipError_t launch_reach(const DevParams &P, const int32_t *start, uint8_t *reach, hipStream_t s);
 
 // ---- neighbour lists (fsa_pairs.hip) -------------------------------------------
@@ -307,6 +311,7 @@ struct ListJob {
     const int32_t *ffine;       // per-key starts of the current order
     const float4 *vpos;         // wall positions (sorted once, at their upload)
     const int32_t *vfine;       // per-key starts of the walls
+    const uint8_t *vreach;      // per cell: walls near (kReachNear) or within two cells (kReachFar)
     NeighbourLists L;
 };
 enum ListOp { kLDecide, kLBuild };
@@ -331,7 +336,7 @@ struct FluidStepArgs {
     float4 *rates;  // (ax, ay, az, drho) or null
     int4 *counts;   // with rates: the partners found per kind (fluid, solid, wall, 0)
     const int32_t *scell;   // the solid's cells in slot order (null without solid particles)
-    const uint8_t *vreach;  // per cell: walls within its 27 cells (null without walls)
+    const uint8_t *vreach;  // per cell: walls within its 27 cells (kReachNear) or two cells (null without walls)
     unsigned long long *health;  // FSA_HEALTH_* bits of the scene (an updated velocity not finite)
     // neighbour lists (P.vlist): one thread per home particle, which reads
     // its state from hpos/hvel, writes the update to hpos_out/hvel_out and to

@@ -15,9 +15,9 @@
 // A build runs after every fluid sort, on the device, when a decision pass
 // finds that one is due (so the host never waits, and recorded HIP graphs
 // stay valid):
-//   1. decide  one workgroup: the largest move (the fluid pass leaves one
-//              value per wavefront, rewritten every pass) or a build forced
-//              by the host (upload).  In a fast flow one splash particle
+//   1. decide  one workgroup: the largest move (the fluid pass's waves
+//              take their maxima into kMoveWords words, which the decision
+//              reads and clears) or a build forced by the host (upload).  In a fast flow one splash particle
 //              can call for a build every few steps, more often than a
 //              build pays for: lists that served fewer than kListMinAge
 //              steps switch the scene to walking its cells for a stretch of
@@ -54,19 +54,9 @@ struct OpListDecide {
         if (blk != 0) return;
         // squared moves are float bits of nonnegative values (or NaN), so
         // their order is the unsigned order and a NaN exceeds every bound
-        // 16 loads in flight per thread per round
-        constexpr int kUnroll = 16;
-        uint32_t m = 0;
-        for (int k0 = 0; k0 < j.L.nmove; k0 += kUnroll * kDecideThreads) {
-            uint32_t v[kUnroll];
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-                const int k = k0 + u * kDecideThreads + (int)threadIdx.x;
-                v[u] = k < j.L.nmove ? j.L.move[k] : 0u;
-            }
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) m = max(m, v[u]);
-        }
+        static_assert(kMoveWords == kDecideThreads, "one move word per thread");
+        uint32_t m = j.L.move[threadIdx.x];
+        j.L.move[threadIdx.x] = 0u;
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d, 64));
         if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
@@ -237,6 +227,11 @@ struct OpListBuild {
 #pragma unroll 1
             for (int kind = 0; kind < (j.L.went ? 2 : 1); ++kind) {
                 const bool wall = kind == 1;
+                // most waves have no wall within two cells (the list reach)
+                if (wall && __ballot(valid && (j.vreach[cell] & kReachFar) != 0) == 0) {
+                    if (valid) j.L.wmeta[i] = 0;
+                    continue;
+                }
                 const int rows = wall ? P.vwrows : P.vrows;
                 const int cnt = list_kind(P, cols, w, zlo, zhi1, valid, xi, wall ? j.vpos : j.spos,
                                           wall ? j.vfine : j.ffine, wall ? j.L.went : j.L.ent, nf, i, rows);

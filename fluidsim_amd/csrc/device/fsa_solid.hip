@@ -87,10 +87,8 @@ __device__ __forceinline__ void store9(float *__restrict__ base, int n, int i, c
 // sum over the kSub lanes of a group; lane 0 of the group always combines
 // in the same order, so its result is deterministic
 __device__ __forceinline__ float group_sum(float v) {
-    v += __shfl_xor(v, 8, kSub);
-    v += __shfl_xor(v, 4, kSub);
-    v += __shfl_xor(v, 2, kSub);
-    v += __shfl_xor(v, 1, kSub);
+#pragma unroll
+    for (int d = kSub / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d, kSub);
     return v;
 }
 

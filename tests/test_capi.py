@@ -107,10 +107,10 @@ def test_footprint_counts_the_neighbour_lists():
     position + velocity double buffers (64 B), home index double buffer and
     sorted slot (12 B), its list -- (rows + 12) 32-bit entries and the
     entry count (4 B) -- its wall list -- (wall rows + 12) entries and a
-    count -- and its build position (16 B), plus one 4-byte move word per
-    64 particles.  rows is 1.5 x the lattice sites within the list reach
-    (1 + skin) R, rounded up to a multiple of 8, at most 256; wall rows half
-    of that, rounded down to a multiple of 8, at least 32."""
+    count -- and its build position (16 B); the lists' few words of state
+    do not grow with the particles.  rows is 1.5 x the lattice sites within
+    the list reach (1 + skin) R, rounded up to a multiple of 8, at most 256;
+    wall rows half of that, rounded down to a multiple of 8, at least 32."""
     import math
     from fluidsim_amd import scenes
     from fluidsim_amd.sim import footprint
@@ -123,7 +123,7 @@ def test_footprint_counts_the_neighbour_lists():
     rows = min(max(math.ceil(1.5 * 4 / 3 * math.pi * reach ** 3 / 8) * 8, 32), 256)
     wrows = max(rows // 2 // 8 * 8, 32)
     per = 96 + 64 + 12 + 4 * (rows + 12) + 4 + 4 * (wrows + 12) + 4 + 16
-    assert r1 - r0 == 640 * per + 4 * (math.ceil(1640 / 64) - math.ceil(1000 / 64))
+    assert r1 - r0 == 640 * per
     # without a skin, or with the packed form, there are no lists
     assert footprint(sc.with_(neighbour_skin=0.0).params((1000, 100, 2000)))[0] < r0
     assert footprint(sc.with_(precision_form=_native.FORM_PACKED).params((1000, 100, 2000)))[0] < r0
