@@ -159,12 +159,14 @@ def cpu_baseline(params, arrays, budget_s: float, threads: int) -> dict:
     }
 
 
-def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: float, streams: int = 2) -> dict:
+def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: float, streams: int = 2,
+                advance: int = 0) -> dict:
     """Many scenes of uneven size and stiffness: one batched step of all of them
     (one launch per stage, replayed as a HIP graph) against stepping them one
     after another, each on its own (the plain per-scene solve).  Scenes/hour
     is for solves of `sim_time` seconds of simulated time, from the measured
-    per-step times."""
+    per-step times.  advance: untimed batched steps first (both forms are then
+    timed from that state of the solve)."""
     from fluidsim_amd.sim import DeviceBatch
 
     t = time.perf_counter()
@@ -188,14 +190,25 @@ def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: flo
     solve_steps = math.ceil(sim_time / dt)
 
     b = DeviceBatch(autotune=True, streams=streams)
+    if advance:
+        t = time.perf_counter()
+        b.step(sims, advance)
+        b.synchronize()
+        log(f"batch: advanced {advance} steps ({advance * family[0].time_step:.4f} s simulated) in "
+            f"{time.perf_counter() - t:.1f}s")
+    lists0 = [d.list_stats() for d in sims]
     b.step(sims, warmup)
     b.synchronize()
     t0 = time.perf_counter()
     b.step(sims, steps)
     b.synchronize()
     batched = (time.perf_counter() - t0) / steps
+    lists1 = [d.list_stats() for d in sims]
+    builds = sum(l1["builds"] - l0["builds"] for l0, l1 in zip(lists0, lists1))
+    walks = sum(l1["walks"] - l0["walks"] for l0, l1 in zip(lists0, lists1))
     log(f"batch: batched {1e3 * batched:.3f} ms/step for all {n_scenes} scenes on {streams} launch chains, "
-        f"fluid workgroup {b.stats()['fluid_threads']}")
+        f"fluid workgroup {b.stats()['fluid_threads']}; list builds {builds}, walking scene-steps {walks} "
+        f"over {warmup + steps} steps")
 
     for d in sims:
         d.step(warmup)
@@ -220,7 +233,8 @@ def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: flo
         "scenes": n_scenes, "fluid": nf, "solid": ns, "walls": nv,
         "solid_sub_steps": [min(subs), max(subs)],
         "solve": f"{sim_time} s simulated = {solve_steps} steps of dt {dt:.4g} s",
-        "steps_timed": steps, "warmup": warmup,
+        "steps_timed": steps, "warmup": warmup, "advanced_steps": advance,
+        "list_builds": builds, "walking_scene_steps": walks,
         "batched_ms_per_step": round(1e3 * batched, 4),
         "sequential_ms_per_step": round(1e3 * sequential, 4),
         "scenes_per_hour_batched": round(per_hour(batched), 1),
@@ -443,6 +457,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=30)
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--batch-streams", type=int, default=2, help="concurrent launch chains of the batch (1..8)")
+    ap.add_argument("--batch-advance", type=int, default=0,
+                    help="untimed batched steps before the batch measurement (time the scenes mid-solve)")
     ap.add_argument("--sim-time", type=float, default=0.5, help="simulated seconds per solve (scenes/hour)")
     ap.add_argument("--requests", type=int, default=64, help="serve workload: number of requests")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement of the step workload")
@@ -527,7 +543,8 @@ def main():
 
     if args.workload == "batch":
         r = batch_bench(args.batch_scenes, args.steps or args.batch_steps,
-                        3 if args.warmup is None else args.warmup, args.batch_seed, args.sim_time, args.batch_streams)
+                        3 if args.warmup is None else args.warmup, args.batch_seed, args.sim_time, args.batch_streams,
+                        args.batch_advance)
         out = {
             "metric": "batched scenes/h",
             "value": r["scenes_per_hour_batched"],
@@ -690,7 +707,7 @@ def main():
     log(f"c1 scene on the GPU: {c1.iend} steps in {c1_s:.3f}s")
     if not args.no_batch:
         result["batched"] = batch_bench(args.batch_scenes, args.batch_steps, 3, args.batch_seed, args.sim_time,
-                                        args.batch_streams)
+                                        args.batch_streams, args.batch_advance)
     if not args.no_serve:
         result["serving"] = serve_bench(args.serve_requests, args.batch_seed, args.serve_sim_time, args.load,
                                         args.chunk, args.saturation or None)
