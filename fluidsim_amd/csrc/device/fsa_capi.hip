@@ -706,8 +706,8 @@ int fsa_upload_fluid(fsa_sim *s, const float *x, const float *u, const float *rh
         if ((rc = build_cells(s, k, false))) return rc;
         if (s->P.vlist) {
             // the counts start again and a build is due whatever the moves say
-            HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t *>(s->lists.flag), 0, kFlagWords, s->stream));
-            HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t *>(s->lists.flag) + kFlagForce, 1, 1, s->stream));
+            HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t *>(s->lists.flag.get()), 0, kFlagWords, s->stream));
+            HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t *>(s->lists.flag.get()) + kFlagForce, 1, 1, s->stream));
             HIP_TRY(launch_list_build(list_job(s, k), s->stream));
         }
     }
@@ -818,7 +818,7 @@ int fsa_upload_virt(fsa_sim *s, const float *x, const int32_t *type) {
         s->vx.normals_ready = false;
         // the fluid's neighbour lists hold wall candidates: the next step builds them again
         if (s->P.vlist)
-            HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t *>(s->lists.flag) + kFlagForce, 1, 1, s->stream));
+            HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t *>(s->lists.flag.get()) + kFlagForce, 1, 1, s->stream));
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->uploaded[FSA_VIRT] = true;

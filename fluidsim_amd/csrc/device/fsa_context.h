@@ -86,6 +86,12 @@ struct fsa_sim {
     int64_t bytes = 0;
     bool dry = false;  // only count the bytes (fsa_footprint)
 
+    template <class T> int alloc(fsa::gptr<T> *p, size_t count) {
+        T *q = nullptr;
+        const int rc = alloc(&q, count);
+        *p = q;
+        return rc;
+    }
     template <class T> int alloc(T **p, size_t count) {
         *p = nullptr;
         if (count == 0) return FSA_OK;

@@ -29,6 +29,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 #include "fluidsim_amd.h"
 
@@ -65,6 +66,33 @@ constexpr int kSubZ = 8;
 constexpr int kSolidListCap = 256 / kSolidLanes;
 
 // the kernels' view of fsa_params
+// A pointer to global memory, as held by the job descriptors.  A batched
+// launch reads its scenes' descriptors from device memory, so the compiler
+// could not tell where the pointers they hold point: it would address them
+// with flat instructions, which take no scalar base and whose waits cover LDS
+// traffic too (measured: the batched fluid pass ran 40 % slower per particle
+// than the same scenes launched one by one).  Typed as address space 1 in
+// device code, every access through them is a global one.  The layout is a
+// plain pointer on both sides.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FSA_GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define FSA_GLOBAL_AS
+#endif
+template <class T>
+struct gptr {
+    FSA_GLOBAL_AS T *p;
+    gptr() = default;
+    __host__ __device__ gptr(T *q) : p((FSA_GLOBAL_AS T *)q) {}
+    __host__ __device__ gptr(std::nullptr_t) : p(nullptr) {}
+    template <class U, class = typename std::enable_if<std::is_convertible<U *, T *>::value &&
+                                                       !std::is_same<U, T>::value>::type>
+    __host__ __device__ gptr(const gptr<U> &o) : p((FSA_GLOBAL_AS T *)o.p) {}
+    __host__ __device__ operator T *() const { return (T *)p; }
+    __host__ __device__ T *get() const { return (T *)p; }
+};
+static_assert(sizeof(gptr<float4>) == sizeof(void *), "a job pointer is a plain pointer");
+
 struct DevParams {
     int nf, ns, nv;
     float gx, gy, gz;
@@ -187,38 +215,38 @@ constexpr uint32_t kListMinAge = 8;
 constexpr uint32_t kListRetry = 64;
 constexpr uint32_t kListRetryMax = 1024;
 struct NeighbourLists {
-    uint32_t *ent = nullptr;  // entry k of home particle i at ent[k * nf + i]; vrows + kListAhead rows
-    int32_t *meta = nullptr;  // entries, or -1: not listed (the particle walks its cells)
+    gptr<uint32_t> ent = nullptr;  // entry k of home particle i at ent[k * nf + i]; vrows + kListAhead rows
+    gptr<int32_t> meta = nullptr;  // entries, or -1: not listed (the particle walks its cells)
     // wall candidates (byte offsets of wall slots), the same layout with
     // vwrows + kListAhead rows; wmeta -1: the particle walks the wall cells
-    uint32_t *went = nullptr;
-    int32_t *wmeta = nullptr;
-    float4 *x0 = nullptr;     // position at the last build
+    gptr<uint32_t> went = nullptr;
+    gptr<int32_t> wmeta = nullptr;
+    gptr<float4> x0 = nullptr;     // position at the last build
     // the largest squared move since the last build (float bits), in
     // kMoveWords words: each wavefront of the fluid pass takes the maximum
     // into one of them, the decision reduces them and clears them
-    uint32_t *move = nullptr;
-    uint32_t *flag = nullptr;  // ListFlag words
+    gptr<uint32_t> move = nullptr;
+    gptr<uint32_t> flag = nullptr;  // ListFlag words
 };
 
 struct SolidExtra {
-    float4 *np = nullptr;   // normal + coupling pressure
-    float4 *acc = nullptr;  // du_dt (w unused)
-    float4 *couple = nullptr;
-    float *F = nullptr, *P = nullptr, *L = nullptr, *A = nullptr;  // 9 * ns each
+    gptr<float4> np = nullptr;   // normal + coupling pressure
+    gptr<float4> acc = nullptr;  // du_dt (w unused)
+    gptr<float4> couple = nullptr;
+    gptr<float> F = nullptr, P = nullptr, L = nullptr, A = nullptr;  // 9 * ns each
     // P L by rows, (row r of particle i) at PL[3 * i + r] (w unused): the
     // force pass gathers it per partner, three loads instead of nine
-    float4 *PL = nullptr;
+    gptr<float4> PL = nullptr;
     // reference-configuration partners of the step: lane l of particle i holds
     // its hits in walk order at nbr[(i * kSolidListCap + k) * kSolidLanes + l],
     // their number (uncapped) at ncnt[i * kSolidLanes + l]
-    int32_t *nbr = nullptr;
-    int32_t *ncnt = nullptr;
+    gptr<int32_t> nbr = nullptr;
+    gptr<int32_t> ncnt = nullptr;
     // fluid candidates within R_rec of the step-start position x0, recorded
     // per lane by the coupling pass in the same layout
-    int32_t *fnbr = nullptr;
-    int32_t *fcnt = nullptr;
-    float4 *x0 = nullptr;
+    gptr<int32_t> fnbr = nullptr;
+    gptr<int32_t> fcnt = nullptr;
+    gptr<float4> x0 = nullptr;
 };
 
 struct VirtExtra {
@@ -268,19 +296,19 @@ struct CellJob {
     int nk;    // keys: ncell * nsub
     int has_ref;
     int count_jumps;
-    const float4 *pos_in, *vel_in, *ref_in;
-    const float4 *cpos_in, *cvel_in;  // compensation terms (null unless compensated)
-    const int32_t *id_in, *prev_cell;
-    float4 *pos_out, *vel_out, *ref_out;
-    float4 *cpos_out, *cvel_out;
-    uint4 *pk_out;  // FP16 partner records of the sorted state (null unless packed)
-    int32_t *id_out, *cell_out;
+    gptr<const float4> pos_in, vel_in, ref_in;
+    gptr<const float4> cpos_in, cvel_in;  // compensation terms (null unless compensated)
+    gptr<const int32_t> id_in, prev_cell;
+    gptr<float4> pos_out, vel_out, ref_out;
+    gptr<float4> cpos_out, cvel_out;
+    gptr<uint4> pk_out;  // FP16 partner records of the sorted state (null unless packed)
+    gptr<int32_t> id_out, cell_out;
     // neighbour lists (else null): home index of each slot in and out, and
     // the sorted slot of each home index
-    const int32_t *hidx_in;
-    int32_t *hidx_out, *hslot;
-    int32_t *key, *rank, *count, *start, *fstart, *tmp_slot, *tmp_id, *tile_sums;
-    unsigned long long *jumps;
+    gptr<const int32_t> hidx_in;
+    gptr<int32_t> hidx_out, hslot;
+    gptr<int32_t> key, rank, count, start, fstart, tmp_slot, tmp_id, tile_sums;
+    gptr<unsigned long long> jumps;
 };
 // the job of kind k reading the current buffers and writing the other ones
 CellJob make_cell_job(const DevParams &P, const KindBuffers &k, unsigned long long *jumps, bool count_jumps);
@@ -304,14 +332,14 @@ hipError_t launch_reach(const DevParams &P, const int32_t *start, uint8_t *reach
 // reads, home index = sorted slot) and the lists.
 struct ListJob {
     DevParams P;
-    const float4 *spos, *svel;  // sorted state of the current order
-    float4 *hpos, *hvel;        // home state the next fluid pass reads
-    int32_t *hidx;              // home index of each sorted slot of the current order
-    int32_t *hslot;             // sorted slot of each home index
-    const int32_t *ffine;       // per-key starts of the current order
-    const float4 *vpos;         // wall positions (sorted once, at their upload)
-    const int32_t *vfine;       // per-key starts of the walls
-    const uint8_t *vreach;      // per cell: walls near (kReachNear) or within two cells (kReachFar)
+    gptr<const float4> spos, svel;  // sorted state of the current order
+    gptr<float4> hpos, hvel;        // home state the next fluid pass reads
+    gptr<int32_t> hidx;              // home index of each sorted slot of the current order
+    gptr<int32_t> hslot;             // sorted slot of each home index
+    gptr<const int32_t> ffine;       // per-key starts of the current order
+    gptr<const float4> vpos;         // wall positions (sorted once, at their upload)
+    gptr<const int32_t> vfine;       // per-key starts of the walls
+    gptr<const uint8_t> vreach;      // per cell: walls near (kReachNear) or within two cells (kReachFar)
     NeighbourLists L;
 };
 enum ListOp { kLDecide, kLBuild };
@@ -323,28 +351,28 @@ hipError_t launch_list_build(const ListJob &j, hipStream_t s);
 
 // ---- fluid pass (fsa_fluid.hip) -----------------------------------------------
 struct FluidStepArgs {
-    const float4 *fpos, *fvel;
-    const float4 *fcpos, *fcvel;     // compensation terms in (null unless compensated)
-    float4 *fcpos_out, *fcvel_out;   // and out
-    const uint4 *fpk;                // FP16 partner records (null unless packed)
-    const int32_t *fcell;
-    const int32_t *fstart, *sstart, *vstart;
-    const int32_t *ffine, *vfine;    // per-key starts of the fluid and the walls (kSubZ keys per cell)
-    const float4 *spos, *svel, *snp;
-    const float4 *vpos, *vvel;
-    float4 *fpos_out, *fvel_out;
-    float4 *rates;  // (ax, ay, az, drho) or null
-    int4 *counts;   // with rates: the partners found per kind (fluid, solid, wall, 0)
-    const int32_t *scell;   // the solid's cells in slot order (null without solid particles)
-    const uint8_t *vreach;  // per cell: walls within its 27 cells (kReachNear) or two cells (null without walls)
-    unsigned long long *health;  // FSA_HEALTH_* bits of the scene (an updated velocity not finite)
+    gptr<const float4> fpos, fvel;
+    gptr<const float4> fcpos, fcvel;     // compensation terms in (null unless compensated)
+    gptr<float4> fcpos_out, fcvel_out;   // and out
+    gptr<const uint4> fpk;                // FP16 partner records (null unless packed)
+    gptr<const int32_t> fcell;
+    gptr<const int32_t> fstart, sstart, vstart;
+    gptr<const int32_t> ffine, vfine;    // per-key starts of the fluid and the walls (kSubZ keys per cell)
+    gptr<const float4> spos, svel, snp;
+    gptr<const float4> vpos, vvel;
+    gptr<float4> fpos_out, fvel_out;
+    gptr<float4> rates;  // (ax, ay, az, drho) or null
+    gptr<int4> counts;   // with rates: the partners found per kind (fluid, solid, wall, 0)
+    gptr<const int32_t> scell;   // the solid's cells in slot order (null without solid particles)
+    gptr<const uint8_t> vreach;  // per cell: walls within its 27 cells (kReachNear) or two cells (null without walls)
+    gptr<unsigned long long> health;  // FSA_HEALTH_* bits of the scene (an updated velocity not finite)
     // neighbour lists (P.vlist): one thread per home particle, which reads
     // its state from hpos/hvel, writes the update to hpos_out/hvel_out and to
     // the sorted buffers at hslot (fpos_out, fvel_out, the compensation
     // terms, rates and counts, all in sorted order)
-    const float4 *hpos, *hvel;
-    float4 *hpos_out, *hvel_out;
-    const int32_t *hslot;
+    gptr<const float4> hpos, hvel;
+    gptr<float4> hpos_out, hvel_out;
+    gptr<const int32_t> hslot;
     NeighbourLists L;
 };
 struct FluidJob {
@@ -358,12 +386,12 @@ hipError_t launch_fluid(const DevParams &P, const FluidStepArgs &a, int threads,
 
 // ---- walls (fsa_virt.hip) ------------------------------------------------------
 struct VirtArgs {
-    const float4 *vpos_in;
-    float4 *vpos, *vvel, *vnrm;
-    const int32_t *vcell;
-    const int32_t *fstart, *vstart;
-    const int32_t *ffine;  // per-key starts of the fluid (kSubZ keys per cell)
-    const float4 *fpos, *fvel;
+    gptr<const float4> vpos_in;
+    gptr<float4> vpos, vvel, vnrm;
+    gptr<const int32_t> vcell;
+    gptr<const int32_t> fstart, vstart;
+    gptr<const int32_t> ffine;  // per-key starts of the fluid (kSubZ keys per cell)
+    gptr<const float4> fpos, fvel;
     int compute_normals;
 };
 struct VirtJob {
@@ -374,16 +402,16 @@ hipError_t launch_virt(const DevParams &P, const VirtArgs &a, hipStream_t s);
 
 // ---- solid (fsa_solid.hip) -------------------------------------------------------
 struct SolidArgs {
-    float4 *spos, *svel;
-    float4 *scpos, *scvel;  // compensation terms (null unless compensated)
-    const float4 *sref;
-    const int32_t *scell;
-    const int32_t *fstart, *sstart;
-    const float4 *fpos, *fvel;
+    gptr<float4> spos, svel;
+    gptr<float4> scpos, scvel;  // compensation terms (null unless compensated)
+    gptr<const float4> sref;
+    gptr<const int32_t> scell;
+    gptr<const int32_t> fstart, sstart;
+    gptr<const float4> fpos, fvel;
     SolidExtra x;  // device pointers, passed by value
     int fluid_rec;  // the coupling pass of this step (this order) recorded the fluid partners
     int solid_rec;  // later passes replay the solid partners the first deformation pass recorded
-    unsigned long long *health;  // FSA_HEALTH_* bits of the scene
+    gptr<unsigned long long> health;  // FSA_HEALTH_* bits of the scene
 };
 struct SolidJob {
     DevParams P;
