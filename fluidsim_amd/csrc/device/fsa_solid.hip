@@ -216,6 +216,11 @@ __device__ __forceinline__ void deformation_body(const DevParams &P, const Solid
     const int i = L.i;
     const float4 Xi = a.sref[i];
     const float4 xi = a.spos[i];
+    // The correction matrix L = B^-1, B = sum dX dX^T k, depends on the
+    // reference positions and the partner order only, both fixed within a
+    // step: a replaying pass (the later sub-steps) reads the L the step's
+    // first pass stored, the same bits as summing B again.
+    const bool reuse_L = MODE == kReplay && a.solid_rec != 0;
     M3 F = mzero(), B = mzero();
     solid_partners<MODE>(P, a, i, L.sub, Xi, [&](int j, const float (&dX)[3], float d2) {
         const float4 xj = a.spos[j];
@@ -224,20 +229,26 @@ __device__ __forceinline__ void deformation_body(const DevParams &P, const Solid
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                F.m[r][c] += (dx[r] * dX[c]) * kc;
-                B.m[r][c] += (dX[r] * dX[c]) * kc;
-            }
+            for (int c = 0; c < 3; ++c) F.m[r][c] += (dx[r] * dX[c]) * kc;
+        if (!reuse_L) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) B.m[r][c] += (dX[r] * dX[c]) * kc;
+        }
     });
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            F.m[r][c] = group_sum(F.m[r][c]);
-            B.m[r][c] = group_sum(B.m[r][c]);
-        }
+        for (int c = 0; c < 3; ++c) F.m[r][c] = group_sum(F.m[r][c]);
+    if (!reuse_L) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) B.m[r][c] = group_sum(B.m[r][c]);
+    }
     if (L.sub != 0) return;
-    const M3 Lc = mdet(B) != 0.0f ? minverse(B) : mident();
+    const M3 Lc = reuse_L ? load9(a.x.L, P.ns, i) : (mdet(B) != 0.0f ? minverse(B) : mident());
     F = mmul(F, Lc);
     // St Venant-Kirchhoff (solidStressCalculation, solid.cu:27-36)
     M3 E;
@@ -258,7 +269,7 @@ __device__ __forceinline__ void deformation_body(const DevParams &P, const Solid
     const int n = P.ns;
     store9(a.x.F, n, i, F);
     store9(a.x.P, n, i, Pk);
-    store9(a.x.L, n, i, Lc);
+    if (!reuse_L) store9(a.x.L, n, i, Lc);
     {
         const M3 PL = mmul(Pk, Lc);
 #pragma unroll
@@ -345,7 +356,12 @@ __device__ __forceinline__ void force_body(const DevParams &P, const SolidArgs &
             }
         });
     }
-    // stress divergence + artificial viscosity (solidSolidForce, solid.cu:119-157)
+    // stress divergence + artificial viscosity (solidSolidForce, solid.cu:119-157):
+    //   sum_j (P_i L_i + P_j L_j) dX k / rho_s + (det F_i F_i^-T L_i) dX (visc k)
+    // with the particle's own matrices taken out of the sums: each lane sums
+    // P_j L_j dX ks, dX ks and dX visc k over its partners, then applies
+    // P_i L_i and det F_i F_i^-T L_i to its two vector sums once.
+    float sum_ks[3] = {0.0f, 0.0f, 0.0f}, sum_vk[3] = {0.0f, 0.0f, 0.0f};
     solid_partners<kReplay>(P, a, i, L.sub, Xi, [&](int j, const float (&dX)[3], float d2) {
         const float kc = hw_exp2(fmaf(-d2, P.kexp2, P.log2kd));
         const float ks = kc * P.rho_s_inv;
@@ -353,11 +369,7 @@ __device__ __forceinline__ void force_body(const DevParams &P, const SolidArgs &
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const float4 row = a.x.PL[3 * (size_t)j + r];
-            const float pl[3] = {row.x, row.y, row.z};
-            float t = 0.0f;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) t += (pl[c] + PLi.m[r][c]) * dX[c];
-            v[r] = t * ks;
+            v[r] = (row.x * dX[0] + row.y * dX[1] + row.z * dX[2]) * ks;
         }
         ax += v[0];
         ay += v[1];
@@ -368,17 +380,23 @@ __device__ __forceinline__ void force_body(const DevParams &P, const SolidArgs &
         const float nu = P.h * dudx * hw_rcp(len2(dx, dy, dz) + eps);
         float visc = P.alpha * nu * P.cs_s;
         visc -= P.beta * nu * nu;
-        // ((det F F^-T visc) L_i) dX k = (det F F^-T L_i) dX (visc k)
         const float vk = visc * kc;
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            const float t = ALi.m[r][0] * dX[0] + ALi.m[r][1] * dX[1] + ALi.m[r][2] * dX[2];
-            v[r] = t * vk;
+        for (int c = 0; c < 3; ++c) {
+            sum_ks[c] += dX[c] * ks;
+            sum_vk[c] += dX[c] * vk;
         }
+    });
+    {
+        float v[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            v[r] = (PLi.m[r][0] * sum_ks[0] + PLi.m[r][1] * sum_ks[1] + PLi.m[r][2] * sum_ks[2]) +
+                   (ALi.m[r][0] * sum_vk[0] + ALi.m[r][1] * sum_vk[1] + ALi.m[r][2] * sum_vk[2]);
         ax += v[0];
         ay += v[1];
         az += v[2];
-    });
+    }
     cx = group_sum(cx);
     cy = group_sum(cy);
     cz = group_sum(cz);
