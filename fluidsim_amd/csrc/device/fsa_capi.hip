@@ -141,7 +141,8 @@ DevParams make_dev_params(const fsa_params &p) {
     // neighbour lists (fsa_pairs.hip); not with the packed form, whose
     // partner records are written in sorted order by the sort
     const double skin = p.neighbour_skin;
-    d.vlist = (skin > 0.0 && !(p.precision_form & FSA_FORM_PACKED) && p.n_fluid > 0) ? 1 : 0;
+    d.vlist = (skin > 0.0 && !(p.precision_form & FSA_FORM_PACKED) && p.n_fluid > 0 &&
+               p.n_fluid <= kMaxListedFluid) ? 1 : 0;
     if (d.vlist) {
         const double R = std::sqrt((double)p.impact_distance_square), reach = (1.0 + skin) * R;
         // FP32 positions and squared distances carry relative errors of a few
@@ -167,8 +168,7 @@ ListJob list_job(const fsa_sim *s, const KindBuffers &f) {
     j.P = s->P;
     j.spos = f.pos[f.st];
     j.svel = f.vel[f.st];
-    j.hpos = f.hpos[f.hst];
-    j.hvel = f.hvel[f.hst];
+    j.hrec = f.hrec[f.hst];
     j.hidx = f.hidx[f.ord];
     j.hslot = f.hslot;
     j.ffine = f.cur_fstart();
@@ -278,14 +278,14 @@ int alloc_lists(fsa_sim *s) {
     const size_t n = (size_t)s->P.nf;
     int rc;
     for (int b = 0; b < 2; ++b) {
-        if ((rc = s->alloc(&f.hpos[b], n + kPosPad)) || (rc = s->alloc(&f.hvel[b], n + kPosPad)) ||
-            (rc = s->alloc(&f.hidx[b], n)))
-            return rc;
+        if ((rc = s->alloc(&f.hrec[b], 2 * (n + kPosPad))) || (rc = s->alloc(&f.hidx[b], n))) return rc;
         if (!s->dry) {
-            // the padding entries hold the sentinel partner
-            HIP_TRY(launch_fill_f4(f.hpos[b] + n, make_float4(kFarPos, kFarPos, kFarPos, s->P.rho0), kPosPad,
+            // the padding records hold the sentinel partner
+            std::vector<float4> pad(2 * kPosPad, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+            for (int k = 0; k < kPosPad; ++k) pad[2 * k] = make_float4(kFarPos, kFarPos, kFarPos, s->P.rho0);
+            HIP_TRY(hipMemcpyAsync(f.hrec[b] + 2 * n, pad.data(), sizeof(float4) * pad.size(), hipMemcpyHostToDevice,
                                    s->stream));
-            HIP_TRY(hipMemsetAsync(f.hvel[b] + n, 0, sizeof(float4) * kPosPad, s->stream));
+            HIP_TRY(hipStreamSynchronize(s->stream));
         }
     }
     if ((rc = s->alloc(&f.hslot, n)) || (rc = s->alloc(&L.ent, (size_t)(s->P.vrows + kListAhead) * n)) ||
@@ -511,10 +511,8 @@ VirtArgs fsa::virt_args(const fsa_sim *s, const KindBuffers &f, int compute_norm
 FluidStepArgs fsa::fluid_args(const fsa_sim *s, const KindBuffers &f, const KindBuffers &so) {
     FluidStepArgs a{};
     if (s->P.vlist) {
-        a.hpos = f.hpos[f.hst];
-        a.hvel = f.hvel[f.hst];
-        a.hpos_out = f.hpos[1 - f.hst];
-        a.hvel_out = f.hvel[1 - f.hst];
+        a.hrec = f.hrec[f.hst];
+        a.hrec_out = f.hrec[1 - f.hst];
         a.hslot = f.hslot;
         a.L = s->lists;
     }

@@ -438,7 +438,8 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, const C
 }
 
 // Fluid (or wall) partners from the particle's neighbour list (home
-// particle i; n particles of the kind in pos, vel): every listed candidate
+// particle i; entries are byte offsets into pos and vel, `sentinel` the
+// offset of the kind's sentinel partner): every listed candidate
 // is evaluated in list order (the walk order of the build) and counts only
 // inside the cut-off.  About nine in ten listed fluid candidates are
 // partners, so this costs less than the walk's test-then-evaluate, and each
@@ -448,10 +449,10 @@ __device__ __forceinline__ void pass(const DevParams &P, const Self &me, const C
 template <int KIND, bool COUNT>
 __device__ __forceinline__ void list_pass(const DevParams &P, const Self &me, int i, int cnt,
                                           const uint32_t *__restrict__ list, const float4 *__restrict__ pos,
-                                          const float4 *__restrict__ vel, int n, Acc &acc, int &found) {
+                                          const float4 *__restrict__ vel, uint32_t sentinel, Acc &acc,
+                                          int &found) {
     if (__ballot(cnt > 0) == 0) return;
     const size_t nf = (size_t)P.nf;
-    const uint32_t sentinel = slot_bytes(n);
     const uint32_t *__restrict__ ent = list + i;
     uint32_t en[kPairBatch];
     float4 pc[kPairBatch], vc[kPairBatch];
@@ -500,8 +501,8 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     if (i >= P.nf) return;  // lists are lane-private: no barrier follows
     const bool LIST = !PACK && P.vlist != 0 && a.L.flag[kFlagMode] != kModeWalk;
 
-    const float4 pi4 = (LIST ? a.hpos : a.fpos)[i];
-    const float4 vi4 = (LIST ? a.hvel : a.fvel)[i];
+    const float4 pi4 = LIST ? a.hrec[2 * (size_t)i] : a.fpos[i];
+    const float4 vi4 = LIST ? a.hrec[2 * (size_t)i + 1] : a.fvel[i];
     Self me;
     me.x = pi4.x;
     me.y = pi4.y;
@@ -540,7 +541,10 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     // lists) walk their cells, over the sorted state of this step (the same
     // values as the home state)
     const int meta = LIST ? a.L.meta[i] : -1;
-    if (!PACK && LIST) list_pass<kFluidFluid, RATES>(P, me, i, meta, a.L.ent, a.hpos, a.hvel, P.nf, acc, found[0]);
+    // (home records: position and velocity 16 bytes apart, entries of 32-byte records)
+    if (!PACK && LIST)
+        list_pass<kFluidFluid, RATES>(P, me, i, meta, a.L.ent, a.hrec, a.hrec + 1, (uint32_t)P.nf << 5, acc,
+                                      found[0]);
     if (__ballot(meta < 0) != 0)
         pass<kFluidFluid, T, PACK, RATES>(P, me, cols, w, P.nf, a.fstart, a.ffine, a.fpos, a.fvel, nullptr, fsrc,
                                           list, acc, found[0], nullptr, nullptr, meta < 0);
@@ -549,7 +553,7 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
     // wall partners likewise (their lists are built with the fluid's)
     const int wmeta = LIST && a.L.went ? a.L.wmeta[i] : -1;
     if (!PACK && LIST && a.L.went)
-        list_pass<kFluidVirt, RATES>(P, me, i, wmeta, a.L.went, a.vpos, a.vvel, P.nv, acc, found[2]);
+        list_pass<kFluidVirt, RATES>(P, me, i, wmeta, a.L.went, a.vpos, a.vvel, slot_bytes(P.nv), acc, found[2]);
     if (__ballot(wmeta < 0) != 0)
         pass<kFluidVirt, T, false, RATES>(P, me, cols, w, P.nv, a.vstart, a.vfine, a.vpos, a.vvel, nullptr, none,
                                           list, acc, found[2], nullptr, a.vreach, wmeta < 0);
@@ -617,8 +621,8 @@ __device__ __forceinline__ void fluid_body(const DevParams &P, const FluidStepAr
         a.fcvel_out[o] = cv;
     }
     if (LIST) {
-        a.hpos_out[i] = xo;
-        a.hvel_out[i] = uo;
+        a.hrec_out[2 * (size_t)i] = xo;
+        a.hrec_out[2 * (size_t)i + 1] = uo;
         // the wave's largest squared move since the last list build, for the
         // next build decision (float bits of a nonnegative value or NaN:
         // unsigned order); through the wave's first lane's base slot, which

@@ -48,6 +48,9 @@ constexpr int kPosPad = 16;
 constexpr float kFarPos = 1.0e15f;
 // slot byte offsets are 32-bit (slot * 16 B)
 constexpr int kMaxKindParticles = (1 << 28) - kPosPad;
+// and so are the neighbour lists' entries (home record * 32 B): a larger
+// fluid walks its cells every step
+constexpr int kMaxListedFluid = (1 << 27) - kPosPad;
 constexpr int kSolidLanes = 16;  // lanes serving one solid particle
 // 16-bit partner entries of the fluid pass (and of the neighbour lists):
 // offset inside a run of slots (kRunBits, high), run-base id (4 bits, low)
@@ -171,13 +174,15 @@ struct KindBuffers {
     // packed form (fluid): the FP16 partner records of the current order
     uint4 *pk = nullptr;
     // neighbour lists (fluid): the fluid pass reads and writes the state in
-    // its home order -- the sorted order of the last list build -- in hpos /
-    // hvel (double-buffered by hst), and also writes it into the sorted
-    // buffers, which every other pass reads.  hidx holds the home index of
-    // each sorted slot (indexed by ord, moved by every sort), hslot the
+    // its home order -- the sorted order of the last list build -- in hrec
+    // (double-buffered by hst), and also writes it into the sorted buffers,
+    // which every other pass reads.  A home record is 32 bytes, position
+    // (x, y, z, rho) at hrec[2 h] and velocity (u, p) at hrec[2 h + 1], so
+    // that a listed candidate's two loads fall in one cache line; kPosPad
+    // padding records hold the sentinel partner.  hidx holds the home index
+    // of each sorted slot (indexed by ord, moved by every sort), hslot the
     // sorted slot of each home index (written by every sort).
-    float4 *hpos[2] = {nullptr, nullptr};
-    float4 *hvel[2] = {nullptr, nullptr};
+    float4 *hrec[2] = {nullptr, nullptr};
     int hst = 0;
     int32_t *hidx[2] = {nullptr, nullptr};
     int32_t *hslot = nullptr;
@@ -333,7 +338,7 @@ hipError_t launch_reach(const DevParams &P, const int32_t *start, uint8_t *reach
 struct ListJob {
     DevParams P;
     gptr<const float4> spos, svel;  // sorted state of the current order
-    gptr<float4> hpos, hvel;        // home state the next fluid pass reads
+    gptr<float4> hrec;              // home records the next fluid pass reads
     gptr<int32_t> hidx;              // home index of each sorted slot of the current order
     gptr<int32_t> hslot;             // sorted slot of each home index
     gptr<const int32_t> ffine;       // per-key starts of the current order
@@ -367,11 +372,11 @@ struct FluidStepArgs {
     gptr<const uint8_t> vreach;  // per cell: walls within its 27 cells (kReachNear) or two cells (null without walls)
     gptr<unsigned long long> health;  // FSA_HEALTH_* bits of the scene (an updated velocity not finite)
     // neighbour lists (P.vlist): one thread per home particle, which reads
-    // its state from hpos/hvel, writes the update to hpos_out/hvel_out and to
-    // the sorted buffers at hslot (fpos_out, fvel_out, the compensation
+    // its state from the home records hrec, writes the update to hrec_out and
+    // to the sorted buffers at hslot (fpos_out, fvel_out, the compensation
     // terms, rates and counts, all in sorted order)
-    gptr<const float4> hpos, hvel;
-    gptr<float4> hpos_out, hvel_out;
+    gptr<const float4> hrec;
+    gptr<float4> hrec_out;
     gptr<const int32_t> hslot;
     NeighbourLists L;
 };

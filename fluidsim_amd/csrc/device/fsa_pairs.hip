@@ -133,12 +133,13 @@ constexpr int kBuildGrid = 2048;  // workgroups of one scene's build (grid-strid
 
 // The candidates of one kind (positions pos, per-key starts fine) within
 // the list reach of xi, in the walk order of the 5x5 columns, written to
-// rows 0.. of ent as the byte offsets of their slots while fewer than
-// `rows`; returns how many there are.
+// rows 0.. of ent as slot << shift -- the byte offsets of their records:
+// 32-byte home records for the fluid, 16-byte positions for the walls --
+// while fewer than `rows`; returns how many there are.
 __device__ __forceinline__ int list_kind(const DevParams &P, const Columns &cols, const Window &w, int zlo, int zhi1,
                                          bool valid, const float4 &xi, const float4 *__restrict__ pos,
                                          const int32_t *__restrict__ fine, uint32_t *__restrict__ ent, size_t nf,
-                                         int i, int rows) {
+                                         int i, int rows, int shift) {
     int cnt = 0;
 #pragma unroll 1
     for (int ox = -2; ox <= 2; ++ox) {
@@ -173,7 +174,7 @@ __device__ __forceinline__ int list_kind(const DevParams &P, const Columns &cols
                 for (int k = 0; k < kBuildBatch; ++k) {
                     const float d2 = len2(q[k].x - xi.x, q[k].y - xi.y, q[k].z - xi.z);
                     if (k < lim && d2 < P.vreach2) {
-                        if (cnt < rows) ent[(size_t)cnt * nf + i] = (uint32_t)(ss + k) << 4;
+                        if (cnt < rows) ent[(size_t)cnt * nf + i] = (uint32_t)(ss + k) << shift;
                         ++cnt;
                     }
                 }
@@ -209,8 +210,8 @@ struct OpListBuild {
             if (valid) {
                 // the new home order (the candidates below are read from the
                 // sorted state, the same values)
-                j.hpos[i] = xi;
-                j.hvel[i] = j.svel[i];
+                j.hrec[2 * (size_t)i] = xi;
+                j.hrec[2 * (size_t)i + 1] = j.svel[i];
                 j.hidx[i] = i;
                 j.hslot[i] = i;
                 j.L.x0[i] = xi;
@@ -234,7 +235,8 @@ struct OpListBuild {
                 }
                 const int rows = wall ? P.vwrows : P.vrows;
                 const int cnt = list_kind(P, cols, w, zlo, zhi1, valid, xi, wall ? j.vpos : j.spos,
-                                          wall ? j.vfine : j.ffine, wall ? j.L.went : j.L.ent, nf, i, rows);
+                                          wall ? j.vfine : j.ffine, wall ? j.L.went : j.L.ent, nf, i, rows,
+                                          wall ? 4 : 5);
                 const bool listed = valid && cnt <= rows;
                 unlisted |= valid && !listed;
                 if (valid) (wall ? j.L.wmeta : j.L.meta)[i] = listed ? cnt : -1;

@@ -129,7 +129,8 @@ typedef struct fsa_params {
      * last build; the exact cut-off test still runs every step, so every
      * step finds the same partner set as a walk of the 27 cells.  From the
      * optional ini key `neighbour_skin` (no counterpart in the reference;
-     * 0 = walk the cells every step).  Not used with FSA_FORM_PACKED. */
+     * 0 = walk the cells every step).  Not used with FSA_FORM_PACKED, nor
+     * beyond 2^27 - 16 fluid particles (32-bit list entries). */
     float neighbour_skin;
 } fsa_params;
 
