@@ -159,7 +159,7 @@ def cpu_baseline(params, arrays, budget_s: float, threads: int) -> dict:
     }
 
 
-def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: float, streams: int = 2,
+def batch_bench(n_scenes: int, steps: int, warmup: int, seed: int, sim_time: float, streams: int = 3,
                 advance: int = 0) -> dict:
     """Many scenes of uneven size and stiffness: one batched step of all of them
     (one launch per stage, replayed as a HIP graph) against stepping them one
@@ -456,7 +456,7 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=30)
     ap.add_argument("--batch-seed", type=int, default=0)
-    ap.add_argument("--batch-streams", type=int, default=2, help="concurrent launch chains of the batch (1..8)")
+    ap.add_argument("--batch-streams", type=int, default=3, help="concurrent launch chains of the batch (1..8)")
     ap.add_argument("--batch-advance", type=int, default=0,
                     help="untimed batched steps before the batch measurement (time the scenes mid-solve)")
     ap.add_argument("--sim-time", type=float, default=0.5, help="simulated seconds per solve (scenes/hour)")

@@ -148,7 +148,7 @@ struct fsa_batch {
     bool own_stream = false;
     unsigned flags = 0;
     hipEvent_t ev_sync = nullptr;
-    int nchains = 2;               // concurrent launch chains the scenes are split over
+    int nchains = 3;               // concurrent launch chains the scenes are split over
     std::vector<Chain *> chains;   // created as needed
     std::vector<hipEvent_t> ev_chain;
     int64_t steps_run = 0;

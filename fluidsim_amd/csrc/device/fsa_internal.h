@@ -7,7 +7,8 @@
 //   solid  pos  float4 (x, y, z, -)     vel float4 (ux, uy, uz, -)
 //          ref  float4 (X, Y, Z, type)  np  float4 (nx, ny, nz, p)
 //          F, P, L, A = det(F) F^-T : 9 SoA planes of ns floats
-//          (plane e = row-major element (e/3, e%3)); PL = P*L as 3 float4 rows
+//          (plane e = row-major element (e/3, e%3)); frec: per particle
+//          the force pass's partner record, 6 float4 (X, x, u, P L rows)
 //   virt   pos  float4 (x, y, z, rho)   vel float4 (ux, uy, uz, p)
 //          nrm  float4 (nx, ny, nz, -)
 //   cells  start[kind][ncell+1]: slots of cell c are [start[c], start[c+1])
@@ -52,6 +53,7 @@ constexpr int kMaxKindParticles = (1 << 28) - kPosPad;
 // fluid walks its cells every step
 constexpr int kMaxListedFluid = (1 << 27) - kPosPad;
 constexpr int kSolidLanes = 16;  // lanes serving one solid particle
+constexpr int kForceRec = 6;     // float4s of a solid particle's force-pass record
 // 16-bit partner entries of the fluid pass (and of the neighbour lists):
 // offset inside a run of slots (kRunBits, high), run-base id (4 bits, low)
 constexpr int kRunBits = 12;
@@ -239,9 +241,11 @@ struct SolidExtra {
     gptr<float4> acc = nullptr;  // du_dt (w unused)
     gptr<float4> couple = nullptr;
     gptr<float> F = nullptr, P = nullptr, L = nullptr, A = nullptr;  // 9 * ns each
-    // P L by rows, (row r of particle i) at PL[3 * i + r] (w unused): the
-    // force pass gathers it per partner, three loads instead of nine
-    gptr<float4> PL = nullptr;
+    // what the force pass reads of a partner, written by the deformation
+    // pass of the same sub-step: frec[kForceRec * i + ...] = reference
+    // position X (w: type), position x, velocity u, then P L by rows (w
+    // unused) -- 96 contiguous bytes instead of six arrays
+    gptr<float4> frec = nullptr;
     // reference-configuration partners of the step: lane l of particle i holds
     // its hits in walk order at nbr[(i * kSolidListCap + k) * kSolidLanes + l],
     // their number (uncapped) at ncnt[i * kSolidLanes + l]

@@ -175,9 +175,10 @@ __device__ __forceinline__ void couple_body(const DevParams &P, const SolidArgs 
 // the overlapped loads gained.)
 enum PartnerMode { kWalkRecord = 0, kReplay = 1 };
 
-template <int MODE, class F>
+// (a partner's reference position X_j is read at Xsrc[XSTRIDE * j])
+template <int MODE, int XSTRIDE, class F>
 __device__ __forceinline__ void solid_partners(const DevParams &P, const SolidArgs &a, int i, int sub,
-                                               const float4 &Xi, F &&body) {
+                                               const float4 &Xi, const float4 *__restrict__ Xsrc, F &&body) {
     int32_t *rec = a.x.nbr + (size_t)i * (kSolidListCap * kSub) + sub;
     int32_t *rec_n = a.x.ncnt + (size_t)i * kSub + sub;
     if (MODE == kReplay && a.solid_rec) {
@@ -185,7 +186,7 @@ __device__ __forceinline__ void solid_partners(const DevParams &P, const SolidAr
         if (cnt <= kSolidListCap) {
             for (int k = 0; k < cnt; ++k) {
                 const int j = rec[k * kSub];
-                const float4 Xj = a.sref[j];
+                const float4 Xj = Xsrc[XSTRIDE * (size_t)j];
                 const float dX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
                 body(j, dX, len2(dX[0], dX[1], dX[2]));
             }
@@ -195,7 +196,7 @@ __device__ __forceinline__ void solid_partners(const DevParams &P, const SolidAr
     int cnt = 0;
     for_each_column(P, a.scell[i], [&](int c0, int c1) {
         for (int j = a.sstart[c0] + sub, e = a.sstart[c1]; j < e; j += kSub) {
-            const float4 Xj = a.sref[j];
+            const float4 Xj = Xsrc[XSTRIDE * (size_t)j];
             const float dX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
             const float d2 = len2(dX[0], dX[1], dX[2]);
             if (d2 < P.R2) {
@@ -222,7 +223,7 @@ __device__ __forceinline__ void deformation_body(const DevParams &P, const Solid
     // first pass stored, the same bits as summing B again.
     const bool reuse_L = MODE == kReplay && a.solid_rec != 0;
     M3 F = mzero(), B = mzero();
-    solid_partners<MODE>(P, a, i, L.sub, Xi, [&](int j, const float (&dX)[3], float d2) {
+    solid_partners<MODE, 1>(P, a, i, L.sub, Xi, a.sref, [&](int j, const float (&dX)[3], float d2) {
         const float4 xj = a.spos[j];
         const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
         const float kc = hw_exp2(fmaf(-d2, P.kexp2, P.log2kd));  // kd exp(-|dX|^2/h^2)
@@ -272,8 +273,12 @@ __device__ __forceinline__ void deformation_body(const DevParams &P, const Solid
     if (!reuse_L) store9(a.x.L, n, i, Lc);
     {
         const M3 PL = mmul(Pk, Lc);
+        float4 *rec = a.x.frec + kForceRec * (size_t)i;
+        rec[0] = Xi;
+        rec[1] = xi;
+        rec[2] = a.svel[i];
 #pragma unroll
-        for (int r = 0; r < 3; ++r) a.x.PL[3 * (size_t)i + r] = make_float4(PL.m[r][0], PL.m[r][1], PL.m[r][2], 0.0f);
+        for (int r = 0; r < 3; ++r) rec[3 + r] = make_float4(PL.m[r][0], PL.m[r][1], PL.m[r][2], 0.0f);
     }
     // det(F) F^-T L, the particle's factor of the artificial-viscosity force
     // (whose pair term is ((det F F^-T visc) L) dX k, solid.cu:156: visc
@@ -299,7 +304,7 @@ __device__ __forceinline__ void force_body(const DevParams &P, const SolidArgs &
     M3 PLi;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-        const float4 row = a.x.PL[3 * (size_t)i + r];
+        const float4 row = a.x.frec[kForceRec * (size_t)i + 3 + r];
         PLi.m[r][0] = row.x;
         PLi.m[r][1] = row.y;
         PLi.m[r][2] = row.z;
@@ -362,19 +367,19 @@ __device__ __forceinline__ void force_body(const DevParams &P, const SolidArgs &
     // P_j L_j dX ks, dX ks and dX visc k over its partners, then applies
     // P_i L_i and det F_i F_i^-T L_i to its two vector sums once.
     float sum_ks[3] = {0.0f, 0.0f, 0.0f}, sum_vk[3] = {0.0f, 0.0f, 0.0f};
-    solid_partners<kReplay>(P, a, i, L.sub, Xi, [&](int j, const float (&dX)[3], float d2) {
+    solid_partners<kReplay, kForceRec>(P, a, i, L.sub, Xi, a.x.frec, [&](int j, const float (&dX)[3], float d2) {
         const float kc = hw_exp2(fmaf(-d2, P.kexp2, P.log2kd));
         const float ks = kc * P.rho_s_inv;
         float v[3];
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-            const float4 row = a.x.PL[3 * (size_t)j + r];
+            const float4 row = a.x.frec[kForceRec * (size_t)j + 3 + r];
             v[r] = (row.x * dX[0] + row.y * dX[1] + row.z * dX[2]) * ks;
         }
         ax += v[0];
         ay += v[1];
         az += v[2];
-        const float4 xj = a.spos[j], uj = a.svel[j];
+        const float4 xj = a.x.frec[kForceRec * (size_t)j + 1], uj = a.x.frec[kForceRec * (size_t)j + 2];
         const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
         const float dudx = (uj.x - ui.x) * dx + (uj.y - ui.y) * dy + (uj.z - ui.z) * dz;
         const float nu = P.h * dudx * hw_rcp(len2(dx, dy, dz) + eps);

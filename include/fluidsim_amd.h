@@ -397,7 +397,7 @@ int fsa_batch_destroy(fsa_batch *batch);
 /* n_steps full steps of every listed context (no context listed twice). */
 int fsa_batch_step(fsa_batch *batch, fsa_sim *const *scenes, int n_scenes, int n_steps);
 /* Number of concurrent launch chains (streams) the contexts are split over,
- * 1..8 (default 2); takes effect on the next fsa_batch_step.  The batch
+ * 1..8 (default 3); takes effect on the next fsa_batch_step.  The batch
  * scheme has no counterpart in the reference (one scene per process). */
 int fsa_batch_set_streams(fsa_batch *batch, int n);
 int fsa_batch_synchronize(fsa_batch *batch);
