@@ -262,7 +262,8 @@ void build_step(const std::vector<fsa_sim *> &sims, const std::vector<Phase> &ph
     for (const FluidJob &j : fluid) blocks.push_back(blocks_fluid(j, fluid_threads));
     add_launch(blob, t, Launch::kFluid, kBFillCounts, blocks);
     for (int sub = 0; sub < smax; ++sub)
-        for (BatchOp op : {sub == 0 ? kBDeformRecord : kBDeform, kBForce, kBSolidUpdate}) {
+        for (BatchOp op : {sub == 0 ? kBDeformRecord : kBDeform, sub == 0 ? kBDeformTailRecord : kBDeformTail,
+                           kBForce, kBSolidUpdate}) {
             blocks.clear();
             for (const SolidJob &j : solid) blocks.push_back(sub < j.P.sub ? blocks_solid(op, j) : 0);
             add_launch(blob, t, Launch::kSolid, op, blocks);

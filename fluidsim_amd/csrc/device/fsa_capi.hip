@@ -319,7 +319,7 @@ int alloc_all(fsa_sim *s) {
         (rc = s->alloc(&s->sx.couple, ns)) ||
         (rc = s->alloc(&s->sx.F, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.P, 9 * (size_t)ns)) ||
         (rc = s->alloc(&s->sx.L, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.frec, kForceRec * (size_t)ns)) ||
-        (rc = s->alloc(&s->sx.A, 9 * (size_t)ns)) ||
+        (rc = s->alloc(&s->sx.A, 9 * (size_t)ns)) || (rc = s->alloc(&s->sx.dsum, 18 * (size_t)ns)) ||
         (rc = s->alloc(&s->sx.nbr, (size_t)ns * kSolidListCap * kSolidLanes)) ||
         (rc = s->alloc(&s->sx.ncnt, (size_t)ns * kSolidLanes)) ||
         (rc = s->alloc(&s->sx.fnbr, (size_t)ns * kSolidListCap * kSolidLanes)) ||

@@ -241,6 +241,8 @@ struct SolidExtra {
     gptr<float4> acc = nullptr;  // du_dt (w unused)
     gptr<float4> couple = nullptr;
     gptr<float> F = nullptr, P = nullptr, L = nullptr, A = nullptr;  // 9 * ns each
+    // batched deformation passes: each particle's pair sums F and B (18 planes of ns)
+    gptr<float> dsum = nullptr;
     // what the force pass reads of a partner, written by the deformation
     // pass of the same sub-step: frec[kForceRec * i + ...] = reference
     // position X (w: type), position x, velocity u, then P L by rows (w
@@ -441,7 +443,8 @@ hipError_t launch_solid_update(const DevParams &P, const SolidArgs &a, hipStream
 // a block's XCD is the same in the batched grid as in a single-scene grid.
 enum BatchOp {
     kBFillCounts, kBHash, kBScanTiles, kBScanSums, kBScanAdd, kBPlace, kBGather,  // CellJob
-    kBCouple, kBDeformRecord, kBDeform, kBForce, kBSolidUpdate                     // SolidJob
+    kBCouple, kBDeformRecord, kBDeform, kBDeformTailRecord, kBDeformTail,         // SolidJob
+    kBForce, kBSolidUpdate
 };
 hipError_t launch_many_cells(BatchOp op, const CellJob *jobs, const int *prefix, int njobs, int nblocks, hipStream_t s);
 int blocks_cells(BatchOp op, const CellJob &j);

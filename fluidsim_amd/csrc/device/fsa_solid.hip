@@ -210,45 +210,13 @@ __device__ __forceinline__ void solid_partners(const DevParams &P, const SolidAr
 }
 
 // ---- K10: solidDeformationCalculate (src/solid.cu:10-79) -----------------
-template <int MODE>
-__device__ __forceinline__ void deformation_body(const DevParams &P, const SolidArgs &a, int blk) {
-    const Lane L = lane_of(blk);
-    if (L.i >= P.ns) return;
-    const int i = L.i;
+// The per-particle part, from the particle's sums F = sum dx dX^T k and B =
+// sum dX dX^T k (or the step's L): the corrected deformation gradient, the
+// stress, and what the force pass reads.
+__device__ __forceinline__ void deformation_tail(const DevParams &P, const SolidArgs &a, int i, M3 F, const M3 &B,
+                                                 bool reuse_L) {
     const float4 Xi = a.sref[i];
     const float4 xi = a.spos[i];
-    // The correction matrix L = B^-1, B = sum dX dX^T k, depends on the
-    // reference positions and the partner order only, both fixed within a
-    // step: a replaying pass (the later sub-steps) reads the L the step's
-    // first pass stored, the same bits as summing B again.
-    const bool reuse_L = MODE == kReplay && a.solid_rec != 0;
-    M3 F = mzero(), B = mzero();
-    solid_partners<MODE, 1>(P, a, i, L.sub, Xi, a.sref, [&](int j, const float (&dX)[3], float d2) {
-        const float4 xj = a.spos[j];
-        const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
-        const float kc = hw_exp2(fmaf(-d2, P.kexp2, P.log2kd));  // kd exp(-|dX|^2/h^2)
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) F.m[r][c] += (dx[r] * dX[c]) * kc;
-        if (!reuse_L) {
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) B.m[r][c] += (dX[r] * dX[c]) * kc;
-        }
-    });
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) F.m[r][c] = group_sum(F.m[r][c]);
-    if (!reuse_L) {
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) B.m[r][c] = group_sum(B.m[r][c]);
-    }
-    if (L.sub != 0) return;
     const M3 Lc = reuse_L ? load9(a.x.L, P.ns, i) : (mdet(B) != 0.0f ? minverse(B) : mident());
     F = mmul(F, Lc);
     // St Venant-Kirchhoff (solidStressCalculation, solid.cu:27-36)
@@ -291,6 +259,68 @@ __device__ __forceinline__ void deformation_body(const DevParams &P, const Solid
 #pragma unroll
         for (int c = 0; c < 3; ++c) A.m[r][c] = dF * Fi.m[c][r];
     store9(a.x.A, n, i, mmul(A, Lc));
+}
+
+// The pair sums, kSub lanes per particle.  SPLIT (batched launches): the
+// group's sums go to dsum (F in planes 0-8, B in 9-17) and OpDeformTail
+// finishes, one thread per particle; otherwise lane 0 finishes here, while
+// the group's other lanes wait.  The same arithmetic either way: the same bits.
+template <int MODE, bool SPLIT>
+__device__ __forceinline__ void deformation_body(const DevParams &P, const SolidArgs &a, int blk) {
+    const Lane L = lane_of(blk);
+    if (L.i >= P.ns) return;
+    const int i = L.i;
+    const float4 Xi = a.sref[i];
+    const float4 xi = a.spos[i];
+    // The correction matrix L = B^-1, B = sum dX dX^T k, depends on the
+    // reference positions and the partner order only, both fixed within a
+    // step: a replaying pass (the later sub-steps) reads the L the step's
+    // first pass stored, the same bits as summing B again.
+    const bool reuse_L = MODE == kReplay && a.solid_rec != 0;
+    M3 F = mzero(), B = mzero();
+    solid_partners<MODE, 1>(P, a, i, L.sub, Xi, a.sref, [&](int j, const float (&dX)[3], float d2) {
+        const float4 xj = a.spos[j];
+        const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+        const float kc = hw_exp2(fmaf(-d2, P.kexp2, P.log2kd));  // kd exp(-|dX|^2/h^2)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) F.m[r][c] += (dx[r] * dX[c]) * kc;
+        if (!reuse_L) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) B.m[r][c] += (dX[r] * dX[c]) * kc;
+        }
+    });
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) F.m[r][c] = group_sum(F.m[r][c]);
+    if (!reuse_L) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) B.m[r][c] = group_sum(B.m[r][c]);
+    }
+    if (L.sub != 0) return;
+    if (SPLIT) {
+        store9(a.x.dsum, P.ns, i, F);
+        if (!reuse_L) store9(a.x.dsum + 9 * (size_t)P.ns, P.ns, i, B);
+        return;
+    }
+    deformation_tail(P, a, i, F, B, reuse_L);
+}
+
+// the per-particle part of a split deformation pass, one thread per particle
+template <int MODE>
+__device__ __forceinline__ void deformation_tail_body(const DevParams &P, const SolidArgs &a, int blk) {
+    const int i = blk * kBlock + threadIdx.x;
+    if (i >= P.ns) return;
+    const bool reuse_L = MODE == kReplay && a.solid_rec != 0;
+    const M3 F = load9(a.x.dsum, P.ns, i);
+    const M3 B = reuse_L ? mzero() : load9(a.x.dsum + 9 * (size_t)P.ns, P.ns, i);
+    deformation_tail(P, a, i, F, B, reuse_L);
 }
 
 // ---- K11: solidForceCalculate (src/solid.cu:81-207) ----------------------
@@ -465,9 +495,13 @@ __device__ __forceinline__ void update_body(const DevParams &P, const SolidArgs 
 struct OpCouple {
     static __device__ void run(const SolidJob &j, int blk, int) { couple_body(j.P, j.a, blk); }
 };
-template <int MODE>
+template <int MODE, bool SPLIT = false>
 struct OpDeform {
-    static __device__ void run(const SolidJob &j, int blk, int) { deformation_body<MODE>(j.P, j.a, blk); }
+    static __device__ void run(const SolidJob &j, int blk, int) { deformation_body<MODE, SPLIT>(j.P, j.a, blk); }
+};
+template <int MODE>
+struct OpDeformTail {
+    static __device__ void run(const SolidJob &j, int blk, int) { deformation_tail_body<MODE>(j.P, j.a, blk); }
 };
 struct OpForce {
     static __device__ void run(const SolidJob &j, int blk, int) { force_body(j.P, j.a, blk); }
@@ -488,7 +522,7 @@ hipError_t one(const DevParams &P, const SolidArgs &a, int blocks, hipStream_t s
 
 int blocks_solid(BatchOp op, const SolidJob &j) {
     if (j.P.ns == 0) return 0;
-    if (op == kBSolidUpdate) return blocks_for(j.P.ns);
+    if (op == kBSolidUpdate || op == kBDeformTailRecord || op == kBDeformTail) return blocks_for(j.P.ns);
     return blocks_for((long long)j.P.ns * kSub);  // couple, deformation, force: kSub lanes per particle
 }
 
@@ -512,10 +546,19 @@ hipError_t launch_many_solid(BatchOp op, const SolidJob *jobs, const int *prefix
     const dim3 g(nblocks), b(kBlock);
     switch (op) {
     case kBCouple: hipLaunchKernelGGL((k_many<OpCouple, SolidJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    // batched deformation passes are split: pair sums, then the per-particle part
     case kBDeformRecord:
-        hipLaunchKernelGGL((k_many<OpDeform<kWalkRecord>, SolidJob>), g, b, 0, s, jobs, prefix, njobs);
+        hipLaunchKernelGGL((k_many<OpDeform<kWalkRecord, true>, SolidJob>), g, b, 0, s, jobs, prefix, njobs);
         break;
-    case kBDeform: hipLaunchKernelGGL((k_many<OpDeform<kReplay>, SolidJob>), g, b, 0, s, jobs, prefix, njobs); break;
+    case kBDeform:
+        hipLaunchKernelGGL((k_many<OpDeform<kReplay, true>, SolidJob>), g, b, 0, s, jobs, prefix, njobs);
+        break;
+    case kBDeformTailRecord:
+        hipLaunchKernelGGL((k_many<OpDeformTail<kWalkRecord>, SolidJob>), g, b, 0, s, jobs, prefix, njobs);
+        break;
+    case kBDeformTail:
+        hipLaunchKernelGGL((k_many<OpDeformTail<kReplay>, SolidJob>), g, b, 0, s, jobs, prefix, njobs);
+        break;
     case kBForce: hipLaunchKernelGGL((k_many<OpForce, SolidJob>), g, b, 0, s, jobs, prefix, njobs); break;
     case kBSolidUpdate: hipLaunchKernelGGL((k_many<OpSolidUpdate, SolidJob>), g, b, 0, s, jobs, prefix, njobs); break;
     default: return hipErrorInvalidValue;

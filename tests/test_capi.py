@@ -81,10 +81,11 @@ def test_footprint_matches_a_hand_count_of_the_buffers():
     position + velocity double buffers (4 x 16 B), id and cell double buffers
     (4 x 4 B) and four sort scratch ints (16 B) = 96 B; per solid particle
     additionally the reference position double buffer (32 B), np/acc/couple
-    (48 B), four 3x3 planes (144 B) and the force pass's partner record
-    (X, x, u and P L as three float4 rows: 96 B), the step's recorded solid
-    and fluid partners (2 x (16 lanes x 16 slots x 4 B + 16 counts x 4 B) =
-    2176 B) and the step-start position (16 B) = 2608 B;
+    (48 B), four 3x3 planes (144 B), the batched deformation pass's pair
+    sums (18 floats: 72 B), the force pass's partner record (X, x, u and P L
+    as three float4 rows: 96 B), the step's recorded solid and fluid
+    partners (2 x (16 lanes x 16 slots x 4 B + 16 counts x 4 B) = 2176 B)
+    and the step-start position (16 B) = 2680 B;
     per wall particle one
     position/velocity buffer (32 B), id and cell double buffers (16 B), sort
     scratch (16 B), normal and constant position (32 B) = 96 B.  (Without
@@ -98,7 +99,7 @@ def test_footprint_matches_a_hand_count_of_the_buffers():
     r2, _ = footprint(sc.params((1000, 150, 2000)))
     r3, p3 = footprint(sc.params((1000, 100, 2400)))
     assert r1 - r0 == 500 * 96
-    assert r2 - r0 == 50 * 2608
+    assert r2 - r0 == 50 * 2680
     assert r3 - r0 == 400 * 96
     assert p0 - r0 == 2000 * 32 and p3 - r3 == 2400 * 32
 

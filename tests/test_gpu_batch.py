@@ -177,7 +177,7 @@ def test_batch_autotune_matches_single_steps(gpu_available):
     with DeviceBatch(autotune=True) as b:
         b.step(mixed, 3)
         st = b.stats()
-        assert st["plans_built"] == 2 and st["fluid_threads"] in (64, 128, 256)  # two chains
+        assert st["plans_built"] == 3 and st["fluid_threads"] in (64, 128, 256)  # three chains (the default)
     for d in alone:
         d.step(3)
     for (n, _), x, y in zip(cases, mixed, alone):
