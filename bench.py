@@ -433,6 +433,25 @@ def precision_bench(steps: int, seed: int, threads: int) -> dict:
     }
 
 
+def c1_gpu_run() -> dict:
+    """The CPU-scale scene's full 500-step run on the GPU, beside the CPU's."""
+    c1 = scenes.c1_cpu_scene(seed=0)
+    a1 = c1.arrays()
+    d1 = DeviceSim(c1.params((len(a1["fluid_position"]), len(a1["solid_position"]), len(a1["virt_position"]))))
+    d1.upload(a1)
+    d1.step(1)
+    sync_all(d1)
+    t0 = time.perf_counter()
+    d1.step(c1.iend)
+    sync_all(d1)
+    c1_s = time.perf_counter() - t0
+    out = {"workload": f"{d1.nf} fluid + {d1.ns} solid + {d1.nv} walls, {c1.iend} steps (after 1 untimed step)",
+           "seconds": round(c1_s, 4), "particle_steps_per_s": (d1.nf + d1.ns) * c1.iend / c1_s}
+    d1.close()
+    log(f"c1 scene on the GPU: {c1.iend} steps in {c1_s:.3f}s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -453,6 +472,7 @@ def main():
                     help="step: one scene's step throughput (the headline line, plus a short batched "
                          "measurement); batch: only the many-scene batched measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched measurement of the step workload")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 500-step run of the 43,750-particle scene")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=30)
     ap.add_argument("--batch-seed", type=int, default=0)
@@ -689,22 +709,8 @@ def main():
         } if lists_on else None,
     }
     dev.close()
-    # the CPU-scale scene's full 500-step run on the GPU, beside the CPU's
-    c1 = scenes.c1_cpu_scene(seed=0)
-    a1 = c1.arrays()
-    d1 = DeviceSim(c1.params((len(a1["fluid_position"]), len(a1["solid_position"]), len(a1["virt_position"]))))
-    d1.upload(a1)
-    d1.step(1)
-    sync_all(d1)
-    t0 = time.perf_counter()
-    d1.step(c1.iend)
-    sync_all(d1)
-    c1_s = time.perf_counter() - t0
-    result["c1_gpu"] = {"workload": f"{d1.nf} fluid + {d1.ns} solid + {d1.nv} walls, {c1.iend} steps "
-                                    "(after 1 untimed step)", "seconds": round(c1_s, 4),
-                        "particle_steps_per_s": (d1.nf + d1.ns) * c1.iend / c1_s}
-    d1.close()
-    log(f"c1 scene on the GPU: {c1.iend} steps in {c1_s:.3f}s")
+    if not args.no_c1:
+        result["c1_gpu"] = c1_gpu_run()
     if not args.no_batch:
         result["batched"] = batch_bench(args.batch_scenes, args.batch_steps, 3, args.batch_seed, args.sim_time,
                                         args.batch_streams, args.batch_advance)
