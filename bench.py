@@ -13,8 +13,9 @@ is the 4,000,000 fluid + 27,000 solid dam break (dp = 1 mm, 3 wall layers,
 h = 1.3 dp, dt = 0.25 h / c0, 4 solid sub-steps).  Inputs are uploaded once
 before the timed region.  The default line also carries the 500-step run of
 the 43,750 + 2,000 particle scene (`c1_gpu`), the 64-scene batched step
-(`batched`), 64 served requests of 0.05 s simulated time each (`serving`,
-p50/p99 time-to-result) and the CPU baseline.  Prints one JSON line on stdout.
+from rest (`batched`) and after 3000 batched steps (`batched_mid_solve`),
+64 served requests of 0.05 s simulated time each (`serving`, p50/p99
+time-to-result) and the CPU baseline.  Prints one JSON line on stdout.
 """
 from __future__ import annotations
 
@@ -479,6 +480,9 @@ def main():
     ap.add_argument("--batch-streams", type=int, default=3, help="concurrent launch chains of the batch (1..8)")
     ap.add_argument("--batch-advance", type=int, default=0,
                     help="untimed batched steps before the batch measurement (time the scenes mid-solve)")
+    ap.add_argument("--batch-mid-advance", type=int, default=3000,
+                    help="step workload: a second batched measurement after this many untimed batched steps "
+                         "(the scenes mid-solve; 0: skip)")
     ap.add_argument("--sim-time", type=float, default=0.5, help="simulated seconds per solve (scenes/hour)")
     ap.add_argument("--requests", type=int, default=64, help="serve workload: number of requests")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement of the step workload")
@@ -714,6 +718,14 @@ def main():
     if not args.no_batch:
         result["batched"] = batch_bench(args.batch_scenes, args.batch_steps, 3, args.batch_seed, args.sim_time,
                                         args.batch_streams, args.batch_advance)
+        if args.batch_mid_advance > 0:
+            # the same family timed mid-solve (columns collapsing onto the
+            # gates, every scene walking its cells), beside the from-rest line
+            m = batch_bench(args.batch_scenes, args.batch_steps, 3, args.batch_seed, args.sim_time,
+                            args.batch_streams, args.batch_mid_advance)
+            result["batched_mid_solve"] = {k: m[k] for k in (
+                "advanced_steps", "steps_timed", "batched_ms_per_step", "sequential_ms_per_step", "speedup",
+                "list_builds", "walking_scene_steps", "scenes_per_hour_batched", "scenes_per_hour_sequential")}
     if not args.no_serve:
         result["serving"] = serve_bench(args.serve_requests, args.batch_seed, args.serve_sim_time, args.load,
                                         args.chunk, args.saturation or None)
