@@ -379,7 +379,7 @@ int fsa_device_memory(int64_t *free_bytes, int64_t *total_bytes);
  * stage calls and downloads on a context stay ordered after the batch step.
  * Per-stage timing and FSA_DEBUG_PAIR_SUMS are not recorded in a batch.
  *
- * The contexts are split over a few launch chains (2 by default), each with
+ * The contexts are split over a few launch chains (3 by default), each with
  * its own stream and recording, so that one chain's latency-bound stages
  * (solid sub-steps, cell build) overlap another chain's fluid pass.  The
  * split is deterministic in the list; results do not depend on it. */
