@@ -129,7 +129,7 @@ __device__ __forceinline__ void wide_window(const DevParams &P, const Columns &c
 // stores coalesced, but its LDS left room for fewer waves to hide the walk's
 // load latency: 3.5 ms per build at 4 M particles against 2.0 ms.)
 constexpr int kBuildThreads = 256;
-constexpr int kBuildGrid = 2048;  // workgroups of one scene's build (grid-stride over its particles)
+constexpr int kBuildGrid = 16384;  // workgroups of one scene's build (grid-stride past 4 M particles)
 
 // The candidates of one kind (positions pos, per-key starts fine) within
 // the list reach of xi, in the walk order of the 5x5 columns, written to
