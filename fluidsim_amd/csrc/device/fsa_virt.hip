@@ -15,10 +15,22 @@ namespace fsa {
 
 namespace {
 
+// A wall particle with no fluid in its 27 cells gets rho0, p = 0 and u = 0,
+// the same values on every update; once they are written (the w of its
+// normal record is then 1) it is left alone until fluid comes into reach.
 template <bool NORMALS>
 __device__ __forceinline__ void virt_body(const DevParams &P, const VirtArgs &a, int blk) {
     const int i = blk * kBlock + threadIdx.x;
     if (i >= P.nv) return;
+    const int cell = a.vcell[i];
+    int cb[9], ce[9];
+    column_ranges(P, cell, a.fstart, cb, ce);
+    bool any = false;
+#pragma unroll
+    for (int col = 0; col < 9; ++col) any |= cb[col] < ce[col];
+    const float4 n4 = NORMALS ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : a.vnrm[i];
+    if (!NORMALS && !any && n4.w != 0.0f) return;
+
     const float4 xi4 = a.vpos_in[i];
     const float xi = xi4.x, yi = xi4.y, zi = xi4.z;
     float ux = 0.0f, uy = 0.0f, uz = 0.0f;
@@ -26,7 +38,6 @@ __device__ __forceinline__ void virt_body(const DevParams &P, const VirtArgs &a,
     float pressure = 0.0f, density = P.rho0, nearest = kLargeFloat;
     int best = -1;
 
-    const int cell = a.vcell[i];
     // nearest fluid particle over the 9 columns in order: strict <, so the
     // first in traversal order wins ties; kScan loads in flight, reads past
     // the run end stay in the padded array and are masked.  Most walls have
@@ -37,11 +48,6 @@ __device__ __forceinline__ void virt_body(const DevParams &P, const VirtArgs &a,
     // not strictly nearer never replaces the one found, so the result is the
     // whole columns' walk.
     {
-        int cb[9], ce[9];
-        column_ranges(P, cell, a.fstart, cb, ce);
-        bool any = false;
-#pragma unroll
-        for (int col = 0; col < 9; ++col) any |= cb[col] < ce[col];
         const Columns cols(P, cell);
         const Window w(P, cols, xi, yi, zi);
         constexpr int kScan = 8;
@@ -102,12 +108,12 @@ __device__ __forceinline__ void virt_body(const DevParams &P, const VirtArgs &a,
             ny *= r;
             nz *= r;
         }
-        a.vnrm[i] = make_float4(nx, ny, nz, 0.0f);
+        a.vnrm[i] = make_float4(nx, ny, nz, any ? 0.0f : 1.0f);
     } else {
-        const float4 n4 = a.vnrm[i];
         nx = n4.x;
         ny = n4.y;
         nz = n4.z;
+        if ((n4.w != 0.0f) == any) reinterpret_cast<float *>(a.vnrm.get() + i)[3] = any ? 0.0f : 1.0f;
     }
     const float m = 2.0f * (ux * nx + uy * ny + uz * nz);
     a.vpos[i] = make_float4(xi, yi, zi, density);
