@@ -462,6 +462,9 @@ def main():
     ap.add_argument("--advance", type=int, default=0,
                     help="step workload: untimed steps before the warm-up, to time a flow in motion "
                          "(the default times the first steps from rest, as the configuration says)")
+    ap.add_argument("--stage-steps", type=int, default=100,
+                    help="step workload: steps after the timed ones with every stage timed (the stage_ms "
+                         "breakdown; the timed steps time the fluid pass only)")
     ap.add_argument("--skin", type=float, default=None,
                     help="neighbour_skin of the step workload's scene (default: the library's; 0: walk every step)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -614,7 +617,9 @@ def main():
     threads = dev.autotune_fluid(3)
     log(f"warmup {args.warmup} steps done; fluid workgroup size {threads} (autotuned)")
 
-    dev.set_timing(True)
+    # the timed steps bracket only the fluid pass with events (on its
+    # stream): events around every stage cost the step ~2.5 %
+    dev.set_timing(True, stages=("fluid",))
     dev.reset_stage_times()
     lists0 = dev.list_stats()
     sync_all(dev)
@@ -622,22 +627,19 @@ def main():
     dev.step(args.steps)
     sync_all(dev)
     t1 = time.perf_counter()
-    stage = dev.stage_times()
+    fluid_ms, fluid_n = dev.stage_times()["fluid"]
     dev.set_timing(False)
     lists1 = dev.list_stats()
     secs = t1 - t0
     ms_per_step = 1e3 * secs / args.steps
     value = (nf + ns) * args.steps / secs
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.4g} particle-steps/s; "
-        f"neighbour lists {lists1}, {lists1['builds'] - lists0['builds']} builds in the timed steps")
-    for k, (ms, n) in stage.items():
-        if n:
-            log(f"  stage {k:6s}: {ms / n:8.4f} ms avg over {n}")
+        f"neighbour lists {lists1}, {lists1['builds'] - lists0['builds']} builds in the timed steps; "
+        f"fluid pass {fluid_ms / max(fluid_n, 1):.4f} ms avg over {fluid_n}")
 
     # roofline of the dominant kernel: the fused fluid pair+update pass
     fl = dev.fluid()
     n_active = active_walls(params, fl["position"], arrays["virt_position"])
-    fluid_ms, fluid_n = stage["fluid"]
     fluid_avg_s = fluid_ms / max(fluid_n, 1) / 1e3
     # the pass is VALU-bound: price it in flops too (8 per candidate test,
     # 90 per in-range Riemann pair) from an exact census of this state
@@ -682,6 +684,22 @@ def main():
         },
     }
 
+    # the stage breakdown: further steps with every stage bracketed
+    dev.set_timing(True)
+    dev.reset_stage_times()
+    sync_all(dev)
+    t2 = time.perf_counter()
+    dev.step(args.stage_steps)
+    sync_all(dev)
+    t3 = time.perf_counter()
+    stage = dev.stage_times()
+    dev.set_timing(False)
+    log(f"stage breakdown over {args.stage_steps} further steps, every stage timed: "
+        f"{1e3 * (t3 - t2) / max(args.stage_steps, 1):.3f} ms/step")
+    for k, (ms, n) in stage.items():
+        if n:
+            log(f"  stage {k:6s}: {ms / n:8.4f} ms avg over {n}")
+
     result = {
         "metric": "particle-steps/s @4M+27k FSI",
         "value": value,
@@ -704,6 +722,7 @@ def main():
             "advanced_steps": args.advance,
         },
         "stage_ms": {k: (ms / n if n else None) for k, (ms, n) in stage.items()},
+        "stage_ms_steps": args.stage_steps,
         "roofline": roofline,
         "neighbour_lists": {
             "skin": round(float(params.neighbour_skin), 4),

@@ -159,6 +159,7 @@ DEVICE_API = {
     "fsa_probe_pressure_noalloc": (C.c_int, [_vp, _f, C.c_int, _f]),
     "fsa_pair_census": (C.c_int, [_vp, _P(C.c_int64)]),
     "fsa_set_timing": (C.c_int, [_vp, C.c_int]),
+    "fsa_set_timing_stages": (C.c_int, [_vp, C.c_uint]),
     "fsa_get_stage_times": (C.c_int, [_vp, _d, _P(C.c_int64)]),
     "fsa_reset_stage_times": (C.c_int, [_vp]),
     "fsa_memory_used": (C.c_int, [_vp, _P(C.c_int64)]),

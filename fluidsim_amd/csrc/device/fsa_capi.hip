@@ -419,7 +419,7 @@ int stage_collect(fsa_sim *s, int st) {
 }
 // a stage's bracket: both events on the stream its work runs on
 int stage_open(fsa_sim *s, int st, hipStream_t on = nullptr) {
-    if (!s->timing) return FSA_OK;
+    if (!((s->timing >> st) & 1u)) return FSA_OK;
     if (s->ev_pending[st] >= fsa_sim::kRing) {
         int rc = stage_collect(s, st);
         if (rc) return rc;
@@ -429,7 +429,7 @@ int stage_open(fsa_sim *s, int st, hipStream_t on = nullptr) {
     return FSA_OK;
 }
 void stage_end(fsa_sim *s, int st, hipStream_t on = nullptr) {
-    if (!s->timing || !s->ev_open[st]) return;
+    if (!((s->timing >> st) & 1u) || !s->ev_open[st]) return;
     if (hipEventRecord(s->ev_end[st][s->ev_head[st]], on ? on : s->stream) != hipSuccess) return;
     s->ev_head[st] = (s->ev_head[st] + 1) % fsa_sim::kRing;
     s->ev_pending[st] += 1;
@@ -1209,19 +1209,25 @@ int fsa_set_debug(fsa_sim *s, unsigned flags) {
     return FSA_OK;
 }
 
-int fsa_set_timing(fsa_sim *s, int enabled) {
+int fsa_set_timing_stages(fsa_sim *s, unsigned stage_mask) {
     CHECK_SIM(s);
+    if (stage_mask >> FSA_STAGE_COUNT) return set_err(FSA_ERR_INVALID, "unknown stage in the timing mask");
     for (int st = 0; st < FSA_STAGE_COUNT; ++st) {
         int rc = stage_collect(s, st);
         if (rc) return rc;
-        if (!enabled) continue;
+        s->ev_open[st] = false;
+        if (!((stage_mask >> st) & 1u)) continue;
         for (int r = 0; r < fsa_sim::kRing; ++r) {
             if (!s->ev_begin[st][r]) HIP_TRY(hipEventCreate(&s->ev_begin[st][r]));
             if (!s->ev_end[st][r]) HIP_TRY(hipEventCreate(&s->ev_end[st][r]));
         }
     }
-    s->timing = enabled != 0;
+    s->timing = stage_mask;
     return FSA_OK;
+}
+
+int fsa_set_timing(fsa_sim *s, int enabled) {
+    return fsa_set_timing_stages(s, enabled ? (1u << FSA_STAGE_COUNT) - 1u : 0u);
 }
 
 int fsa_get_stage_times(fsa_sim *s, double *ms, int64_t *samples) {

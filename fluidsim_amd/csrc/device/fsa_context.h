@@ -73,7 +73,7 @@ struct fsa_sim {
     // stage timing: a ring of event pairs per stage, read back lazily so the
     // host never waits on the GPU while it enqueues steps
     static constexpr int kRing = 256;
-    bool timing = false;
+    unsigned timing = 0;  // stages timed: bit 1u << FSA_STAGE_x
     hipEvent_t ev_begin[FSA_STAGE_COUNT][kRing] = {};
     hipEvent_t ev_end[FSA_STAGE_COUNT][kRing] = {};
     int ev_head[FSA_STAGE_COUNT] = {};     // next ring slot

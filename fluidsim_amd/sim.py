@@ -194,8 +194,16 @@ class DeviceSim:
     def set_debug(self, flags: int):
         check(self.lib.fsa_set_debug(self.handle, flags))
 
-    def set_timing(self, on: bool):
-        check(self.lib.fsa_set_timing(self.handle, 1 if on else 0))
+    def set_timing(self, on: bool, stages=None):
+        """Stage timing: every stage, or only the named ones
+        (e.g. ("fluid",)), whose events cost the step less."""
+        if stages is None:
+            check(self.lib.fsa_set_timing(self.handle, 1 if on else 0))
+            return
+        mask = 0
+        for name in stages:
+            mask |= 1 << _native.STAGES.index(name)
+        check(self.lib.fsa_set_timing_stages(self.handle, mask if on else 0))
 
     def stage_times(self) -> dict:
         ms = (C.c_double * 5)()

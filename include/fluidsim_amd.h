@@ -324,6 +324,12 @@ enum fsa_stage {
 
 /* When enabled, HIP events bracket every stage on the context stream. */
 int fsa_set_timing(fsa_sim *sim, int enabled);
+/* Timing for the stages in stage_mask only (bit 1u << FSA_STAGE_x; 0 = off).
+ * Every timed stage adds two event records per step to its stream, and
+ * each costs the stream a few microseconds; timing one stage keeps a timed
+ * run's step close to an untimed one.  No counterpart in the reference
+ * (a measurement aid, like fsa_set_timing). */
+int fsa_set_timing_stages(fsa_sim *sim, unsigned stage_mask);
 /* Sum of event-measured stage times (ms) and number of samples per stage
  * since the last reset; synchronizes the stream. */
 int fsa_get_stage_times(fsa_sim *sim, double *ms, int64_t *samples);

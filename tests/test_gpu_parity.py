@@ -476,3 +476,11 @@ def test_errors_are_status_codes(gpu_available, tiny):
     dev.step(2)
     t = dev.stage_times()
     assert t["fluid"][1] == 2 and t["fluid"][0] > 0
+    # timing the fluid pass alone: only its samples grow
+    dev.set_timing(True, stages=("fluid",))
+    dev.reset_stage_times()
+    dev.step(3)
+    t = dev.stage_times()
+    assert t["fluid"][1] == 3 and t["fluid"][0] > 0
+    assert all(n == 0 for k, (ms, n) in t.items() if k != "fluid"), t
+    dev.set_timing(False)
