@@ -463,8 +463,8 @@ def main():
                     help="step workload: untimed steps before the warm-up, to time a flow in motion "
                          "(the default times the first steps from rest, as the configuration says)")
     ap.add_argument("--stage-steps", type=int, default=100,
-                    help="step workload: steps after the timed ones with every stage timed (the stage_ms "
-                         "breakdown; the timed steps time the fluid pass only)")
+                    help="step workload: the first steps of the timed run, run again on a fresh upload with "
+                         "every stage timed (the stage_ms breakdown; the timed steps time the fluid pass only)")
     ap.add_argument("--skin", type=float, default=None,
                     help="neighbour_skin of the step workload's scene (default: the library's; 0: walk every step)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -684,17 +684,24 @@ def main():
         },
     }
 
-    # the stage breakdown: further steps with every stage bracketed
+    # the stage breakdown: the same scene uploaded again and stepped from the
+    # same start (warm-up, then the first steps of the timed run) with every
+    # stage bracketed
+    dev.close()
+    dev = DeviceSim(params)
+    dev.upload(arrays)
+    dev.set_fluid_threads(threads)
+    dev.step(args.advance + args.warmup)
+    sync_all(dev)
     dev.set_timing(True)
     dev.reset_stage_times()
-    sync_all(dev)
     t2 = time.perf_counter()
     dev.step(args.stage_steps)
     sync_all(dev)
     t3 = time.perf_counter()
     stage = dev.stage_times()
     dev.set_timing(False)
-    log(f"stage breakdown over {args.stage_steps} further steps, every stage timed: "
+    log(f"stage breakdown over the first {args.stage_steps} of the timed steps again, every stage timed: "
         f"{1e3 * (t3 - t2) / max(args.stage_steps, 1):.3f} ms/step")
     for k, (ms, n) in stage.items():
         if n:
