@@ -484,3 +484,59 @@ def test_errors_are_status_codes(gpu_available, tiny):
     assert t["fluid"][1] == 3 and t["fluid"][0] > 0
     assert all(n == 0 for k, (ms, n) in t.items() if k != "fluid"), t
     dev.set_timing(False)
+
+
+def test_wall_pass_follows_fluid_coming_and_going(gpu_available):
+    """The wall pass leaves a wall with no fluid in its 27 cells alone once its
+    far-field values (rho0, p = 0, u = 0) stand.  With the fluid thrown in all
+    directions walls gain and lose fluid in reach from step to step; after
+    every step's wall pass the walls must be the oracle's wall pass over the
+    device's own fluid state: density bit for bit, velocity and pressure
+    within the tolerances, and walls must change sides both ways."""
+    sc = scenes.tiny_scene(seed=41)
+    a = sc.arrays()
+    rng = np.random.default_rng(41)
+    n = len(a["fluid_position"])
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    speed = rng.uniform(0.0, 0.9 * 0.3 * sc.sound_speed, size=(n, 1))
+    a["fluid_velocity"] = np.ascontiguousarray((d * speed).astype(np.float32))
+    p = sc.params((n, len(a["solid_position"]), len(a["virt_position"])))
+    print(f"case thrown fluid, walls: {p.n_fluid} fluid, {p.n_virt} walls", flush=True)
+    dev = DeviceSim(p)
+    dev.upload(a)
+    rho0 = np.float32(p.fluid_reference_density)
+    prev, gained, lost = None, 0, 0
+    for k in range(24):
+        dev.particle_cell_update()
+        dev.fill_cell()
+        dev.couple_calculation()
+        dev.virt_update()
+        gv = dev.virt()
+        b = dict(a)
+        f = dev.fluid()
+        b["fluid_position"], b["fluid_velocity"] = f["position"], f["velocity"]
+        b["fluid_density"], b["fluid_pressure"] = f["density"], f["pressure"]
+        s = dev.solid()
+        b["solid_position"], b["solid_velocity"] = s["position"], s["velocity"]
+        o = orc.Oracle(p, 32)
+        o.upload(b)
+        o.particle_cell_update()
+        o.fill_cell()
+        o.virt_update()
+        ov = o.virt()
+        o.close()
+        np.testing.assert_array_equal(gv["density"], ov["density"].astype(np.float32), err_msg=f"step {k}")
+        assert max_abs(gv["velocity"], ov["velocity"]) <= REL * velocity_scale(sc, ov["velocity"]), k
+        pmax = max(float(np.abs(ov["pressure"]).max()), 1.0)
+        assert max_abs(gv["pressure"], ov["pressure"]) <= REL * pmax, k
+        far = (gv["density"] == rho0) & np.all(gv["velocity"] == 0.0, axis=1) & (gv["pressure"] == 0.0)
+        if prev is not None:
+            gained += int(np.sum(prev & ~far))
+            lost += int(np.sum(~prev & far))
+        prev = far
+        dev.fluid_calculation()
+        dev.solid_calculation()
+    print(f"walls that gained fluid in reach {gained}, lost it {lost}", flush=True)
+    assert gained > 0 and lost > 0
+    dev.close()
