@@ -687,25 +687,27 @@ def main():
     # the stage breakdown: the same scene uploaded again and stepped from the
     # same start (warm-up, then the first steps of the timed run) with every
     # stage bracketed
-    dev.close()
-    dev = DeviceSim(params)
-    dev.upload(arrays)
-    dev.set_fluid_threads(threads)
-    dev.step(args.advance + args.warmup)
-    sync_all(dev)
-    dev.set_timing(True)
-    dev.reset_stage_times()
-    t2 = time.perf_counter()
-    dev.step(args.stage_steps)
-    sync_all(dev)
-    t3 = time.perf_counter()
-    stage = dev.stage_times()
-    dev.set_timing(False)
-    log(f"stage breakdown over the first {args.stage_steps} of the timed steps again, every stage timed: "
-        f"{1e3 * (t3 - t2) / max(args.stage_steps, 1):.3f} ms/step")
-    for k, (ms, n) in stage.items():
-        if n:
-            log(f"  stage {k:6s}: {ms / n:8.4f} ms avg over {n}")
+    stage = {}
+    if args.stage_steps > 0:
+        dev.close()
+        dev = DeviceSim(params)
+        dev.upload(arrays)
+        dev.set_fluid_threads(threads)
+        dev.step(args.advance + args.warmup)
+        sync_all(dev)
+        dev.set_timing(True)
+        dev.reset_stage_times()
+        t2 = time.perf_counter()
+        dev.step(args.stage_steps)
+        sync_all(dev)
+        t3 = time.perf_counter()
+        stage = dev.stage_times()
+        dev.set_timing(False)
+        log(f"stage breakdown over the first {args.stage_steps} of the timed steps again, every stage timed: "
+            f"{1e3 * (t3 - t2) / args.stage_steps:.3f} ms/step")
+        for k, (ms, n) in stage.items():
+            if n:
+                log(f"  stage {k:6s}: {ms / n:8.4f} ms avg over {n}")
 
     result = {
         "metric": "particle-steps/s @4M+27k FSI",
