@@ -191,6 +191,11 @@ constexpr int kBaseSlots = 12;  // run bases per lane (4-bit ids): the columns, 
 constexpr int kSentinelSlot = kBaseSlots - 1;  // holds the kind's particle count: the first padding slot
 constexpr int kSpareSlots = kSentinelSlot - kColumns;
 constexpr int kPairBatch = 4;   // partners per round of the pair loop (the next round is gathered meanwhile)
+// listed candidates per round of the list pass (4, 5, 6, 7 and 8 measured:
+// 6 keeps the most partner bytes in flight without running past the rounds
+// the waves can hide; 0.957 ms against 1.009 at 4 per round, 4 M particles)
+constexpr int kListBatch = 6;
+static_assert(3 * kListBatch - 1 <= kListAhead, "the list pass reads up to 3 kListBatch - 2 rows past a list");
 constexpr int kScanBatch = 8;   // candidate loads in flight in the test loop
 constexpr uint16_t kSentinelEntry = kSentinelSlot;  // offset 0 of the sentinel's run
 static_assert(kPosPad >= kScanBatch, "candidate batches may read kScanBatch-1 entries past a range");
@@ -454,32 +459,32 @@ __device__ __forceinline__ void list_pass(const DevParams &P, const Self &me, in
     if (__ballot(cnt > 0) == 0) return;
     const size_t nf = (size_t)P.nf;
     const uint32_t *__restrict__ ent = list + i;
-    uint32_t en[kPairBatch];
-    float4 pc[kPairBatch], vc[kPairBatch];
+    uint32_t en[kListBatch];
+    float4 pc[kListBatch], vc[kListBatch];
 #pragma unroll
-    for (int q = 0; q < kPairBatch; ++q) {
+    for (int q = 0; q < kListBatch; ++q) {
         const uint32_t e = ent[(size_t)q * nf];
-        en[q] = ent[(size_t)(kPairBatch + q) * nf];
+        en[q] = ent[(size_t)(kListBatch + q) * nf];
         const uint32_t off = q < cnt ? e : sentinel;
         pc[q] = at(pos, off);
         vc[q] = at(vel, off);
     }
     int hits = 0;
-    for (int k = 0; __ballot(k < cnt) != 0; k += kPairBatch) {
-        const int kn = k + kPairBatch;
-        float4 pn[kPairBatch], vn[kPairBatch];
+    for (int k = 0; __ballot(k < cnt) != 0; k += kListBatch) {
+        const int kn = k + kListBatch;
+        float4 pn[kListBatch], vn[kListBatch];
 #pragma unroll
-        for (int q = 0; q < kPairBatch; ++q) {
+        for (int q = 0; q < kListBatch; ++q) {
             const uint32_t off = kn + q < cnt ? en[q] : sentinel;
             pn[q] = at(pos, off);
             vn[q] = at(vel, off);
         }
 #pragma unroll
-        for (int q = 0; q < kPairBatch; ++q) en[q] = ent[(size_t)(kn + kPairBatch + q) * nf];
+        for (int q = 0; q < kListBatch; ++q) en[q] = ent[(size_t)(kn + kListBatch + q) * nf];
 #pragma unroll
-        for (int q = 0; q < kPairBatch; ++q) listed_pair<KIND>(P, me, pc[q], vc[q], acc, hits);
+        for (int q = 0; q < kListBatch; ++q) listed_pair<KIND>(P, me, pc[q], vc[q], acc, hits);
 #pragma unroll
-        for (int q = 0; q < kPairBatch; ++q) {
+        for (int q = 0; q < kListBatch; ++q) {
             pc[q] = pn[q];
             vc[q] = vn[q];
         }

@@ -194,7 +194,7 @@ struct KindBuffers {
 // in the walk order of its build (the 25 columns of the 5x5 cells around the
 // particle's, x-major, each column's z-window in slot order), each as the
 // byte offset of its home slot in the home state arrays.
-constexpr int kListAhead = 12;     // rows past the longest list the fluid pass may read (and discard)
+constexpr int kListAhead = 18;     // rows past the longest list the fluid pass may read (and discard)
 constexpr int kMoveWords = 1024;   // words the fluid pass's wavefronts spread their largest moves over
 constexpr int kListRowsMax = 256;  // candidates a list holds at most
 enum ListFlag {

@@ -107,8 +107,8 @@ def test_footprint_matches_a_hand_count_of_the_buffers():
 def test_footprint_counts_the_neighbour_lists():
     """With neighbour lists a fluid particle also holds the home-order
     position + velocity double buffers (64 B), home index double buffer and
-    sorted slot (12 B), its list -- (rows + 12) 32-bit entries and the
-    entry count (4 B) -- its wall list -- (wall rows + 12) entries and a
+    sorted slot (12 B), its list -- (rows + 18) 32-bit entries and the
+    entry count (4 B) -- its wall list -- (wall rows + 18) entries and a
     count -- and its build position (16 B); the lists' few words of state
     do not grow with the particles.  rows is 1.5 x the lattice sites within
     the list reach (1 + skin) R, rounded up to a multiple of 8, at most 256;
@@ -124,7 +124,7 @@ def test_footprint_counts_the_neighbour_lists():
     reach = (1 + skin) * math.sqrt(p0.impact_distance_square) / p0.particle_diameter
     rows = min(max(math.ceil(1.5 * 4 / 3 * math.pi * reach ** 3 / 8) * 8, 32), 256)
     wrows = max(rows // 2 // 8 * 8, 32)
-    per = 96 + 64 + 12 + 4 * (rows + 12) + 4 + 4 * (wrows + 12) + 4 + 16
+    per = 96 + 64 + 12 + 4 * (rows + 18) + 4 + 4 * (wrows + 18) + 4 + 16
     assert r1 - r0 == 640 * per
     # without a skin, or with the packed form, there are no lists
     assert footprint(sc.with_(neighbour_skin=0.0).params((1000, 100, 2000)))[0] < r0
