@@ -254,13 +254,26 @@ struct OpGatherImpl {
         if (t >= j.n) return;
         const int s = j.tmp_slot[t];
         const int my = j.tmp_id[t];
+        // what depends on the slot alone is loaded before the key's range and
+        // the rank search, so those loads wait on tmp_slot only
         const int k = j.key[s];
+        const float4 x = j.pos_in[s], u = j.vel_in[s];
+        const int h = j.hidx_out ? j.hidx_in[s] : 0;
         const int c = k / j.nsub;
         const int b = j.fstart[k], e = j.fstart[k + 1];
+        // rank of `my` among the key's ids (t lies in [b, e)), four ids per
+        // round with independent loads; positions past the range re-read its
+        // last id and are not counted
         int r = 0;
-        for (int u = b; u < e; ++u) r += j.tmp_id[u] < my ? 1 : 0;
+        for (int v = b; v < e; v += 4) {
+            const int i0 = j.tmp_id[v];
+            const int i1 = j.tmp_id[min(v + 1, e - 1)];
+            const int i2 = j.tmp_id[min(v + 2, e - 1)];
+            const int i3 = j.tmp_id[min(v + 3, e - 1)];
+            r += (int)(i0 < my) + (int)((v + 1 < e) & (i1 < my)) + (int)((v + 2 < e) & (i2 < my)) +
+                 (int)((v + 3 < e) & (i3 < my));
+        }
         const int d = b + r;
-        const float4 x = j.pos_in[s], u = j.vel_in[s];
         j.pos_out[d] = x;
         j.vel_out[d] = u;
         if (j.has_ref) j.ref_out[d] = j.ref_in[s];
@@ -272,7 +285,6 @@ struct OpGatherImpl {
         j.id_out[d] = my;
         j.cell_out[d] = c;
         if (j.hidx_out) {
-            const int h = j.hidx_in[s];
             j.hidx_out[d] = h;
             j.hslot[h] = d;
         }

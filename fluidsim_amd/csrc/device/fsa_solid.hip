@@ -184,8 +184,11 @@ __device__ __forceinline__ void solid_partners(const DevParams &P, const SolidAr
     if (MODE == kReplay && a.solid_rec) {
         const int cnt = *rec_n;
         if (cnt <= kSolidListCap) {
+            // the next partner's index is loaded while this one is evaluated
+            int jn = cnt > 0 ? rec[0] : 0;
             for (int k = 0; k < cnt; ++k) {
-                const int j = rec[k * kSub];
+                const int j = jn;
+                if (k + 1 < cnt) jn = rec[(k + 1) * kSub];
                 const float4 Xj = Xsrc[XSTRIDE * (size_t)j];
                 const float dX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
                 body(j, dX, len2(dX[0], dX[1], dX[2]));
